@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Throughput of the fused FiVoNAGI time step on one MI355X.
+
+Workload: the HIFU-beam-like single instance -- 4096x1024 FP32 grid, quiescent
+medium driven from the left edge (rho' = A g(j) sin(omega T), A = 0.02,
+Gaussian aperture of half-width 200 cells, 20 cells per wavelength),
+beta = 3.5, delta = 5e-5, CFL 0.9, zero-gradient on the other edges.
+A "step" is one step attempt of the whole grid (one launch of the fused
+kernel); value = accepted cell-updates per second over the timed steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  Timing uses HIP events recorded on the stream the step
+kernel runs on (created here through the HIP runtime, handed to the library).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s + % HBM peak (fwd FP32); batched fwd+adjoint gradients/s vs plain"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+BYTES_PER_CELL = 24            # read q (3 x 4 B) + write q (3 x 4 B)
+KERNEL = "fv_step_kernel"
+
+
+class Hip:
+    """Minimal HIP runtime calls through ctypes: streams and events."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        for n in ("hipStreamCreate", "hipEventCreate", "hipEventRecord", "hipEventSynchronize",
+                  "hipEventElapsedTime", "hipStreamSynchronize", "hipEventDestroy",
+                  "hipStreamDestroy", "hipGetDeviceCount"):
+            getattr(self.lib, n).restype = C.c_int
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with hipError {rc}")
+
+    def device_count(self):
+        n = C.c_int()
+        self.check(self.lib.hipGetDeviceCount(C.byref(n)), "hipGetDeviceCount")
+        return n.value
+
+    def stream(self):
+        s = C.c_void_p()
+        self.check(self.lib.hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        return s
+
+    def event(self):
+        e = C.c_void_p()
+        self.check(self.lib.hipEventCreate(C.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e, s):
+        self.check(self.lib.hipEventRecord(e, s), "hipEventRecord")
+
+    def elapsed_ms(self, e0, e1):
+        self.check(self.lib.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float()
+        self.check(self.lib.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        return ms.value
+
+
+def traffic_from_profiles(workload_tag):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    recorded under profiles/ (see DESIGN.md); None if not measured."""
+    path = os.path.join(ROOT, "profiles", f"{KERNEL}_{workload_tag}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(prob, seconds=12.0):
+    """The FP32 build of the CPU oracle (a port of the reference step), on this
+    host's cores, over a bounded number of steps of the same grid."""
+    from oracle import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    op = po.Problem(prob.nx, prob.ny, prob.dx, prob.dy, prob.beta, prob.delta, prob.cfl_max,
+                    prob.cfl_wish, prob.dt0, prob.T0, bc=prob.bc, cfl_mode=prob.cfl_mode,
+                    form=po.FORM_RHO1, omega=prob.omega, edge_amp=prob.edge_amp,
+                    nthreads=threads)
+    q = prob.q0.astype(np.float32)
+    # calibrate on one step, then time one bounded run of the same grid
+    t0 = time.perf_counter()
+    q, st, rc = po.run(op, q, 1, dtype=np.float32, fast=True)
+    per_step = max(time.perf_counter() - t0, 1e-3)
+    steps = max(2, int(seconds / per_step))
+    t0 = time.perf_counter()
+    q, st, rc = po.run(op, q, steps, dtype=np.float32, state=st, fast=True)
+    el = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"CPU oracle failed: {rc}")
+    value = prob.nx * prob.ny * steps / el / 1e9
+    return {"value": value, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} accepted steps of the same {prob.nx}x{prob.ny} grid from the "
+                      f"quiescent start, FP32 CPU oracle (-O3, OpenMP over rows), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--rows-per-wave", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step "
+              "shards by instances or by y-slabs (DESIGN.md) but no multi-GPU path is built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import fivonagi_amd as fa
+    from fivonagi_amd import workloads as wl
+
+    hip = Hip()
+    if hip.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    prob = wl.edge_drive(args.nx, args.ny, rows_per_wave=args.rows_per_wave)
+    stream = hip.stream()
+    ctx = fa.Context(prob)
+    ctx.set_stream(stream.value)
+    info0 = ctx.info(0)
+
+    ctx.enqueue(args.warmup)
+    rc = ctx.sync()
+    if rc != 0:
+        raise RuntimeError(f"warmup failed: {rc}")
+    before = ctx.info(0)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    ctx.enqueue(args.steps)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1)
+    rc = ctx.sync()
+    if rc != 0:
+        raise RuntimeError(f"timed steps failed: {rc}")
+    after = ctx.info(0)
+    accepted = after["n_accepted"] - before["n_accepted"]
+    attempts = after["n_attempts"] - before["n_attempts"]
+    cells = prob.nx * prob.ny
+    secs = ms / 1e3
+    value = cells * accepted / secs / 1e9
+    ms_per_step = ms / args.steps
+    # dominant (only) kernel: one launch per attempt
+    achieved = BYTES_PER_CELL * cells / (ms_per_step / 1e3) / 1e9
+    tag = f"edge_drive_{prob.nx}x{prob.ny}"
+    traffic = traffic_from_profiles(tag)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"edge_drive {prob.nx}x{prob.ny} fp32 forward (A=0.02, aperture "
+                        f"half-width 200 cells, 20 cells/wavelength, beta=3.5, delta=5e-5, CFL 0.9)",
+            "accepted_steps": accepted, "attempts": attempts,
+            "rows_per_wave": after["rows_per_wave"], "waves_per_step": after["n_items"],
+            "T_start": before["T"], "T_end": after["T"],
+        },
+        "hbm_peak_frac": round(achieved / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": KERNEL,
+            "bytes_per_launch": BYTES_PER_CELL * cells,
+            "avg_launch_ms": round(ms_per_step, 5),
+        },
+        "cpu_baseline": None,
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

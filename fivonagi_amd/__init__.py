@@ -1,0 +1,217 @@
+"""fivonagi_amd -- MI355X-native FiVoNAGI time step.
+
+Python view of the C ABI in include/fivonagi.h (libfivonagi.so, built by
+fivonagi_amd.build).  Loading fails loudly when the library is missing: there
+is no CPU fallback for the step.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfivonagi.so")
+
+BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR = 0, 1, 2
+CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
+OK, ERR_EIGEN, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 8, 9, 10
+
+EXPORTS = (
+    "fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence", "fivonagi_step",
+    "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect", "fivonagi_info",
+    "fivonagi_destroy", "fivonagi_last_error",
+)
+
+
+class Params(C.Structure):
+    """fivonagi_params (include/fivonagi.h)."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("beta", C.c_double), ("delta", C.c_double),
+        ("cfl_max", C.c_double), ("cfl_wish", C.c_double),
+        ("dt0", C.c_double), ("T0", C.c_double),
+        ("bc", C.c_int32), ("cfl_mode", C.c_int32),
+        ("omega", C.c_double),
+        ("edge_amp", C.POINTER(C.c_double)), ("edge_phase", C.POINTER(C.c_double)),
+        ("tay_rhoa", C.c_double), ("tay_c", C.c_double),
+        ("tay_theta", C.c_double), ("tay_yoff", C.c_double),
+        ("tay_jc", C.c_int32), ("rows_per_wave", C.c_int32),
+    ]
+
+
+class Info(C.Structure):
+    """fivonagi_info_t (include/fivonagi.h)."""
+    _fields_ = [
+        ("T", C.c_double), ("dt", C.c_double), ("last_cfl", C.c_double),
+        ("n_accepted", C.c_int64), ("n_rejected", C.c_int64), ("n_attempts", C.c_int64),
+        ("error", C.c_int32), ("cur", C.c_int32),
+        ("n_items", C.c_int32), ("rows_per_wave", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load libfivonagi.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m fivonagi_amd.build` "
+            "(the step has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER(Params)
+    vp = C.c_void_p
+    lib.fivonagi_init.argtypes = [C.POINTER(vp), P, C.c_int, C.POINTER(C.POINTER(C.c_float))]
+    lib.fivonagi_set_stream.argtypes = [vp, vp]
+    lib.fivonagi_set_dt_sequence.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_int64]
+    lib.fivonagi_step.argtypes = [vp, C.c_int64]
+    lib.fivonagi_enqueue.argtypes = [vp, C.c_int64]
+    lib.fivonagi_sync.argtypes = [vp]
+    fp = C.POINTER(C.c_float)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int64)
+    lib.fivonagi_collect.argtypes = [vp, C.c_int, fp, fp, fp, dp, dp, ip, ip]
+    lib.fivonagi_info.argtypes = [vp, C.c_int, C.POINTER(Info)]
+    lib.fivonagi_destroy.argtypes = [vp]
+    lib.fivonagi_destroy.restype = None
+    lib.fivonagi_last_error.argtypes = []
+    lib.fivonagi_last_error.restype = C.c_char_p
+    for name in ("fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
+                 "fivonagi_step", "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect",
+                 "fivonagi_info"):
+        getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class FivonagiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fivonagi error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc):
+    if rc != 0:
+        raise FivonagiError(rc, load().fivonagi_last_error().decode(errors="replace"))
+    return rc
+
+
+class Problem:
+    """One grid instance: runtime form of an app's parameters.h + init.h.
+
+    q0 has shape (3, ny, nx): rho', rho*u, rho*v (float32 on upload)."""
+
+    def __init__(self, nx, ny, dx, dy, beta, delta, q0, cfl_max=1.0, cfl_wish=0.9,
+                 dt0=0.005, T0=0.0, bc=BC_ZEROGRAD, cfl_mode=CFL_REFERENCE_FOLD,
+                 omega=0.0, edge_amp=None, edge_phase=None, taylor=None, rows_per_wave=0):
+        self.nx, self.ny = int(nx), int(ny)
+        self.dx, self.dy, self.beta, self.delta = float(dx), float(dy), float(beta), float(delta)
+        self.cfl_max, self.cfl_wish, self.dt0, self.T0 = map(float, (cfl_max, cfl_wish, dt0, T0))
+        self.bc, self.cfl_mode, self.omega = int(bc), int(cfl_mode), float(omega)
+        self.edge_amp = None if edge_amp is None else np.ascontiguousarray(edge_amp, np.float64)
+        self.edge_phase = None if edge_phase is None else np.ascontiguousarray(edge_phase, np.float64)
+        self.taylor = taylor
+        self.rows_per_wave = int(rows_per_wave)
+        self.q0 = np.ascontiguousarray(q0, np.float32)
+        assert self.q0.shape == (3, self.ny, self.nx), self.q0.shape
+
+    def cparams(self):
+        p = Params()
+        p.nx, p.ny, p.dx, p.dy = self.nx, self.ny, self.dx, self.dy
+        p.beta, p.delta, p.cfl_max, p.cfl_wish = self.beta, self.delta, self.cfl_max, self.cfl_wish
+        p.dt0, p.T0, p.bc, p.cfl_mode, p.omega = self.dt0, self.T0, self.bc, self.cfl_mode, self.omega
+        if self.edge_amp is not None:
+            p.edge_amp = self.edge_amp.ctypes.data_as(C.POINTER(C.c_double))
+        if self.edge_phase is not None:
+            p.edge_phase = self.edge_phase.ctypes.data_as(C.POINTER(C.c_double))
+        if self.taylor:
+            t = self.taylor
+            p.tay_rhoa, p.tay_c, p.tay_theta = t["rhoa"], t["c"], t["theta"]
+            p.tay_yoff, p.tay_jc = t["yoff"], int(t["jc"])
+        p.rows_per_wave = self.rows_per_wave
+        return p
+
+
+class Context:
+    """A fivonagi_ctx: one or more instances advanced together on one stream."""
+
+    def __init__(self, problems):
+        if isinstance(problems, Problem):
+            problems = [problems]
+        self.problems = list(problems)
+        lib = load()
+        n = len(self.problems)
+        parr = (Params * n)(*[p.cparams() for p in self.problems])
+        qs = []
+        for p in self.problems:
+            qs += [np.ascontiguousarray(p.q0[k]) for k in range(3)]
+        self._keep = qs
+        qptr = (C.POINTER(C.c_float) * (3 * n))(
+            *[q.ctypes.data_as(C.POINTER(C.c_float)) for q in qs])
+        h = C.c_void_p()
+        _check(lib.fivonagi_init(C.byref(h), parr, n, qptr))
+        self._h = h
+        self._dtseq = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fivonagi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_stream(self, stream_ptr):
+        _check(load().fivonagi_set_stream(self._h, C.c_void_p(stream_ptr)))
+
+    def set_dt_sequence(self, inst, dts):
+        if dts is None:
+            _check(load().fivonagi_set_dt_sequence(self._h, inst, None, 0))
+            return
+        a = np.ascontiguousarray(dts, np.float64)
+        self._dtseq[inst] = a
+        _check(load().fivonagi_set_dt_sequence(
+            self._h, inst, a.ctypes.data_as(C.POINTER(C.c_double)), len(a)))
+
+    def step(self, nsteps):
+        return _check(load().fivonagi_step(self._h, int(nsteps)))
+
+    def step_rc(self, nsteps):
+        """fivonagi_step returning the raw code instead of raising."""
+        return load().fivonagi_step(self._h, int(nsteps))
+
+    def enqueue(self, nattempts):
+        return _check(load().fivonagi_enqueue(self._h, int(nattempts)))
+
+    def sync(self):
+        return load().fivonagi_sync(self._h)
+
+    def collect(self, inst=0):
+        p = self.problems[inst]
+        q = np.zeros((3, p.ny, p.nx), np.float32)
+        T, dt = C.c_double(), C.c_double()
+        na, nr = C.c_int64(), C.c_int64()
+        fpp = [np.ascontiguousarray(q[k]) for k in range(3)]
+        _check(load().fivonagi_collect(self._h, inst,
+                                       *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in fpp],
+                                       C.byref(T), C.byref(dt), C.byref(na), C.byref(nr)))
+        return np.stack(fpp), dict(T=T.value, dt=dt.value, n_accepted=na.value,
+                                   n_rejected=nr.value)
+
+    def info(self, inst=0):
+        i = Info()
+        _check(load().fivonagi_info(self._h, inst, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in Info._fields_}

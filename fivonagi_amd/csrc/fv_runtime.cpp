@@ -1,0 +1,474 @@
+// fv_runtime.cpp -- host side of the C ABI (include/fivonagi.h): device
+// memory, instance tables, work lists, hipGraph replay of step launches.
+//
+// The reference's host loop (common/main.cu:310-635) copies dt and simError
+// to the host up to seven times per step; here the step kernel keeps dt, T,
+// the accepted/rejected counters and the buffer index on the device, and the
+// host only synchronises once per fivonagi_step call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fivonagi.h"
+#include "fv_types.h"
+
+extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
+                                     int n_items, hipStream_t stream);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return -(int)e;
+}
+
+#define HIP_TRY(expr)                               \
+  do {                                              \
+    hipError_t e_ = (expr);                         \
+    if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+  } while (0)
+
+// Cells whose cfl reaches cfl[0] under the reference's fold
+// (common/main.cu:503-524): follow each cell's value forward through the
+// launches.  A launch of S threads moves slot p to p - S when S <= p < 2S and
+// strands slots >= 2S.
+std::vector<uint8_t> fold_subset(int nx, int ny) {
+  const long N = (long)nx * ny;
+  std::vector<long> strides;
+  long sizered = N;
+  while (sizered > 32) { sizered /= 2; strides.push_back((sizered / 16) * 16); }
+  while (sizered > 2) { sizered /= 2; strides.push_back(sizered); }
+  std::vector<uint8_t> m(N, 0);
+  for (long c = 0; c < N; ++c) {
+    long p = c;
+    bool alive = true;
+    for (long S : strides) {
+      if (p >= S && p < 2 * S) p -= S;
+      else if (p >= 2 * S) { alive = false; break; }
+    }
+    m[c] = (alive && p == 0) ? 1 : 0;
+  }
+  return m;
+}
+
+}  // namespace
+
+struct fivonagi_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int n_inst = 0;
+  std::vector<fivonagi_params> params;
+  std::vector<fv::InstDesc> desc;          // host copy of the instance table
+  std::vector<fv::WorkItem> items;
+  std::vector<std::vector<double>> dt_seq;
+  std::vector<void*> allocs;               // every device allocation
+  fv::InstDesc* d_desc = nullptr;
+  fv::WorkItem* d_items = nullptr;
+  fv::Ctl* d_ctl = nullptr;
+  int graph_len = 0;
+  bool unlimited = false;                  // targets set to "run every attempt"
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
+  void* dalloc(size_t bytes, hipError_t* e) {
+    void* p = nullptr;
+    *e = hipMalloc(&p, bytes);
+    if (*e == hipSuccess) allocs.push_back(p);
+    return p;
+  }
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_len = 0;
+  }
+  ~fivonagi_ctx() {
+    drop_graph();
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) (void)hipFree(p);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+  }
+};
+
+namespace {
+
+constexpr int kTargetWaves = 2048;   // ~2 waves per SIMD on a 256-CU part
+
+// Rows per work item: the smallest height whose total wave count over all
+// instances fits kTargetWaves (fewer overlapping halo rows), never below 16.
+int auto_rows(const std::vector<fivonagi_params>& ps) {
+  for (int H = 16; H <= 4096; ++H) {
+    long waves = 0;
+    for (const auto& p : ps) {
+      const long strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
+      waves += strips * std::max(1L, std::lround((double)p.ny / H));
+    }
+    if (waves <= kTargetWaves) return H;
+  }
+  return 4096;
+}
+
+void build_items(fivonagi_ctx* c, int auto_h) {
+  c->items.clear();
+  for (int k = 0; k < c->n_inst; ++k) {
+    const auto& p = c->params[k];
+    const int H = p.rows_per_wave > 0 ? p.rows_per_wave : auto_h;
+    const int chunks = std::max(1, (int)std::lround((double)p.ny / H));
+    const int strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
+    int n = 0;
+    for (int s = 0; s < strips; ++s) {
+      for (int ch = 0; ch < chunks; ++ch) {
+        fv::WorkItem w;
+        w.inst = k;
+        w.x0 = s * fv::kStripCols;
+        w.y0 = (int)((long)p.ny * ch / chunks);
+        w.y1 = (int)((long)p.ny * (ch + 1) / chunks);
+        c->items.push_back(w);
+        ++n;
+      }
+    }
+    c->desc[k].n_items = n;
+    c->desc[k].first_item = (int)c->items.size() - n;
+  }
+}
+
+int upload_tables(fivonagi_ctx* c) {
+  HIP_TRY(hipMemcpyAsync(c->d_desc, c->desc.data(), sizeof(fv::InstDesc) * c->n_inst,
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+void set_ctl_derived(fv::Ctl& t, const fivonagi_params& p) {
+  t.dtx = (float)(t.dt / p.dx);
+  t.dty = (float)(t.dt / p.dy);
+  t.ddx2 = (float)(p.delta * t.dt / (p.dx * p.dx));
+  t.ddy2 = (float)(p.delta * t.dt / (p.dy * p.dy));
+  const double th = std::fmod(p.omega * t.T, 6.283185307179586476925286766559);
+  t.drive_s = std::sin(th);
+  t.drive_c = std::cos(th);
+}
+
+int read_ctls(fivonagi_ctx* c, std::vector<fv::Ctl>& out) {
+  out.resize(c->n_inst);
+  HIP_TRY(hipMemcpyAsync(out.data(), c->d_ctl, sizeof(fv::Ctl) * c->n_inst,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int launch_attempts(fivonagi_ctx* c, int64_t n) {
+  const int G = 32;   // launches per captured graph
+  if (n >= G) {
+    if (!c->graph_exec) {
+      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < G; ++k) {
+        hipError_t e = fv_launch_step(c->d_desc, c->d_items, (int)c->items.size(), c->stream);
+        if (e != hipSuccess) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(c->stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          return hip_fail(e, "fv_launch_step (capture)");
+        }
+      }
+      HIP_TRY(hipStreamEndCapture(c->stream, &c->graph));
+      HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+      c->graph_len = G;
+    }
+    while (n >= G) {
+      HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+      n -= G;
+    }
+  }
+  for (int64_t k = 0; k < n; ++k) {
+    hipError_t e = fv_launch_step(c->d_desc, c->d_items, (int)c->items.size(), c->stream);
+    if (e != hipSuccess) return hip_fail(e, "fv_launch_step");
+  }
+  return 0;
+}
+
+int check_ctx(fivonagi_ctx* c, int inst) {
+  if (!c) return fail(FIVONAGI_ERR_ARG, "null context");
+  if (inst < 0 || inst >= c->n_inst) return fail(FIVONAGI_ERR_ARG, "instance out of range");
+  return 0;
+}
+
+int write_targets(fivonagi_ctx* c, const std::vector<int64_t>& tgt) {
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  for (int k = 0; k < c->n_inst; ++k) ctl[k].target_acc = tgt[k];
+  HIP_TRY(hipMemcpyAsync(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * c->n_inst,
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fivonagi_last_error(void) { return g_err.c_str(); }
+
+int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
+                  const float* const* q_host) {
+  if (!out || !p || !q_host || n_inst <= 0) return fail(FIVONAGI_ERR_ARG, "null argument");
+  *out = nullptr;
+  for (int k = 0; k < n_inst; ++k) {
+    const auto& q = p[k];
+    if (q.nx < 8 || q.ny < 8) return fail(FIVONAGI_ERR_ARG, "nx, ny must be >= 8");
+    if ((long)q.nx * q.ny > (1L << 30)) return fail(FIVONAGI_ERR_ARG, "grid too large");
+    if (!(q.dx > 0) || !(q.dy > 0) || !(q.dt0 > 0) || !(q.cfl_wish > 0))
+      return fail(FIVONAGI_ERR_ARG, "dx, dy, dt0, cfl_wish must be > 0");
+    if (q.bc < 0 || q.bc > 2) return fail(FIVONAGI_ERR_ARG, "unknown boundary kind");
+    if (q.cfl_mode < 0 || q.cfl_mode > 1) return fail(FIVONAGI_ERR_ARG, "unknown cfl mode");
+    if (!q_host[3 * k] || !q_host[3 * k + 1] || !q_host[3 * k + 2])
+      return fail(FIVONAGI_ERR_ARG, "null state pointer");
+  }
+  auto* c = new fivonagi_ctx();
+  c->n_inst = n_inst;
+  c->params.assign(p, p + n_inst);
+  c->desc.resize(n_inst);
+  c->dt_seq.resize(n_inst);
+  hipError_t e;
+  e = hipGetDevice(&c->device);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipGetDevice"); }
+  e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+  c->stream = c->own_stream;
+
+  c->d_ctl = (fv::Ctl*)c->dalloc(sizeof(fv::Ctl) * n_inst, &e);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc ctl"); }
+  auto* d_sync = (fv::SyncSlots*)c->dalloc(sizeof(fv::SyncSlots) * n_inst, &e);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc sync"); }
+  e = hipMemset(d_sync, 0, sizeof(fv::SyncSlots) * n_inst);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemset sync"); }
+  std::vector<fv::Ctl> ctl(n_inst);
+  for (int k = 0; k < n_inst; ++k) {
+    const auto& q = p[k];
+    auto& d = c->desc[k];
+    std::memset(&d, 0, sizeof(d));
+    const size_t n = (size_t)q.nx * q.ny;
+    float* buf = (float*)c->dalloc(sizeof(float) * n * 6, &e);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc state"); }
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < 3; ++f) d.q[b][f] = buf + (size_t)(3 * b + f) * n;
+    for (int f = 0; f < 3; ++f) {
+      e = hipMemcpy(d.q[0][f], q_host[3 * k + f], sizeof(float) * n, hipMemcpyHostToDevice);
+      if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy state"); }
+      e = hipMemcpy(d.q[1][f], q_host[3 * k + f], sizeof(float) * n, hipMemcpyHostToDevice);
+      if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy state"); }
+    }
+    // edge drive amplitude/phase folded into amp*cos(phase), amp*sin(phase)
+    std::vector<float> ec(q.ny, 0.f), es(q.ny, 0.f);
+    for (int j = 0; j < q.ny; ++j) {
+      const double a = q.edge_amp ? q.edge_amp[j] : 0.0;
+      const double ph = q.edge_phase ? q.edge_phase[j] : 0.0;
+      ec[j] = (float)(a * std::cos(ph));
+      es[j] = (float)(a * std::sin(ph));
+    }
+    float* edge = (float*)c->dalloc(sizeof(float) * 2 * q.ny, &e);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc edge"); }
+    e = hipMemcpy(edge, ec.data(), sizeof(float) * q.ny, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(edge + q.ny, es.data(), sizeof(float) * q.ny, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy edge"); }
+    d.edge_c = edge;
+    d.edge_s = edge + q.ny;
+    // CFL subset
+    d.cfl_mask = fv::kCflAll;
+    if (q.cfl_mode == FIVONAGI_CFL_REFERENCE_FOLD) {
+      std::vector<uint8_t> m = fold_subset(q.nx, q.ny);
+      bool even = true;
+      for (size_t o = 0; o < n && even; ++o) even = (m[o] != 0) == ((o & 1) == 0);
+      if (even) {
+        d.cfl_mask = fv::kCflEven;
+      } else {
+        std::vector<uint32_t> bits((n + 31) / 32, 0u);
+        for (size_t o = 0; o < n; ++o)
+          if (m[o]) bits[o >> 5] |= 1u << (o & 31);
+        uint32_t* db = (uint32_t*)c->dalloc(sizeof(uint32_t) * bits.size(), &e);
+        if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc cfl bits"); }
+        e = hipMemcpy(db, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy cfl bits"); }
+        d.cfl_bits = db;
+        d.cfl_mask = fv::kCflBitmap;
+      }
+    }
+    d.ctl = c->d_ctl + k;
+    d.sync = d_sync + k;
+    d.dt_seq = nullptr;
+    d.dt_seq_len = 0;
+    d.dx = q.dx; d.dy = q.dy; d.delta = q.delta;
+    d.cfl_max = q.cfl_max; d.cfl_wish = q.cfl_wish; d.omega = q.omega;
+    d.tay_rhoa = q.tay_rhoa; d.tay_c = q.tay_c;
+    d.tay_cos = std::cos(q.tay_theta); d.tay_sin = std::sin(q.tay_theta);
+    d.tay_yoff = q.tay_yoff; d.tay_jc = q.tay_jc;
+    d.nx = q.nx; d.ny = q.ny; d.bc = q.bc;
+    d.beta2m2 = (float)(2.0 * (q.beta - 1.0));
+
+    auto& t = ctl[k];
+    std::memset(&t, 0, sizeof(t));
+    t.dt = q.dt0;
+    t.T = q.T0;
+    t.target_acc = 0;
+    set_ctl_derived(t, q);
+  }
+  e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * n_inst, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
+
+  build_items(c, auto_rows(c->params));
+  c->d_desc = (fv::InstDesc*)c->dalloc(sizeof(fv::InstDesc) * n_inst, &e);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc desc"); }
+  c->d_items = (fv::WorkItem*)c->dalloc(sizeof(fv::WorkItem) * c->items.size(), &e);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc items"); }
+  e = hipMemcpy(c->d_items, c->items.data(), sizeof(fv::WorkItem) * c->items.size(),
+                hipMemcpyHostToDevice);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy items"); }
+  int rc = upload_tables(c);
+  if (rc) { delete c; return rc; }
+  *out = c;
+  return 0;
+}
+
+int fivonagi_set_stream(fivonagi_ctx* c, void* s) {
+  if (!c) return fail(FIVONAGI_ERR_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->drop_graph();
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return 0;
+}
+
+int fivonagi_set_dt_sequence(fivonagi_ctx* c, int inst, const double* dt, int64_t n) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  if (dt && n <= 0) return fail(FIVONAGI_ERR_ARG, "empty dt sequence");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  auto& d = c->desc[inst];
+  if (dt) {
+    for (int64_t k = 0; k < n; ++k)
+      if (!(dt[k] > 0)) return fail(FIVONAGI_ERR_ARG, "dt sequence values must be > 0");
+    hipError_t e;
+    double* dd = (double*)c->dalloc(sizeof(double) * n, &e);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc dt_seq");
+    HIP_TRY(hipMemcpy(dd, dt, sizeof(double) * n, hipMemcpyHostToDevice));
+    d.dt_seq = dd;
+    d.dt_seq_len = n;
+    // the sequence is indexed by the attempt counter
+    auto& t = ctl[inst];
+    if (t.n_att >= n) return fail(FIVONAGI_ERR_ARG, "dt sequence shorter than attempts made");
+    t.dt = dt[t.n_att];
+    set_ctl_derived(t, c->params[inst]);
+    HIP_TRY(hipMemcpy(c->d_ctl + inst, &t, sizeof(fv::Ctl), hipMemcpyHostToDevice));
+  } else {
+    d.dt_seq = nullptr;
+    d.dt_seq_len = 0;
+  }
+  return upload_tables(c);
+}
+
+int fivonagi_enqueue(fivonagi_ctx* c, int64_t n) {
+  if (!c || n < 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
+  if (!c->unlimited) {
+    // let every instance run every attempt enqueued from now on
+    std::vector<int64_t> tgt(c->n_inst, INT64_MAX);
+    if (int rc = write_targets(c, tgt)) return rc;
+    c->unlimited = true;
+  }
+  return launch_attempts(c, n);
+}
+
+int fivonagi_sync(fivonagi_ctx* c) {
+  if (!c) return fail(FIVONAGI_ERR_ARG, "null context");
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  for (int k = 0; k < c->n_inst; ++k) {
+    if (ctl[k].err == 2) return fail(FIVONAGI_ERR_EIGEN, "imaginary eigenvalue (tmp3 < 0)");
+    if (ctl[k].err == 9) return fail(FIVONAGI_ERR_DTSEQ, "prescribed dt sequence exhausted");
+  }
+  return 0;
+}
+
+int fivonagi_step(fivonagi_ctx* c, int64_t nsteps) {
+  if (!c || nsteps < 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  std::vector<int64_t> target(c->n_inst);
+  for (int k = 0; k < c->n_inst; ++k) {
+    if (ctl[k].err) return fivonagi_sync(c);
+    target[k] = ctl[k].n_acc + nsteps;
+  }
+  if (int rc = write_targets(c, target)) return rc;
+  c->unlimited = false;
+  int64_t remaining = nsteps;
+  for (int round = 0; remaining > 0; ++round) {
+    if (round > 10000) return fail(FIVONAGI_ERR_STALL, "steps keep getting rejected");
+    if (int rc = launch_attempts(c, remaining)) return rc;
+    if (int rc = read_ctls(c, ctl)) return rc;
+    remaining = 0;
+    for (int k = 0; k < c->n_inst; ++k) {
+      if (ctl[k].err) return fivonagi_sync(c);
+      remaining = std::max(remaining, target[k] - ctl[k].n_acc);
+    }
+  }
+  return 0;
+}
+
+int fivonagi_collect(fivonagi_ctx* c, int inst, float* q1, float* q2, float* q3, double* T,
+                     double* dt, int64_t* n_acc, int64_t* n_rej) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  const auto& t = ctl[inst];
+  const auto& d = c->desc[inst];
+  const size_t n = (size_t)d.nx * d.ny;
+  float* dst[3] = {q1, q2, q3};
+  for (int f = 0; f < 3; ++f)
+    if (dst[f])
+      HIP_TRY(hipMemcpy(dst[f], d.q[t.cur][f], sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (T) *T = t.T;
+  if (dt) *dt = t.dt;
+  if (n_acc) *n_acc = t.n_acc;
+  if (n_rej) *n_rej = t.n_rej;
+  return 0;
+}
+
+int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  if (!info) return fail(FIVONAGI_ERR_ARG, "null info");
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  const auto& t = ctl[inst];
+  info->T = t.T;
+  info->dt = t.dt;
+  info->last_cfl = t.last_cfl;
+  info->n_accepted = t.n_acc;
+  info->n_rejected = t.n_rej;
+  info->n_attempts = t.n_att;
+  info->error = t.err;
+  info->cur = t.cur;
+  info->n_items = c->desc[inst].n_items;
+  int h = 0;
+  for (const auto& w : c->items)
+    if (w.inst == inst) { h = w.y1 - w.y0; break; }
+  info->rows_per_wave = h;
+  return 0;
+}
+
+void fivonagi_destroy(fivonagi_ctx* c) { delete c; }
+
+}  // extern "C"

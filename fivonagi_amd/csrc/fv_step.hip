@@ -1,0 +1,352 @@
+// fv_step.hip -- the fused FiVoNAGI time step for gfx950 (MI355X).
+//
+// One launch = one step attempt of every active instance.  It replaces the
+// reference's per-step sequence (common/main.cu:310-635): texCpy, boundary,
+// getWavesSpeedsCFL_y, step_y, calcLimiters_y, writeLimiters_y, the same four
+// in x, 21 reduceCFL launches, reduceCFL2, source, update_T, update_dt and the
+// seven blocking device->host copies.
+//
+// Work decomposition ("march"): one wave64 per work item.  Lane l owns column
+// x0 - 3 + l; the wave marches upward through rows [y0-3, y1+3) of its strip:
+//   load row r+1 (prefetch; boundary rules applied when the row is consumed)
+//   -> y-Riemann at interface r        (registers, same lane)
+//   -> y-limiter at interface r-1      (3-interface register window)
+//   -> q_y(r-2)                        (first + second order, fused)
+//   -> x-Riemann/limiter on row r-2    (neighbours via DPP wave_shr/wave_shl)
+//   -> q_xy(r-2)
+//   -> source on row r-3               (3-row register window + DPP)
+//   -> store row r-3 (lanes 3..60)
+// The loop is unrolled three times so the period-3 register windows rotate
+// by renaming instead of moves.  No LDS and no barriers: the 3-column halo on
+// each side makes the waves independent.
+//
+// CFL and accept/reject: each wave reduces its cells' cfl to one float, then
+// one relaxed agent-scope atomicMax + one ticket atomicAdd on its group's
+// 128-byte line (8 groups per instance); the last wave of a group bumps the
+// instance ticket, and the last group's last wave (the finalizer) decides
+// accept/reject, dt, T, and flips the ping-pong index in device memory.  No
+// memory fences are used: the only data crossing waves inside a launch is
+// read and written with atomics, and each wave waits for its max to complete
+// before it takes a ticket.  Everything else crosses launches.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../apps/common/boundary.h"
+#include "../apps/na/solver.h"
+#include "fv_types.h"
+
+namespace fv {
+
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float cgfloat;
+typedef __attribute__((address_space(1))) const uint32_t cguint;
+
+// DPP cross-lane moves over the whole wave64 (gfx9 wave_shr:1 / wave_shl:1).
+// shr1: lane l receives lane l-1's value (lane 0 receives 0).
+// shl1: lane l receives lane l+1's value (lane 63 receives 0).
+__device__ __forceinline__ float shr1(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shl1(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ uint32_t amax_relaxed(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t aadd_relaxed(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t axchg_relaxed(uint32_t* p, uint32_t v) {
+  return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct Row {         // one cell of a row as the y sweep sees it
+  float q1, q2, q3;  // rho', rho u, rho v
+  float un, ut;      // y normal velocity v = q3/rho, tangential u = q2/rho
+};
+
+struct YI {          // y interface with its limited flux towards the + cell
+  fvna::Iface f;
+  float P[3];        // apdq - fadd, in (density, normal, tangential) order
+};
+
+struct XY {          // one cell of q_xy, with its velocities for the source
+  float q1, q2, q3, u, v;
+};
+
+__global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
+                                                     const WorkItem* __restrict__ items,
+                                                     int n_items) {
+  // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
+  // run of work items so vertically adjacent chunks of a strip (which share
+  // halo rows) hit the same L2.
+  int item;
+  {
+    const int b = blockIdx.x;
+    const int q = n_items >> 3, rem = n_items & 7, x = b & 7, k = b >> 3;
+    item = x * q + min(x, rem) + k;
+  }
+  const WorkItem it = items[item];
+  const InstDesc& D = insts[it.inst];
+  Ctl* ctl = D.ctl;
+
+  // ---- per-step scalars (written by the previous step's finalizer) ----
+  if (ctl->err != 0 || ctl->n_acc >= ctl->target_acc) return;
+  const int cur = ctl->cur;
+  const double dt = ctl->dt;
+  const double Tnow = ctl->T;
+  const float dtx = ctl->dtx, dty = ctl->dty;
+  const float ddx2 = ctl->ddx2, ddy2 = ctl->ddy2;
+  const float sin_t = (float)ctl->drive_s, cos_t = (float)ctl->drive_c;
+  const float beta2m2 = D.beta2m2;
+  const int nx = D.nx, ny = D.ny;
+  const int bc = D.bc;
+  const int cfl_mask = D.cfl_mask;
+
+  cgfloat* __restrict__ in1 = (cgfloat*)D.q[cur][0];
+  cgfloat* __restrict__ in2 = (cgfloat*)D.q[cur][1];
+  cgfloat* __restrict__ in3 = (cgfloat*)D.q[cur][2];
+  gfloat* __restrict__ o1 = (gfloat*)D.q[cur ^ 1][0];
+  gfloat* __restrict__ o2 = (gfloat*)D.q[cur ^ 1][1];
+  gfloat* __restrict__ o3 = (gfloat*)D.q[cur ^ 1][2];
+  cgfloat* __restrict__ edge_c = (cgfloat*)D.edge_c;
+  cgfloat* __restrict__ edge_s = (cgfloat*)D.edge_s;
+  cguint* __restrict__ cbits = (cguint*)D.cfl_bits;
+
+  const int lane = threadIdx.x;
+  const int i = it.x0 - kHalo + lane;
+  const bool col_in = (i >= 0) && (i < nx);
+  const int ic = min(max(i, 0), nx - 1);
+  const bool col_int = (i >= 2) && (i <= nx - 3);
+  const bool out_lane = (lane >= kHalo) && (lane < kLanes - kHalo) && (i < nx);
+  const bool cfl_col = out_lane && (i >= 1);
+  const int y0 = it.y0, y1 = it.y1;
+
+  // Ghost-cell source columns (apps/common/boundary.h): colA on rows
+  // 2..ny-3, colB on ghost rows (top/bottom rules beat the right rule; the
+  // left rule beats everything and keeps its own row).
+  const bool taylor = (bc == kBcTaylor);
+  const bool left = !taylor && (ic <= 1);
+  const int colA = taylor ? ic : (left ? 2 : (ic >= nx - 2 ? nx - 3 : ic));
+  const int colB = taylor ? ic : (left ? 2 : ic);
+  const bool drive_lane = left && (bc == kBcEdgeDrive);
+
+  // raw (unpatched) load of row r; the boundary patch is applied when the row
+  // is consumed one iteration later, so the load latency overlaps a full step
+  auto load_raw = [&](int r, float& a, float& b, float& c) {
+    const int rc = min(max(r, 0), ny - 1);
+    const bool grow = (rc <= 1) || (rc >= ny - 2);
+    const int jm = taylor ? rc : (rc >= ny - 2 ? ny - 3 : (rc <= 1 ? 2 : rc));
+    const int o = (left ? rc : jm) * nx + (grow ? colB : colA);
+    a = in1[o];
+    b = in2[o];
+    c = in3[o];
+  };
+  auto patch = [&](int r, Row& q) {
+    const int rc = min(max(r, 0), ny - 1);
+    if (bc == kBcEdgeDrive) {
+      float a, b, c;
+      fvbc::drive(edge_c[rc], edge_s[rc], sin_t, cos_t, a, b, c);
+      q.q1 = drive_lane ? a : q.q1;
+      q.q2 = drive_lane ? b : q.q2;
+      q.q3 = drive_lane ? c : q.q3;
+    } else if (taylor) {
+      if (fvbc::is_ghost(ic, rc, nx, ny))
+        fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx,
+                     D.dy, Tnow, ic, rc, q.q1, q.q2, q.q3);
+    }
+    const float ir = fvna::rcp(1.0f + q.q1);
+    q.un = q.q3 * ir;
+    q.ut = q.q2 * ir;
+  };
+  auto counted = [&](int j) -> bool {
+    if (cfl_mask == kCflAll) return true;
+    const uint32_t o = (uint32_t)i + (uint32_t)j * (uint32_t)nx;
+    if (cfl_mask == kCflEven) return (o & 1u) == 0u;
+    return (cbits[o >> 5] >> (o & 31u)) & 1u;
+  };
+
+  float Sx = 0.0f, Sy = 0.0f;
+  bool err = false;
+
+  // One march step.  bn/bm/bo: q_b rows r, r-1, r-2; fn/fc/fl: y interfaces
+  // r, r-1, r-2; xn/xm/xo: q_xy rows r-2, r-3, r-4.  n1..n3 hold the raw
+  // prefetched row r on entry and row r+1 on exit.
+  float n1, n2, n3;
+  auto body = [&](int r, Row& bn, Row& bm, Row& bo, YI& fn, YI& fc, YI& fl, XY& xn, XY& xm,
+                  XY& xo) {
+    bn.q1 = n1; bn.q2 = n2; bn.q3 = n3;
+    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3);
+    patch(r, bn);
+    if (r < y0 - 2) return;
+
+    // ---- y interface r (rows r-1 | r): normal = q3, tangential = q2 ----
+    fvna::riemann(bm.q1, bm.q3, bm.q2, bm.un, bm.ut, bn.q1, bn.q3, bn.q2, bn.un, bn.ut,
+                  beta2m2, fn.f);
+    err |= fn.f.bad && col_in && (r >= 1) && (r < ny);
+    if (cfl_col && r >= y0 && r < y1 && r >= 1 && counted(r)) Sy = fmaxf(Sy, fn.f.smax);
+    if (r < y0) return;
+
+    // ---- y limiter at interface r-1 ----
+    float M[3];
+    {
+      const float dl1 = fvna::dot1(fc.f.w, fl.f.w), dr1 = fvna::dot1(fc.f.w, fn.f.w);
+      const float dl2 = fvna::dot2(fc.f.w, fl.f.w), dr2 = fvna::dot2(fc.f.w, fn.f.w);
+      const float dl3 = fc.f.w.a3 * fl.f.w.a3, dr3 = fc.f.w.a3 * fn.f.w.a3;
+      fvna::limited_flux(fc.f, dl1, dr1, dl2, dr2, dl3, dr3, dty, fc.P, M);
+    }
+    if (r < y0 + 1) return;
+
+    // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2)
+    const int m = r - 2;
+    const bool row_int = (m >= 2) && (m <= ny - 3);
+    const bool upd = col_int && row_int;
+    const float y1q = upd ? bo.q1 - dty * (fl.P[0] + M[0]) : bo.q1;
+    const float y3q = upd ? bo.q3 - dty * (fl.P[1] + M[1]) : bo.q3;
+    const float y2q = upd ? bo.q2 - dty * (fl.P[2] + M[2]) : bo.q2;
+
+    // ---- x sweep on row m: normal = q2, tangential = q3 ----
+    {
+      const float ir = fvna::rcp(1.0f + y1q);
+      const float un = y2q * ir, ut = y3q * ir;
+      const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
+      const float lun = shr1(un), lut = shr1(ut);
+      fvna::Iface fx;
+      fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, beta2m2, fx);
+      err |= fx.bad && col_in && (i >= 1) && (m >= 0) && (m < ny);
+      if (cfl_col && m >= y0 && m < y1 && counted(m)) Sx = fmaxf(Sx, fx.smax);
+      // waves of the interface to the left (k-1)
+      fvna::Waves wl;
+      wl.a1 = shr1(fx.w.a1); wl.w21 = shr1(fx.w.w21); wl.w31 = shr1(fx.w.w31);
+      wl.a2 = shr1(fx.w.a2); wl.w22 = shr1(fx.w.w22); wl.w32 = shr1(fx.w.w32);
+      wl.a3 = shr1(fx.w.a3);
+      const float dl1 = fvna::dot1(fx.w, wl), dl2 = fvna::dot2(fx.w, wl);
+      const float dl3 = fx.w.a3 * wl.a3;
+      // W(k).W(k+1) is the right neighbour's W(k+1).W(k)
+      const float dr1 = shl1(dl1), dr2 = shl1(dl2), dr3 = shl1(dl3);
+      float PX[3], MX[3];
+      fvna::limited_flux(fx, dl1, dr1, dl2, dr2, dl3, dr3, dtx, PX, MX);
+      const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
+      xn.q1 = upd ? y1q - dtx * (PX[0] + mr0) : y1q;
+      xn.q2 = upd ? y2q - dtx * (PX[1] + mr1) : y2q;
+      xn.q3 = upd ? y3q - dtx * (PX[2] + mr2) : y3q;
+      const float ir2 = fvna::rcp(1.0f + xn.q1);
+      xn.u = xn.q2 * ir2;
+      xn.v = xn.q3 * ir2;
+    }
+    if (r < y0 + 3) return;
+
+    // ---- source on row s = r-3 (apps/na/source.h:86-104), then store ----
+    const int s = r - 3;
+    const bool s_int = col_int && (s >= 2) && (s <= ny - 3);
+    const float uw = shr1(xm.u), ue = shl1(xm.u);
+    const float vw = shr1(xm.v), ve = shl1(xm.v);
+    const float r2 = (xm.q2 + ddx2 * (fmaf(-2.0f, xm.u, uw) + ue))
+                     + ddy2 * (fmaf(-2.0f, xm.u, xo.u) + xn.u);
+    const float r3 = (xm.q3 + ddx2 * (fmaf(-2.0f, xm.v, vw) + ve))
+                     + ddy2 * (fmaf(-2.0f, xm.v, xo.v) + xn.v);
+    if (out_lane) {
+      const int o = s * nx + i;
+      o1[o] = xm.q1;            // q1 passes the source unchanged
+      o2[o] = s_int ? r2 : xm.q2;
+      o3[o] = s_int ? r3 : xm.q3;
+    }
+  };
+
+  Row B0{}, B1{}, B2{};
+  YI F0{}, F1{}, F2{};
+  XY X0{}, X1{}, X2{};
+  load_raw(y0 - kHalo, n1, n2, n3);
+  for (int r = y0 - kHalo; r <= y1 + 2; r += 3) {
+    body(r, B0, B2, B1, F0, F2, F1, X0, X2, X1);
+    if (r + 1 > y1 + 2) break;
+    body(r + 1, B1, B0, B2, F1, F0, F2, X1, X0, X2);
+    if (r + 2 > y1 + 2) break;
+    body(r + 2, B2, B1, B0, F2, F1, F0, X2, X1, X0);
+  }
+
+  // ---- CFL / error reduction, then the instance finalizer ----
+  // per-cell cfl = |s| * dt/d (fv.h:148-180); max over the wave's cells
+  const float cflw = wave_max(fmaxf(Sy * dty, Sx * dtx));
+  const bool any_err = __any(err);
+  if (lane != 0) return;
+  SyncSlots* sy = D.sync;
+  const int local = item - D.first_item;
+  const int n = D.n_items;
+  const int g = local % kGroups;
+  const int gsize = n / kGroups + (g < n % kGroups ? 1 : 0);
+  const int ngroups = n < kGroups ? n : kGroups;
+  if (any_err) aadd_relaxed(&sy->err_flag, 1u);
+  const uint32_t prev = amax_relaxed(&sy->grp[g].cfl_bits, __float_as_uint(cflw));
+  // the max (and the error flag) must be performed before the ticket is
+  // taken: consume the returned value, then wait for every memory operation
+  asm volatile("" ::"v"(prev));
+  __builtin_amdgcn_s_waitcnt(0);
+  if (aadd_relaxed(&sy->grp[g].ticket, 1u) != (uint32_t)(gsize - 1)) return;
+  __builtin_amdgcn_s_waitcnt(0);
+  if (aadd_relaxed(&sy->top_ticket, 1u) != (uint32_t)(ngroups - 1)) return;
+
+  // finalizer: every wave of the instance has taken its ticket
+  float cfl32 = 0.0f;
+  for (int k = 0; k < ngroups; ++k) {
+    cfl32 = fmaxf(cfl32, __uint_as_float(axchg_relaxed(&sy->grp[k].cfl_bits, 0u)));
+    axchg_relaxed(&sy->grp[k].ticket, 0u);
+  }
+  axchg_relaxed(&sy->top_ticket, 0u);
+  const uint32_t e = axchg_relaxed(&sy->err_flag, 0u);
+  if (e) {
+    ctl->err = 2;   // imaginary eigenvalue: step discarded, instance stops
+    return;
+  }
+  // common/main.cu:199-230 with a clean retry on rejection
+  const double cfl = (double)cfl32;
+  const bool presc = D.dt_seq != nullptr;
+  const bool acc = presc || (cfl < D.cfl_max);
+  const int64_t att = ctl->n_att + 1;
+  ctl->n_att = att;
+  ctl->last_cfl = cfl;
+  double T = Tnow;
+  if (acc) {
+    T = Tnow + dt;
+    ctl->T = T;
+    ctl->n_acc = ctl->n_acc + 1;
+    ctl->cur = cur ^ 1;
+  } else {
+    ctl->n_rej = ctl->n_rej + 1;
+  }
+  double ndt;
+  if (presc) {
+    ndt = (att < D.dt_seq_len) ? D.dt_seq[att] : dt;
+    if (att >= D.dt_seq_len) ctl->err = 9;   // ran past the prescribed sequence
+  } else {
+    ndt = dt * D.cfl_wish / cfl;
+    if (!acc && ndt > dt * 0.9) ndt = dt * 0.9;
+  }
+  ctl->dt = ndt;
+  ctl->dtx = (float)(ndt / D.dx);
+  ctl->dty = (float)(ndt / D.dy);
+  ctl->ddx2 = (float)(D.delta * ndt / (D.dx * D.dx));
+  ctl->ddy2 = (float)(D.delta * ndt / (D.dy * D.dy));
+  const double th = fmod(D.omega * T, 6.283185307179586476925286766559);
+  ctl->drive_s = sin(th);
+  ctl->drive_c = cos(th);
+}
+
+}  // namespace fv
+
+extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
+                                     int n_items, hipStream_t stream) {
+  if (n_items <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fv::fv_step_kernel, dim3(n_items), dim3(fv::kLanes), 0, stream, insts,
+                     items, n_items);
+  return hipGetLastError();
+}

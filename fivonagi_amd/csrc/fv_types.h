@@ -1,0 +1,84 @@
+// fv_types.h -- device-side tables shared by the step kernel and the host
+// runtime (fv_runtime.cpp).  Plain C layout, no torch types.
+#ifndef FIVONAGI_FV_TYPES_H
+#define FIVONAGI_FV_TYPES_H
+
+#include <stdint.h>
+
+namespace fv {
+
+// Output columns per strip: one wave = 64 lanes = 58 output columns plus a
+// 3-column halo on each side (dependency radius of y-sweep -> x-sweep -> source).
+constexpr int kLanes = 64;
+constexpr int kHalo = 3;
+constexpr int kStripCols = kLanes - 2 * kHalo;   // 58
+
+enum BoundaryKind : int32_t { kBcZeroGrad = 0, kBcEdgeDrive = 1, kBcTaylor = 2 };
+enum CflMask : int32_t { kCflAll = 0, kCflEven = 1, kCflBitmap = 2 };
+
+// Mutable per-instance control block (device memory).  Written only by the
+// last wave of an instance in each step ("finalizer"), read by every wave at
+// the start of the next step.
+struct Ctl {
+  double dt;            // dt of the next attempt
+  double T;             // time at the start of the next attempt
+  double last_cfl;      // cfl of the last attempt
+  double drive_s, drive_c;   // sin/cos(omega*T) for the next attempt
+  int64_t n_acc, n_rej, n_att;
+  int64_t target_acc;   // waves skip the instance once n_acc reaches this
+  int32_t cur;          // buffer holding the current state (0/1)
+  int32_t err;          // 0 ok, 2 imaginary eigenvalue
+  float dtx, dty;       // dt/dx, dt/dy of the next attempt
+  float ddx2, ddy2;     // delta*dt/dx^2, delta*dt/dy^2
+  uint32_t pad0, pad1;
+};
+
+// Reduction slots of the running attempt, per instance: the waves of an
+// instance are split into kGroups groups (local item index mod kGroups), each
+// with its own 128-byte line, so the same-address atomics of one step are
+// spread over kGroups + 1 lines.  Reset by the finalizer.
+constexpr int kGroups = 8;
+struct alignas(128) GroupSlot {
+  uint32_t cfl_bits;    // max cfl of the group's cells (float bits, >= 0)
+  uint32_t ticket;      // waves of the group done
+  uint32_t pad[30];
+};
+struct alignas(128) SyncSlots {
+  GroupSlot grp[kGroups];
+  uint32_t top_ticket;  // groups done
+  uint32_t err_flag;    // any imaginary eigenvalue in the attempt
+  uint32_t pad[30];
+};
+
+// Read-only per-instance description (device memory).
+struct InstDesc {
+  float* q[2][3];       // ping-pong SoA state buffers, offset i + j*nx
+  Ctl* ctl;
+  const float* edge_c;  // amp_j * cos(phase_j), length ny (kBcEdgeDrive)
+  const float* edge_s;  // amp_j * sin(phase_j), length ny
+  const uint32_t* cfl_bits;  // kCflBitmap: bit (i + j*nx) set = cell counted
+  const double* dt_seq; // prescribed dt per attempt, or null
+  int64_t dt_seq_len;
+  double dx, dy, delta, cfl_max, cfl_wish, omega;
+  double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff;
+  int32_t nx, ny;
+  int32_t bc, cfl_mask;
+  SyncSlots* sync;
+  int32_t n_items;      // work items (waves) of this instance per step
+  int32_t first_item;   // index of the instance's first work item
+  int32_t tay_jc;
+  int32_t pad1;
+  float beta2m2;        // 2*(BETA - 1)
+  float pad;
+};
+
+// One wave's work: a 58-column strip of one instance, rows [y0, y1).
+struct WorkItem {
+  int32_t inst;
+  int32_t x0;           // first output column of the strip
+  int32_t y0, y1;
+};
+
+}  // namespace fv
+
+#endif  // FIVONAGI_FV_TYPES_H

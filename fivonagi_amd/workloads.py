@@ -1,0 +1,121 @@
+"""The five workloads of the time step, as seeded synthetic inputs.
+
+Each builder returns fivonagi_amd.Problem objects (parameters + initial state):
+
+  gauss_pulse   400x400 Gaussian density pulse, zero-gradient edges, 1000 steps
+  edge_drive    4096x1024 left-edge sinusoidal drive through a Gaussian aperture,
+                zero-gradient on the other edges, 8000 steps (the HIFU-beam-like
+                single instance the throughput metric is quoted on)
+  batched       64 instances of uneven size (nx, ny in 256..2048, step 64),
+                Gaussian pulses or windowed plane waves, 2000 steps each
+  taylor        the closed-form Taylor shock of apps/taylor-angle-xy
+
+Units follow the reference apps: length in wavelengths of the 1-unit period
+(20 cells per unit, apps/hifu-beam/parameters.h:28-30, :96-97), c0 = 1, rho0 = 1.
+"""
+import math
+
+import numpy as np
+
+from . import BC_EDGE_DRIVE, BC_TAYLOR, BC_ZEROGRAD, CFL_REFERENCE_FOLD, Problem
+
+CELLS_PER_UNIT = 20.0
+
+
+def gauss_pulse(nx=400, ny=400, amp=0.01, sigma_cells=10.0, beta=3.5, delta=1e-4,
+                cfl_mode=CFL_REFERENCE_FOLD, dt0=0.005, rows_per_wave=0):
+    """Gaussian rho' pulse centred at ((nx-1)/2, (ny-1)/2), zero velocity."""
+    x = np.arange(nx) - (nx - 1) / 2.0
+    y = np.arange(ny) - (ny - 1) / 2.0
+    X, Y = np.meshgrid(x, y)
+    q = np.zeros((3, ny, nx))
+    q[0] = amp * np.exp(-(X ** 2 + Y ** 2) / (2.0 * sigma_cells ** 2))
+    d = 1.0 / CELLS_PER_UNIT
+    return Problem(nx, ny, d, d, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=dt0,
+                   T0=0.0, bc=BC_ZEROGRAD, cfl_mode=cfl_mode, rows_per_wave=rows_per_wave)
+
+
+def edge_profile(ny, amp=0.02, halfwidth_cells=200.0, centre=None):
+    jc = (ny - 1) / 2.0 if centre is None else centre
+    j = np.arange(ny)
+    return amp * np.exp(-((j - jc) / halfwidth_cells) ** 2)
+
+
+def edge_drive(nx=4096, ny=1024, amp=0.02, halfwidth_cells=200.0, cells_per_wavelength=20.0,
+               beta=3.5, delta=5e-5, cfl_mode=CFL_REFERENCE_FOLD, dt0=0.005,
+               edge_amp=None, edge_phase=None, rows_per_wave=0):
+    """Quiescent medium driven from the left ghost columns:
+    rho' = amp_j sin(omega T + phase_j), omega = 2*pi/(wavelength in time units)."""
+    d = 1.0 / CELLS_PER_UNIT
+    omega = 2.0 * math.pi * CELLS_PER_UNIT / cells_per_wavelength
+    q = np.zeros((3, ny, nx))
+    if edge_amp is None:
+        edge_amp = edge_profile(ny, amp, halfwidth_cells)
+    return Problem(nx, ny, d, d, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=dt0, T0=0.0,
+                   bc=BC_EDGE_DRIVE, cfl_mode=cfl_mode, omega=omega, edge_amp=edge_amp,
+                   edge_phase=edge_phase, rows_per_wave=rows_per_wave)
+
+
+def plane_packet(nx, ny, amp, theta, wavelength_cells=20.0, width_cells=60.0):
+    """Right-going (along theta) Gaussian-windowed linear plane wave."""
+    x = np.arange(nx, dtype=np.float64)
+    y = np.arange(ny, dtype=np.float64)
+    X, Y = np.meshgrid(x, y)
+    s = X * math.cos(theta) + Y * math.sin(theta)
+    sc = 0.5 * (nx * math.cos(theta) + ny * math.sin(theta))
+    r = amp * np.sin(2 * math.pi * s / wavelength_cells) * np.exp(-((s - sc) / width_cells) ** 2)
+    q = np.zeros((3, ny, nx))
+    q[0] = r
+    q[1] = r * math.cos(theta)
+    q[2] = r * math.sin(theta)
+    return q
+
+
+def batched(n=64, seed=1234, lo=256, hi=2048, step=64, cfl_mode=CFL_REFERENCE_FOLD,
+            rows_per_wave=0):
+    rng = np.random.default_rng(seed)
+    sizes = np.arange(lo, hi + 1, step)
+    probs = []
+    d = 1.0 / CELLS_PER_UNIT
+    for _ in range(n):
+        nx, ny = int(rng.choice(sizes)), int(rng.choice(sizes))
+        amp = float(rng.uniform(1e-3, 2e-2))
+        beta = float(rng.uniform(3.0, 4.0))
+        delta = float(10 ** rng.uniform(-5.0, -3.0))
+        if rng.uniform() < 0.5:
+            sig = float(rng.uniform(5.0, 20.0))
+            x = np.arange(nx) - (nx - 1) / 2.0
+            y = np.arange(ny) - (ny - 1) / 2.0
+            X, Y = np.meshgrid(x, y)
+            q = np.zeros((3, ny, nx))
+            q[0] = amp * np.exp(-(X ** 2 + Y ** 2) / (2 * sig ** 2))
+        else:
+            q = plane_packet(nx, ny, amp, float(rng.uniform(0.0, math.pi / 2)))
+        probs.append(Problem(nx, ny, d, d, beta, delta, q, cfl_max=1.0, cfl_wish=0.9,
+                             dt0=0.005, T0=0.0, bc=BC_ZEROGRAD, cfl_mode=cfl_mode,
+                             rows_per_wave=rows_per_wave))
+    return probs
+
+
+TAYLOR_C = 2.1972245773362196   # apps/taylor-angle-xy/parameters.h:105
+
+
+def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_REFERENCE_FOLD):
+    """apps/taylor-angle-xy: NX = NY = 64*ETAC, XMAX = YMAX = 12.5; the exact
+    Taylor profile is both the initial state and the Dirichlet ghost data."""
+    nx = ny = 64 * eta_mult
+    dx = dy = xmax / (nx - 1)
+    rhoa = -delta / beta
+    mdt = xmax * 0.5 / math.cos(theta)
+    tide = math.ceil(4 + (xmax * 0.5 - mdt * math.sin(theta)) * math.sin(theta))
+    yoff = mdt * math.sin(theta)
+    jc = ny // 2
+    X = np.arange(nx)[None, :].astype(np.float64)
+    Y = np.arange(ny)[:, None].astype(np.float64)
+    T0 = -float(tide)
+    r = rhoa * np.tanh(TAYLOR_C * (X * dx * math.cos(theta) + ((Y - jc) * dy + yoff)
+                                   * math.sin(theta) - T0))
+    q = np.stack([r, r * math.cos(theta), r * math.sin(theta)])
+    t = dict(rhoa=rhoa, c=TAYLOR_C, theta=theta, yoff=yoff, jc=jc)
+    return Problem(nx, ny, dx, dy, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=0.005,
+                   T0=T0, bc=BC_TAYLOR, cfl_mode=cfl_mode, taylor=t)

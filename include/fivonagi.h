@@ -1,0 +1,110 @@
+/*
+ * fivonagi.h -- C ABI of the MI355X-native FiVoNAGI time step.
+ *
+ * The reference has no library boundary: an app is compiled into the step
+ * loop (apps/<app>/driver.cu:40-52 includes parameters.h, init.h, boundary.h,
+ * apps/na/{source,fv}.h and common/main.cu), memory is owned by gpu_data
+ * (common/data_definitions.h:17-89, gpu_init :523-871, clean_gpu :415-517)
+ * and the loop is calcFrame (common/main.cu:248-756).  These entry points
+ * replace that surface; each cites what it stands for.
+ *
+ * Conventions
+ *   - State is SoA FP32, three grids per instance, offset i + j*nx, x fastest,
+ *     ghost frame (2 wide) inside the nx*ny grid; q1 = rho' = rho - 1,
+ *     q2 = rho*u, q3 = rho*v (apps/hifu-beam/init.h:18-20).
+ *   - Host arrays are copied; the context owns every device allocation.
+ *   - Return codes keep the reference's simError numbering: 0 ok,
+ *     2 imaginary eigenvalue (apps/na/fv.h:102); rejected steps (simError 1,
+ *     common/main.cu:207-211) are retried internally and counted.
+ *     8 = bad argument, 9 = prescribed dt sequence exhausted,
+ *     negative = -(hipError_t).  Nothing calls exit().
+ *   - One context per stream; a context is used by one thread at a time; the
+ *     library starts no threads.
+ */
+#ifndef FIVONAGI_H
+#define FIVONAGI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fivonagi_ctx fivonagi_ctx;
+
+enum {
+  FIVONAGI_BC_ZEROGRAD = 0,   /* common/boundary.h:11-82                        */
+  FIVONAGI_BC_EDGE_DRIVE = 1, /* apps/hifu-beam/boundary.h:77-93, nu = (1,0)    */
+  FIVONAGI_BC_TAYLOR = 2      /* apps/taylor-angle-xy/boundary.h:38-55          */
+};
+enum {
+  FIVONAGI_CFL_REFERENCE_FOLD = 0, /* reduceCFL schedule, common/main.cu:503-524 */
+  FIVONAGI_CFL_ALL = 1             /* exact max over every cell                  */
+};
+enum {
+  FIVONAGI_OK = 0, FIVONAGI_ERR_EIGEN = 2, FIVONAGI_ERR_ARG = 8,
+  FIVONAGI_ERR_DTSEQ = 9, FIVONAGI_ERR_STALL = 10
+};
+
+/* Per-instance parameters: the runtime form of apps/<app>/parameters.h
+   (e.g. apps/hifu-beam/parameters.h:12-104) plus the boundary choice. */
+typedef struct {
+  int32_t nx, ny;          /* NX, NY (>= 8)                                   */
+  double dx, dy;           /* XMAX/(NX-1), YMAX/(NY-1) (common/main.cu:789-790) */
+  double beta, delta;      /* BETA, DIFFDELTA                                 */
+  double cfl_max, cfl_wish;/* CFLMAX, CFLWHISH                                */
+  double dt0, T0;          /* DTINI, -TIDE (common/main.cu:788, :792)         */
+  int32_t bc, cfl_mode;    /* FIVONAGI_BC_*, FIVONAGI_CFL_*                   */
+  double omega;            /* edge drive: p_j = amp_j sin(omega T + phase_j)  */
+  const double *edge_amp;  /* host, ny values (NULL = 0)                      */
+  const double *edge_phase;/* host, ny values (NULL = 0)                      */
+  double tay_rhoa, tay_c, tay_theta, tay_yoff; /* Taylor closed form          */
+  int32_t tay_jc;
+  int32_t rows_per_wave;   /* 0 = automatic; else rows per work item          */
+} fivonagi_params;
+
+typedef struct {
+  double T, dt, last_cfl;
+  int64_t n_accepted, n_rejected, n_attempts;
+  int32_t error, cur;
+  int32_t n_items, rows_per_wave;
+} fivonagi_info_t;
+
+/* gpu_init (common/data_definitions.h:523-871) + init() upload
+   (apps/APP/init.h): n_inst instances, p[k] and q_host[3k..3k+2] (q1,q2,q3). */
+int fivonagi_init(fivonagi_ctx **ctx, const fivonagi_params *p, int n_inst,
+                  const float *const *q_host);
+
+/* Run on the caller's HIP stream (hipStream_t) instead of the context's own. */
+int fivonagi_set_stream(fivonagi_ctx *ctx, void *hip_stream);
+
+/* Prescribed dt: attempt k of instance inst uses dt[k]; no CFL feedback and
+   no rejection (NULL/0 restores the feedback of common/main.cu:214-230). */
+int fivonagi_set_dt_sequence(fivonagi_ctx *ctx, int inst, const double *dt, int64_t n);
+
+/* The while (T < Tprint) body of common/main.cu:310-635, nsteps accepted
+   steps for every instance (rejected attempts retried).  Blocking. */
+int fivonagi_step(fivonagi_ctx *ctx, int64_t nsteps);
+
+/* Asynchronous: enqueue nattempts step attempts (graph replays) on the
+   context stream; no host synchronisation.  For timing loops. */
+int fivonagi_enqueue(fivonagi_ctx *ctx, int64_t nattempts);
+/* Wait for the stream; return the first instance error code (0 if none). */
+int fivonagi_sync(fivonagi_ctx *ctx);
+
+/* dataCollect/dataExport stand-in (apps/hifu-beam/data_export.h:29-75):
+   copy instance inst's state and counters to the host.  Any pointer may be NULL. */
+int fivonagi_collect(fivonagi_ctx *ctx, int inst, float *q1, float *q2, float *q3,
+                     double *T, double *dt, int64_t *n_accepted, int64_t *n_rejected);
+int fivonagi_info(fivonagi_ctx *ctx, int inst, fivonagi_info_t *info);
+
+/* clean_gpu (common/data_definitions.h:415-517) */
+void fivonagi_destroy(fivonagi_ctx *ctx);
+
+/* Message of the last error returned on this thread. */
+const char *fivonagi_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FIVONAGI_H */

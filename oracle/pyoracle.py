@@ -1,0 +1,195 @@
+"""ctypes binding of the CPU oracle (oracle/fvoracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+never as a stand-in for the HIP path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR = 0, 1, 2
+CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
+FORM_LITERAL, FORM_RHO1 = 0, 1
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("beta", C.c_double), ("delta", C.c_double),
+        ("cfl_max", C.c_double), ("cfl_wish", C.c_double),
+        ("dt0", C.c_double), ("T0", C.c_double),
+        ("bc", C.c_int32), ("cfl_mode", C.c_int32), ("form", C.c_int32),
+        ("nthreads", C.c_int32),
+        ("omega", C.c_double),
+        ("edge_amp", C.POINTER(C.c_double)), ("edge_phase", C.POINTER(C.c_double)),
+        ("tay_rhoa", C.c_double), ("tay_c", C.c_double),
+        ("tay_theta", C.c_double), ("tay_yoff", C.c_double),
+        ("tay_jc", C.c_int32), ("pad0", C.c_int32),
+        ("dt_seq", C.POINTER(C.c_double)), ("dt_seq_len", C.c_int64),
+    ]
+
+
+class State(C.Structure):
+    _fields_ = [
+        ("T", C.c_double), ("dt", C.c_double), ("last_cfl", C.c_double),
+        ("n_accepted", C.c_int64), ("n_rejected", C.c_int64),
+        ("n_attempts", C.c_int64), ("error", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+def _load(name):
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} missing: build it with `make -C oracle` (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
+    P = C.POINTER(Params)
+    S = C.POINTER(State)
+    lib.orc_state_init.argtypes = [P, S]
+    lib.orc_run_f64.argtypes = [P, dp, dp, dp, C.c_int64, S]
+    lib.orc_run_f64.restype = C.c_int
+    lib.orc_run_f32.argtypes = [P, fp, fp, fp, C.c_int64, S]
+    lib.orc_run_f32.restype = C.c_int
+    lib.orc_riemann_f64.argtypes = [dp, dp, C.c_double, C.c_int, dp]
+    lib.orc_riemann_f64.restype = C.c_int
+    lib.orc_philim_f64.argtypes = [C.c_double] * 4
+    lib.orc_philim_f64.restype = C.c_double
+    lib.orc_cfl_subset.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_uint8)]
+    lib.orc_boundary_f64.argtypes = [P, dp, dp, dp, dp, dp, dp, C.c_double]
+    lib.orc_boundary_f64.restype = C.c_int
+    lib.orc_source_f64.argtypes = [P, dp, dp, dp, dp, dp, dp, C.c_double]
+    lib.orc_source_f64.restype = C.c_int
+    lib.orc_sweep_f64.argtypes = [P, C.c_int, dp, dp, dp, C.c_double, dp]
+    lib.orc_sweep_f64.restype = C.c_int
+    return lib
+
+
+_LIBS = {}
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _LIBS:
+        _LIBS[name] = _load(name)
+    return _LIBS[name]
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Problem:
+    """Parameters of one grid instance, in the reference's vocabulary."""
+
+    def __init__(self, nx, ny, dx, dy, beta, delta, cfl_max=1.0, cfl_wish=0.9,
+                 dt0=0.005, T0=0.0, bc=BC_ZEROGRAD, cfl_mode=CFL_REFERENCE_FOLD,
+                 form=FORM_LITERAL, omega=0.0, edge_amp=None, edge_phase=None,
+                 taylor=None, dt_seq=None, nthreads=0):
+        self.nx, self.ny = int(nx), int(ny)
+        self.dx, self.dy = float(dx), float(dy)
+        self.beta, self.delta = float(beta), float(delta)
+        self.cfl_max, self.cfl_wish = float(cfl_max), float(cfl_wish)
+        self.dt0, self.T0 = float(dt0), float(T0)
+        self.bc, self.cfl_mode, self.form = int(bc), int(cfl_mode), int(form)
+        self.omega = float(omega)
+        self.edge_amp = None if edge_amp is None else np.ascontiguousarray(edge_amp, np.float64)
+        self.edge_phase = None if edge_phase is None else np.ascontiguousarray(edge_phase, np.float64)
+        self.taylor = taylor  # dict(rhoa, c, theta, yoff, jc)
+        self.dt_seq = None if dt_seq is None else np.ascontiguousarray(dt_seq, np.float64)
+        self.nthreads = int(nthreads)
+
+    def cparams(self):
+        p = Params()
+        p.nx, p.ny, p.dx, p.dy = self.nx, self.ny, self.dx, self.dy
+        p.beta, p.delta = self.beta, self.delta
+        p.cfl_max, p.cfl_wish, p.dt0, p.T0 = self.cfl_max, self.cfl_wish, self.dt0, self.T0
+        p.bc, p.cfl_mode, p.form, p.nthreads = self.bc, self.cfl_mode, self.form, self.nthreads
+        p.omega = self.omega
+        if self.edge_amp is not None:
+            p.edge_amp = _ptr(self.edge_amp, C.c_double)
+        if self.edge_phase is not None:
+            p.edge_phase = _ptr(self.edge_phase, C.c_double)
+        if self.taylor:
+            t = self.taylor
+            p.tay_rhoa, p.tay_c = t["rhoa"], t["c"]
+            p.tay_theta, p.tay_yoff, p.tay_jc = t["theta"], t["yoff"], int(t["jc"])
+        if self.dt_seq is not None:
+            p.dt_seq = _ptr(self.dt_seq, C.c_double)
+            p.dt_seq_len = len(self.dt_seq)
+        return p
+
+
+def run(prob, q, nsteps, dtype=np.float64, state=None, fast=False):
+    """Advance q (3, ny, nx) by nsteps accepted steps.  Returns (q, state, rc)."""
+    L = lib(fast)
+    p = prob.cparams()
+    q = np.ascontiguousarray(q, dtype=dtype).copy()
+    assert q.shape == (3, prob.ny, prob.nx)
+    if state is None:
+        state = State()
+        L.orc_state_init(C.byref(p), C.byref(state))
+    q1, q2, q3 = (np.ascontiguousarray(q[k]) for k in range(3))
+    if dtype == np.float64:
+        rc = L.orc_run_f64(C.byref(p), _ptr(q1, C.c_double), _ptr(q2, C.c_double),
+                           _ptr(q3, C.c_double), int(nsteps), C.byref(state))
+    else:
+        rc = L.orc_run_f32(C.byref(p), _ptr(q1, C.c_float), _ptr(q2, C.c_float),
+                           _ptr(q3, C.c_float), int(nsteps), C.byref(state))
+    return np.stack([q1, q2, q3]), state, rc
+
+
+def riemann(qL, qR, beta, form=FORM_LITERAL):
+    out = np.zeros(18)
+    qL = np.ascontiguousarray(qL, np.float64)
+    qR = np.ascontiguousarray(qR, np.float64)
+    rc = lib().orc_riemann_f64(_ptr(qL, C.c_double), _ptr(qR, C.c_double), float(beta),
+                               int(form), _ptr(out, C.c_double))
+    return rc, dict(s=out[0:3], W=out[3:12].reshape(3, 3).T.copy(),  # W[m, p]: comp m of wave p
+                    amdq=out[12:15], apdq=out[15:18])
+
+
+def philim(wnorm2, dotl, dotr, s):
+    return lib().orc_philim_f64(float(wnorm2), float(dotl), float(dotr), float(s))
+
+
+def cfl_subset(nx, ny):
+    m = np.zeros(nx * ny, np.uint8)
+    lib().orc_cfl_subset(nx, ny, _ptr(m, C.c_uint8))
+    return m.reshape(ny, nx)
+
+
+def boundary(prob, q, T):
+    p = prob.cparams()
+    q = np.ascontiguousarray(q, np.float64)
+    o = np.zeros_like(q)
+    ins = [np.ascontiguousarray(q[k]) for k in range(3)]
+    outs = [np.ascontiguousarray(o[k]) for k in range(3)]
+    rc = lib().orc_boundary_f64(C.byref(p), *[_ptr(a, C.c_double) for a in ins],
+                                *[_ptr(a, C.c_double) for a in outs], float(T))
+    return np.stack(outs), rc
+
+
+def source(prob, q, dt):
+    p = prob.cparams()
+    q = np.ascontiguousarray(q, np.float64)
+    ins = [np.ascontiguousarray(q[k]) for k in range(3)]
+    outs = [np.zeros_like(ins[k]) for k in range(3)]
+    rc = lib().orc_source_f64(C.byref(p), *[_ptr(a, C.c_double) for a in ins],
+                              *[_ptr(a, C.c_double) for a in outs], float(dt))
+    return np.stack(outs), rc
+
+
+def sweep(prob, axis, q, dt):
+    p = prob.cparams()
+    qs = [np.ascontiguousarray(q[k], np.float64).copy() for k in range(3)]
+    cfl = np.zeros_like(qs[0])
+    rc = lib().orc_sweep_f64(C.byref(p), int(axis), *[_ptr(a, C.c_double) for a in qs],
+                             float(dt), _ptr(cfl, C.c_double))
+    return np.stack(qs), cfl, rc

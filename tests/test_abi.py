@@ -1,0 +1,70 @@
+"""The C-ABI library builds for gfx950, loads, and exports every entry point
+include/fivonagi.h declares.  No GPU calls (runs on CPU-only hosts)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import fivonagi_amd as fa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "fivonagi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fivonagi_[a-z_]+)\s*\(", src)))
+
+
+def test_header_declares_the_abi():
+    names = declared_functions()
+    assert set(fa.EXPORTS) == set(names), names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(fa.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), f"{name} not exported by {fa.LIB_PATH}"
+
+
+def test_library_has_gfx950_code_object():
+    data = open(fa.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """ctypes mirrors of fivonagi_params / fivonagi_info_t agree with the C
+    compiler's layout of include/fivonagi.h (sizes and every field offset)."""
+    import subprocess
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "fivonagi.h"', "int main(void){"]
+    for cname, cls in (("fivonagi_params", fa.Params), ("fivonagi_info_t", fa.Info)):
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                  check=True).stdout.splitlines())
+    for cname, cls in (("fivonagi_params", fa.Params), ("fivonagi_info_t", fa.Info)):
+        assert int(out[cname]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"
+
+
+def test_argument_errors_do_not_touch_the_gpu():
+    lib = fa.load()
+    h = ctypes.c_void_p()
+    # null params: rejected before any HIP call
+    assert lib.fivonagi_init(ctypes.byref(h), None, 1, None) == fa.ERR_ARG
+    assert b"null" in lib.fivonagi_last_error()
+    p = fa.Params()
+    p.nx, p.ny = 4, 4   # below the 8x8 minimum
+    arr = (ctypes.POINTER(ctypes.c_float) * 3)()
+    assert lib.fivonagi_init(ctypes.byref(h), ctypes.byref(p), 1, arr) == fa.ERR_ARG
+    assert lib.fivonagi_step(None, 1) == fa.ERR_ARG
+    assert lib.fivonagi_sync(None) == fa.ERR_ARG

@@ -1,0 +1,357 @@
+"""Pins of the CPU oracle (oracle/fvoracle.c) against facts the reference's
+mathematics fixes -- closed forms, identities, invariants and hand-computed
+tables -- chosen so that a plausible transcription slip (a dropped term, a
+wrong sign, index or operand) fails one of them.  The reference ships no test
+vectors of its own (it has no tests), so these are the pins.
+
+Which reference code each group pins:
+  Riemann solve      apps/na/fv.h:88-180 (x), :276-368 (y)
+  MC limiter         apps/na/fv.h:579-662
+  boundary rules     common/boundary.h:27-81, apps/hifu-beam/boundary.h:77-93,
+                     apps/taylor-angle-xy/boundary.h:38-55
+  source             apps/na/source.h:84-104
+  CFL fold           common/main.cu:179-212, :503-524
+  whole step         common/main.cu:310-635
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fivonagi_amd import workloads as wl
+from tests.oracle_bridge import to_oracle
+
+BETA = 3.5
+
+
+# ----------------------------------------------------------------- Riemann --
+
+def test_rest_state_speeds(oracle):
+    rc, r = oracle.riemann([0, 0, 0], [0, 0, 0], BETA)
+    assert rc == 0
+    np.testing.assert_array_equal(r["s"], [1.0, -1.0, 0.0])
+    assert not r["W"].any() and not r["amdq"].any() and not r["apdq"].any()
+
+
+@pytest.mark.parametrize("rho1", [-0.1, 0.01, 0.3])
+def test_still_medium_sound_speed(oracle, rho1):
+    # uniform density, no flow: L1,2 = +-sqrt(1 + 2(beta-1) rho'), L3 = 0
+    rc, r = oracle.riemann([rho1, 0, 0], [rho1, 0, 0], BETA)
+    c = math.sqrt(1 + 2 * (BETA - 1) * rho1)
+    np.testing.assert_allclose(r["s"], [c, -c, 0.0], rtol=1e-14, atol=1e-15)
+    assert not r["W"].any()   # uniform state -> no waves
+
+
+@pytest.mark.parametrize("U", [-0.6, 0.25, 0.9])
+def test_uniform_flow_speeds(oracle, U):
+    # rho = 1 moving at U: tmp3 = 1 - U^2, L = U +- sqrt(1 - U^2), L3 = U
+    rc, r = oracle.riemann([0, U, 0], [0, U, 0], BETA)
+    s = math.sqrt(1 - U * U)
+    np.testing.assert_allclose(r["s"], [U + s, U - s, U], rtol=1e-14, atol=1e-15)
+
+
+def test_supersonic_is_imaginary_eigenvalue(oracle):
+    # rho = 1 at U = 1.5: tmp3 = 1 - U^2 < 0 -> simError = 2 (fv.h:102)
+    rc, _ = oracle.riemann([0, 1.5, 0], [0, 1.5, 0], BETA)
+    assert rc == 2
+
+
+@pytest.mark.parametrize("v", [0.05, -0.2])
+def test_tangential_eigenvector_closed_form(oracle, v):
+    # rho' = 0, u = 0, v uniform: Z1 = 1, Z2 = 0, Z3 = v, L = +-1, so
+    # A1 = v/(1 - v), A2 = v/(1 + v): W31/W11 and W32/W12 (fv.h:110-113)
+    rc, r = oracle.riemann([0.0, 0.0, v], [0.001, 0.0, v * 1.001], BETA)
+    # tiny jump so the waves exist; Z's move by O(1e-3)
+    W = r["W"]
+    A1, A2 = W[2, 0] / W[0, 0], W[2, 1] / W[0, 1]
+    assert abs(A1 - v / (1 - v)) < 5e-3 * abs(v) and abs(A2 - v / (1 + v)) < 5e-3 * abs(v)
+    rc, r = oracle.riemann([0.0, 0.0, v], [0.0, 1e-9, v], BETA)
+    W = r["W"]
+    np.testing.assert_allclose(W[2, 0] / W[0, 0], v / (1 - v), rtol=1e-6)
+    np.testing.assert_allclose(W[2, 1] / W[0, 1], v / (1 + v), rtol=1e-6)
+
+
+def random_pair(rng):
+    qL = [rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05)]
+    qR = [rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05)]
+    return qL, qR
+
+
+def test_waves_sum_to_the_jump(oracle):
+    # sum_p W_p = q_R - q_L component-wise (fv.h:116-139 substituted)
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        qL, qR = random_pair(rng)
+        for form in (oracle.FORM_LITERAL, oracle.FORM_RHO1):
+            rc, r = oracle.riemann(qL, qR, BETA, form)
+            assert rc == 0
+            np.testing.assert_allclose(r["W"].sum(axis=1), np.subtract(qR, qL), rtol=1e-9,
+                                       atol=1e-15)
+
+
+def test_waves_solve_the_eigensystem(oracle):
+    # alpha solves [r1 r2 r3] alpha = dq with r_p = (1, L_p, A_p), r3 = (0,0,1):
+    # an independent linear solve of the same system
+    rng = np.random.default_rng(8)
+    for _ in range(100):
+        qL, qR = random_pair(rng)
+        rc, r = oracle.riemann(qL, qR, BETA)
+        L1, L2 = r["s"][0], r["s"][1]
+        W = r["W"]
+        a1, a2, a3 = W[0, 0], W[0, 1], W[2, 2]
+        A1, A2 = W[2, 0] / a1, W[2, 1] / a2
+        R = np.array([[1, 1, 0], [L1, L2, 0], [A1, A2, 1.0]])
+        alpha = np.linalg.solve(R, np.subtract(qR, qL))
+        np.testing.assert_allclose([a1, a2, a3], alpha, rtol=1e-8, atol=1e-14)
+        np.testing.assert_allclose(W[1, 0], a1 * L1, rtol=1e-14)
+        np.testing.assert_allclose(W[1, 1], a2 * L2, rtol=1e-14)
+        assert W[0, 2] == 0.0 and W[1, 2] == 0.0
+
+
+def test_fluctuations_route_by_sign(oracle):
+    rng = np.random.default_rng(9)
+    for _ in range(100):
+        qL, qR = random_pair(rng)
+        rc, r = oracle.riemann(qL, qR, BETA)
+        s, W = r["s"], r["W"]
+        am = sum(s[p] * W[:, p] for p in range(3) if s[p] < 0)
+        ap = sum(s[p] * W[:, p] for p in range(3) if s[p] > 0)
+        np.testing.assert_allclose(r["amdq"], am, rtol=1e-13, atol=1e-17)
+        np.testing.assert_allclose(r["apdq"], ap, rtol=1e-13, atol=1e-17)
+        # mass flux: amdq1 + apdq1 = jump of the normal momentum
+        np.testing.assert_allclose(r["amdq"][0] + r["apdq"][0], qR[1] - qL[1], rtol=1e-9,
+                                   atol=1e-16)
+
+
+def test_zero_speed_wave_goes_nowhere(oracle):
+    # u = 0 on both sides: L3 = 0; a tangential jump is all wave 3 and
+    # appears in neither fluctuation
+    rc, r = oracle.riemann([0, 0, 0.0], [0, 0, 0.01], BETA)
+    assert r["s"][2] == 0.0 and abs(r["W"][2, 2] - 0.01) < 1e-15
+    assert r["amdq"][2] == 0.0 and r["apdq"][2] == 0.0
+
+
+# ---------------------------------------------------------------- limiter --
+
+@pytest.mark.parametrize("r,phi", [(-1.0, 0.0), (0.0, 0.0), (0.2, 0.4), (1 / 3, 2 / 3),
+                                   (0.5, 0.75), (1.0, 1.0), (2.0, 1.5), (3.0, 2.0),
+                                   (10.0, 2.0)])
+def test_mc_limiter_table(oracle, r, phi):
+    # hand-computed MC values; wnorm2 = 1 so r = dot
+    assert oracle.philim(1.0, r, 99.0, +1.0) == pytest.approx(phi, abs=1e-15)   # s > 0 -> left
+    assert oracle.philim(1.0, 99.0, r, -1.0) == pytest.approx(phi, abs=1e-15)   # s < 0 -> right
+    assert oracle.philim(1.0, 99.0, r, 0.0) == pytest.approx(phi, abs=1e-15)    # s = 0 -> right
+
+
+def test_mc_limiter_vanishing_wave(oracle):
+    assert oracle.philim(0.0, 5.0, -5.0, 1.0) == 1.0
+
+
+# --------------------------------------------------------------- boundary --
+
+def ids(nx, ny):
+    q = np.zeros((3, ny, nx))
+    J, I = np.mgrid[0:ny, 0:nx]
+    q[0] = I + 1000.0 * J
+    q[1] = -(I + 1000.0 * J)
+    q[2] = 0.5 * (I + 1000.0 * J)
+    return q
+
+
+def test_zero_gradient_index_map(oracle):
+    nx, ny = 12, 10
+    p = wl.gauss_pulse(nx, ny)
+    out, rc = oracle.boundary(to_oracle(p), ids(nx, ny), 0.0)
+    src = out[0]
+    # (i, j) -> source cell, later rules of common/boundary.h win:
+    table = {
+        (5, 5): (5, 5),                       # interior untouched
+        (0, 5): (2, 5), (1, 5): (2, 5),       # left
+        (nx - 1, 5): (nx - 3, 5), (nx - 2, 5): (nx - 3, 5),   # right
+        (5, 0): (5, 2), (5, 1): (5, 2),       # bottom
+        (5, ny - 1): (5, ny - 3), (5, ny - 2): (5, ny - 3),   # top
+        (0, 0): (2, 0), (1, 1): (2, 1),       # left beats bottom, keeps its row
+        (0, ny - 1): (2, ny - 1),             # left beats top
+        (nx - 1, 0): (nx - 1, 2),             # bottom beats right
+        (nx - 2, ny - 1): (nx - 2, ny - 3),   # top beats right
+    }
+    for (i, j), (si, sj) in table.items():
+        assert src[j, i] == si + 1000.0 * sj, (i, j)
+    assert np.array_equal(out[1], -out[0]) and np.array_equal(out[2], 0.5 * out[0])
+
+
+def test_edge_drive_values(oracle):
+    nx, ny = 16, 12
+    amp = np.linspace(0.001, 0.02, ny)
+    ph = np.linspace(0.0, 1.0, ny)
+    p = wl.edge_drive(nx, ny, edge_amp=amp, edge_phase=ph)
+    T = 0.37
+    out, rc = oracle.boundary(to_oracle(p), ids(nx, ny), T)
+    pk = amp * np.sin(p.omega * T + ph)
+    for i in (0, 1):
+        np.testing.assert_allclose(out[0][:, i], pk, rtol=1e-15)
+        np.testing.assert_allclose(out[1][:, i], pk * (pk + 1), rtol=1e-15)
+        assert not out[2][:, i].any()
+    # the other edges stay zero-gradient
+    assert out[0][5, nx - 1] == (nx - 3) + 1000.0 * 5
+
+
+def test_taylor_ghosts_closed_form(oracle):
+    p = wl.taylor(1, theta=math.pi / 8)
+    t = p.taylor
+    T = 1.25
+    out, rc = oracle.boundary(to_oracle(p), np.zeros((3, p.ny, p.nx)), T)
+    J, I = np.mgrid[0:p.ny, 0:p.nx]
+    exact = t["rhoa"] * np.tanh(t["c"] * (I * p.dx * math.cos(t["theta"])
+                                          + ((J - t["jc"]) * p.dy + t["yoff"])
+                                          * math.sin(t["theta"]) - T))
+    ghost = (I <= 1) | (J <= 1) | (I >= p.nx - 2) | (J >= p.ny - 2)
+    np.testing.assert_allclose(out[0][ghost], exact[ghost], rtol=1e-14, atol=1e-22)
+    np.testing.assert_allclose(out[1][ghost], exact[ghost] * math.cos(t["theta"]), rtol=1e-14,
+                               atol=1e-22)
+    assert not out[0][~ghost].any()
+
+
+# ----------------------------------------------------------------- source --
+
+def test_source_quadratic_velocity_closed_form(oracle):
+    # rho' = 0, rho u = (i dx)^2 + (j dy)^2: the discrete Laplacian of u is
+    # exactly 2 + 2, so q2 gains 4*delta*dt in the interior; q1, q3 untouched
+    nx, ny = 20, 14
+    p = wl.gauss_pulse(nx, ny, delta=1e-3)
+    J, I = np.mgrid[0:ny, 0:nx]
+    q = np.zeros((3, ny, nx))
+    q[1] = (I * p.dx) ** 2 + (J * p.dy) ** 2
+    q[2] = 0.1
+    dt = 0.01
+    out, rc = oracle.source(to_oracle(p), q, dt)
+    inner = (slice(2, ny - 2), slice(2, nx - 2))
+    np.testing.assert_allclose(out[1][inner] - q[1][inner], 4 * p.delta * dt, rtol=1e-9)
+    assert np.array_equal(out[2], q[2]) and np.array_equal(out[0], q[0])
+    ring = np.ones((ny, nx), bool)
+    ring[inner] = False
+    assert np.array_equal(out[1][ring], q[1][ring])
+
+
+def test_source_divides_by_density(oracle):
+    # rho' = 1 (rho = 2) everywhere: velocity is q2/2
+    nx, ny = 12, 12
+    p = wl.gauss_pulse(nx, ny, delta=1e-3)
+    I = np.mgrid[0:ny, 0:nx][1]
+    q = np.zeros((3, ny, nx))
+    q[0] = 1.0
+    q[1] = 2.0 * (I * p.dx) ** 2
+    out, rc = oracle.source(to_oracle(p), q, 0.02)
+    np.testing.assert_allclose(out[1][2:-2, 2:-2] - q[1][2:-2, 2:-2], 2 * p.delta * 0.02,
+                               rtol=1e-9)
+
+
+# -------------------------------------------------------------- CFL fold --
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (128, 32), (4096, 1024), (16, 8)])
+def test_fold_subset_is_even_offsets_for_powers_of_two(oracle, nx, ny):
+    # every stride of the schedule is an even power of two, so an offset keeps
+    # its parity and reaches 0 exactly when it is even
+    m = oracle.cfl_subset(nx, ny).ravel()
+    np.testing.assert_array_equal(m, (np.arange(nx * ny) % 2 == 0).astype(np.uint8))
+
+
+def test_fold_subset_400(oracle):
+    m = oracle.cfl_subset(400, 400)
+    frac = m.mean()
+    assert 0.40 < frac < 0.42 and m[0, 0] == 1
+
+
+# ------------------------------------------------------------- whole step --
+
+def run(oracle, prob, n, form=None, dtype=np.float64):
+    op = to_oracle(prob, form=oracle.FORM_LITERAL if form is None else form)
+    q, st, rc = oracle.run(op, prob.q0.astype(np.float64), n, dtype=dtype)
+    assert rc == 0
+    return q, st
+
+
+def test_uniform_state_is_a_fixed_point(oracle):
+    p = wl.gauss_pulse(32, 24, amp=0.0)
+    q0 = np.zeros((3, 24, 32))
+    q0[0] = 0.01
+    p.q0 = q0.astype(np.float32)
+    q, st = run(oracle, p, 10)
+    assert np.array_equal(q, q0.astype(np.float32).astype(np.float64))
+
+
+def test_mass_conservation_away_from_edges(oracle):
+    # the density flux is exactly the momentum jump and the source leaves
+    # rho' alone, so sum(rho') only changes through edge interfaces
+    p = wl.gauss_pulse(160, 160, sigma_cells=4.0)
+    m0 = p.q0[0].astype(np.float64).sum()
+    q, st = run(oracle, p, 15)
+    assert np.abs(q[0][:4]).max() < 1e-20 and np.abs(q[0][:, :4]).max() < 1e-20
+    # rounding of ~4e5 cell updates; a wrong flux term moves it by O(1e-3)
+    assert abs(q[0].sum() - m0) < 1e-12 * abs(m0)
+
+
+def test_mirror_symmetry_small_amplitude(oracle):
+    # i <-> nx-1-i maps the scheme onto itself with u -> -u (exact up to the
+    # O(amplitude^2) asymmetry of the reference's A_k); at 1e-6 amplitude the
+    # residual is rounding-level
+    p = wl.gauss_pulse(64, 48, amp=1e-6, sigma_cells=4.0)
+    q, st = run(oracle, p, 15)
+    s = np.abs(q[0]).max()
+    assert np.abs(q[0] - q[0][:, ::-1]).max() < 1e-9 * s
+    assert np.abs(q[1] + q[1][:, ::-1]).max() < 1e-9 * s
+    assert np.abs(q[2] - q[2][:, ::-1]).max() < 1e-9 * s
+    assert np.abs(q[0] - q[0][::-1, :]).max() < 1e-9 * s
+
+
+def test_linear_wave_travels_at_c0(oracle):
+    # y-uniform right-going pulse (rho u = rho'), tiny amplitude: the centroid
+    # of rho' moves at c0 = 1
+    nx, ny = 400, 8
+    x = np.arange(nx)
+    r = 1e-6 * np.exp(-((x - 100.0) / 8.0) ** 2)
+    q0 = np.zeros((3, ny, nx))
+    q0[0] = r[None, :]
+    q0[1] = r[None, :]
+    p = wl.gauss_pulse(nx, ny)
+    p.q0 = q0
+    q, st = run(oracle, p, 100)
+    row = q[0][ny // 2]
+    cx = (row * x).sum() / row.sum()
+    moved = (cx - 100.0) / (1.0 / p.dx)     # in length units
+    assert abs(moved - st.T) < 0.02 * st.T
+
+
+def test_taylor_error_falls_with_resolution(oracle):
+    # the reference's own known-answer app (apps/taylor-angle-xy): the L1
+    # error against the closed form at the same time must fall at close to
+    # second order when the resolution doubles (eta = 10 -> 20 cells/unit)
+    errs = []
+    for eta in (2, 4):
+        p = wl.taylor(eta)
+        q, st = run(oracle, p, 20 * eta)
+        t = p.taylor
+        I = np.arange(p.nx)
+        exact = t["rhoa"] * np.tanh(t["c"] * (I * p.dx - st.T))
+        errs.append(np.abs(q[0][p.ny // 2] - exact).mean() / abs(t["rhoa"]))
+    assert errs[1] < 0.4 * errs[0], errs
+
+
+def test_literal_and_rho1_forms_agree_in_fp64(oracle):
+    p = wl.edge_drive(96, 64, halfwidth_cells=10.0)
+    a, _ = run(oracle, p, 60, form=oracle.FORM_LITERAL)
+    b, _ = run(oracle, p, 60, form=oracle.FORM_RHO1)
+    for k in range(3):
+        assert np.abs(a[k] - b[k]).max() <= 1e-10 * np.abs(a[k]).max()
+
+
+def test_rejection_is_a_clean_retry(oracle):
+    # a huge first dt is rejected and the state is left untouched; the run
+    # then matches one started from the dt the retry used
+    p = wl.gauss_pulse(48, 48, sigma_cells=4.0, dt0=1.0)
+    q, st = run(oracle, p, 5)
+    assert st.n_rejected >= 1 and st.n_accepted == 5
+    p2 = wl.gauss_pulse(48, 48, sigma_cells=4.0, dt0=1.0)
+    op = to_oracle(p2)
+    _, st1, _ = oracle.run(op, p2.q0.astype(np.float64), 0)   # no attempts
+    assert st1.n_attempts == 0
