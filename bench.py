@@ -91,11 +91,11 @@ def cpu_baseline(prob, seconds=12.0):
                     form=po.FORM_RHO1, omega=prob.omega, edge_amp=prob.edge_amp,
                     nthreads=threads)
     q = prob.q0.astype(np.float32)
-    # calibrate on one step, then time one bounded run of the same grid
+    # calibrate on a few steps, then time one bounded run of the same grid
     t0 = time.perf_counter()
-    q, st, rc = po.run(op, q, 1, dtype=np.float32, fast=True)
-    per_step = max(time.perf_counter() - t0, 1e-3)
-    steps = max(2, int(seconds / per_step))
+    q, st, rc = po.run(op, q, 5, dtype=np.float32, fast=True)
+    per_step = max((time.perf_counter() - t0) / 5, 1e-4)
+    steps = max(5, int(seconds / per_step))
     t0 = time.perf_counter()
     q, st, rc = po.run(op, q, steps, dtype=np.float32, state=st, fast=True)
     el = time.perf_counter() - t0
@@ -105,6 +105,26 @@ def cpu_baseline(prob, seconds=12.0):
     return {"value": value, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} accepted steps of the same {prob.nx}x{prob.ny} grid from the "
                       f"quiescent start, FP32 CPU oracle (-O3, OpenMP over rows), {el:.1f} s"}
+
+
+def copy_probe(nbytes=256 << 20, reps=5):
+    """A plain device-to-device copy of known size (the HIP runtime's copy
+    kernel), run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to calibrate
+    those counters: bytes the counters report per byte actually moved."""
+    hip = Hip()
+    lib = hip.lib
+    lib.hipMalloc.restype = C.c_int
+    lib.hipMemcpy.restype = C.c_int
+    lib.hipDeviceSynchronize.restype = C.c_int
+    a, b = C.c_void_p(), C.c_void_p()
+    hip.check(lib.hipMalloc(C.byref(a), C.c_size_t(nbytes)), "hipMalloc")
+    hip.check(lib.hipMalloc(C.byref(b), C.c_size_t(nbytes)), "hipMalloc")
+    lib.hipMemset.restype = C.c_int
+    hip.check(lib.hipMemset(a, 1, C.c_size_t(nbytes)), "hipMemset")
+    for _ in range(reps):
+        hip.check(lib.hipMemcpy(b, a, C.c_size_t(nbytes), 3), "hipMemcpy D2D")
+    hip.check(lib.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def main():
@@ -117,7 +137,12 @@ def main():
     ap.add_argument("--rows-per-wave", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain copy of known size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step "

@@ -16,6 +16,8 @@
 //     not lose the ~7 bits the reference loses to the "+1" (fv.h:62-63).
 //   * a3 = dt - (a1*A1 + a2*A2), which is fv.h:119-125 after substituting a1, a2.
 //   * 1/x through v_rcp_f32 / v_rsq_f32 (1 ulp); L1 - L2 = 2*sqrt(tmp3).
+//   * the fluctuations and the correction flux are folded per wave family
+//     (see limited_flux).
 //   * the two sweep updates (step + writeLimiters) are fused into one:
 //     q - dtd*((apdq - fadd)(k) + (amdq + fadd)(k+1)).
 #ifndef FIVONAGI_APPS_NA_SOLVER_H
@@ -39,7 +41,6 @@ struct Waves {
 struct Iface {
   Waves w;
   float L1, L2, L3;     // speeds s1, s2, s3
-  float amdq[3], apdq[3];
   float smax;           // max |s_p| (|L3| <= max(|L1|,|L2|) since L3 is their mean)
   bool bad;             // tmp3 < 0: imaginary eigenvalue (simError = 2)
 };
@@ -47,7 +48,8 @@ struct Iface {
 // Riemann solve at the interface between cell L and cell R.
 // r1 = rho', n = normal momentum, t = tangential momentum,
 // un/ut = n/rho, t/rho of the same cell (precomputed once per cell).
-// beta2m2 = 2*(BETA - 1).
+// beta2m2 = 2*(BETA - 1).  The fluctuations amdq/apdq (fv.h:141-180) are not
+// formed here: limited_flux() folds them with the correction flux.
 __device__ __forceinline__ void riemann(float r1L, float nL, float tL, float unL,
                                         float utL, float r1R, float nR, float tR,
                                         float unR, float utR, float beta2m2,
@@ -65,34 +67,28 @@ __device__ __forceinline__ void riemann(float r1L, float nL, float tL, float unL
   const float L1 = tmp1 + sq, L2 = tmp1 - sq;       // fv.h:105-107
   f.L1 = L1; f.L2 = L2; f.L3 = tmp1;
   const float t2 = 2.0f * Z2 * Z3;
-  const float A1 = (L1 * Z3 - t2) * rcp(fmaf(L1, Z1, -Z3));   // fv.h:110-113
-  const float A2 = (L2 * Z3 - t2) * rcp(fmaf(L2, Z1, -Z3));
+  // fv.h:110-113 with one reciprocal for both denominators (each is ~ +-1)
+  const float d1 = fmaf(L1, Z1, -Z3), d2 = fmaf(L2, Z1, -Z3);
+  const float rd = rcp(d1 * d2);
+  const float A1 = fmaf(L1, Z3, -t2) * (d2 * rd);
+  const float A2 = fmaf(L2, Z3, -t2) * (d1 * rd);
   const float drho = r1R - r1L, dn = nR - nL, dtg = tR - tL;
   const float a1 = fmaf(-drho, L2, dn) * (0.5f * rs);         // fv.h:116-117
   const float a2 = drho - a1;                                   // fv.h:118
   f.w.a1 = a1; f.w.w21 = a1 * L1; f.w.w31 = a1 * A1;
   f.w.a2 = a2; f.w.w22 = a2 * L2; f.w.w32 = a2 * A2;
-  const float a3 = dtg - (f.w.w31 + f.w.w32);                   // fv.h:119-125
-  f.w.a3 = a3;
-  // fv.h:141-180: left-going (amdq) and right-going (apdq) fluctuations;
-  // a wave with s == 0 goes to neither (min/max give an exact +0 term).
-  const float m1 = fminf(L1, 0.0f), m2 = fminf(L2, 0.0f), m3 = fminf(tmp1, 0.0f);
-  const float p1 = fmaxf(L1, 0.0f), p2 = fmaxf(L2, 0.0f), p3 = fmaxf(tmp1, 0.0f);
-  f.amdq[0] = fmaf(m1, a1, m2 * a2);
-  f.amdq[1] = fmaf(m1, f.w.w21, m2 * f.w.w22);
-  f.amdq[2] = fmaf(m3, a3, fmaf(m1, f.w.w31, m2 * f.w.w32));
-  f.apdq[0] = fmaf(p1, a1, p2 * a2);
-  f.apdq[1] = fmaf(p1, f.w.w21, p2 * f.w.w22);
-  f.apdq[2] = fmaf(p3, a3, fmaf(p1, f.w.w31, p2 * f.w.w32));
+  f.w.a3 = dtg - (f.w.w31 + f.w.w32);                           // fv.h:119-125
   f.smax = fmaxf(fabsf(L1), fabsf(L2));
 }
 
-// MC limiter phi(r) = max(0, min((1+r)/2, 2, 2r)); phi = 1 where the wave
-// vanishes (fv.h:587-605).
+// MC limiter phi(r) = max(0, min((1+r)/2, 2, 2r)) (fv.h:587-605).  The
+// reference sets phi = 1 where wnorm2 == 0; such a wave is the zero vector,
+// so its correction phi*W is 0 for any finite phi.  Here wnorm2 == 0 gives
+// r = dot*inf (NaN or inf), which fminf/fmaxf turn into phi = 2 (finite), and
+// the select is skipped.
 __device__ __forceinline__ float philim(float wnorm2, float dot) {
   const float r = dot * rcp(wnorm2);
-  const float phi = fmaxf(fminf(fminf(fmaf(0.5f, r, 0.5f), 2.0f), 2.0f * r), 0.0f);
-  return (wnorm2 != 0.0f) ? phi : 1.0f;
+  return fmaxf(fminf(fminf(fmaf(0.5f, r, 0.5f), 2.0f), 2.0f * r), 0.0f);
 }
 
 __device__ __forceinline__ float dot1(const Waves& a, const Waves& b) {
@@ -102,9 +98,12 @@ __device__ __forceinline__ float dot2(const Waves& a, const Waves& b) {
   return fmaf(a.a2, b.a2, fmaf(a.w22, b.w22, a.w32 * b.w32));
 }
 
-// Correction flux fadd (fv.h:666-711) at one interface from its limiter
-// values, folded into P = apdq - fadd (applied to the cell on the + side)
-// and M = amdq + fadd (applied to the cell on the - side).
+// Fluctuations (fv.h:141-180) and correction flux fadd (fv.h:666-711) of one
+// interface, folded into what the two adjacent cells need:
+//   P = apdq - fadd = sum_p (max(s_p,0) - c_p) W_p   (cell on the + side)
+//   M = amdq + fadd = sum_p (min(s_p,0) + c_p) W_p   (cell on the - side)
+// with c_p = |s_p| (1 - |s_p| dt/d) phi_p / 2.  A wave with s_p = 0 enters
+// neither fluctuation, as in the reference.
 // dotl_p = W_p . W_p(k-1), dotr_p = W_p . W_p(k+1); upwind by sign(s_p).
 __device__ __forceinline__ void limited_flux(const Iface& f, float dotl1, float dotr1,
                                              float dotl2, float dotr2, float dotl3,
@@ -117,16 +116,19 @@ __device__ __forceinline__ void limited_flux(const Iface& f, float dotl1, float 
   const float phi1 = philim(n1, f.L1 > 0.0f ? dotl1 : dotr1);
   const float phi2 = philim(n2, f.L2 > 0.0f ? dotl2 : dotr2);
   const float phi3 = philim(n3, f.L3 > 0.0f ? dotl3 : dotr3);
-  // 0.5*|s|*(1 - |s|*dtd)*phi
   const float s1 = fabsf(f.L1), s2 = fabsf(f.L2), s3 = fabsf(f.L3);
   const float c1 = 0.5f * s1 * fmaf(-s1, dtd, 1.0f) * phi1;
   const float c2 = 0.5f * s2 * fmaf(-s2, dtd, 1.0f) * phi2;
   const float c3 = 0.5f * s3 * fmaf(-s3, dtd, 1.0f) * phi3;
-  const float fa0 = fmaf(c1, w.a1, c2 * w.a2);
-  const float fa1 = fmaf(c1, w.w21, c2 * w.w22);
-  const float fa2 = fmaf(c3, w.a3, fmaf(c1, w.w31, c2 * w.w32));
-  P[0] = f.apdq[0] - fa0; P[1] = f.apdq[1] - fa1; P[2] = f.apdq[2] - fa2;
-  M[0] = f.amdq[0] + fa0; M[1] = f.amdq[1] + fa1; M[2] = f.amdq[2] + fa2;
+  const float al1 = fmaxf(f.L1, 0.0f) - c1, be1 = fminf(f.L1, 0.0f) + c1;
+  const float al2 = fmaxf(f.L2, 0.0f) - c2, be2 = fminf(f.L2, 0.0f) + c2;
+  const float al3 = fmaxf(f.L3, 0.0f) - c3, be3 = fminf(f.L3, 0.0f) + c3;
+  P[0] = fmaf(al1, w.a1, al2 * w.a2);
+  P[1] = fmaf(al1, w.w21, al2 * w.w22);
+  P[2] = fmaf(al3, w.a3, fmaf(al1, w.w31, al2 * w.w32));
+  M[0] = fmaf(be1, w.a1, be2 * w.a2);
+  M[1] = fmaf(be1, w.w21, be2 * w.w22);
+  M[2] = fmaf(be3, w.a3, fmaf(be1, w.w31, be2 * w.w32));
 }
 
 }  // namespace fvna

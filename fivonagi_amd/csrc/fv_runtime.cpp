@@ -20,6 +20,9 @@
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      int n_items, hipStream_t stream);
+extern "C" hipError_t fv_step_waves_per_cu(int* waves);
+extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
+                                              int max_ring, hipStream_t stream);
 
 namespace {
 
@@ -79,6 +82,7 @@ struct fivonagi_ctx {
   fv::WorkItem* d_items = nullptr;
   fv::Ctl* d_ctl = nullptr;
   int graph_len = 0;
+  int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
   bool unlimited = false;                  // targets set to "run every attempt"
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -106,20 +110,28 @@ struct fivonagi_ctx {
 
 namespace {
 
-constexpr int kTargetWaves = 2048;   // ~2 waves per SIMD on a 256-CU part
-
-// Rows per work item: the smallest height whose total wave count over all
-// instances fits kTargetWaves (fewer overlapping halo rows), never below 16.
-int auto_rows(const std::vector<fivonagi_params>& ps) {
-  for (int H = 16; H <= 4096; ++H) {
+// Rows per work item: the smallest height (never below 12) whose total wave
+// count over all instances fits one resident round of the device
+// (CUs x resident step waves per CU): every wave then starts at once, and
+// taller work items repeat fewer halo rows.
+int auto_rows(const std::vector<fivonagi_params>& ps, long target_waves) {
+  for (int H = 12; H <= 4096; ++H) {
     long waves = 0;
     for (const auto& p : ps) {
       const long strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
       waves += strips * std::max(1L, std::lround((double)p.ny / H));
     }
-    if (waves <= kTargetWaves) return H;
+    if (waves <= target_waves) return H;
   }
   return 4096;
+}
+
+long resident_waves(int device) {
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      fv_step_waves_per_cu(&per_cu) != hipSuccess || cus <= 0 || per_cu <= 0)
+    return 3072;   // 256 CUs x 12 waves
+  return (long)cus * per_cu;
 }
 
 void build_items(fivonagi_ctx* c, int auto_h) {
@@ -171,13 +183,23 @@ int read_ctls(fivonagi_ctx* c, std::vector<fv::Ctl>& out) {
   return 0;
 }
 
+// One step attempt: the Taylor ghost pre-pass (only if a Taylor instance
+// exists) and the fused step.
+hipError_t launch_one(fivonagi_ctx* c) {
+  if (c->taylor_ring > 0) {
+    hipError_t e = fv_launch_taylor_ghosts(c->d_desc, c->n_inst, c->taylor_ring, c->stream);
+    if (e != hipSuccess) return e;
+  }
+  return fv_launch_step(c->d_desc, c->d_items, (int)c->items.size(), c->stream);
+}
+
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
-  const int G = 32;   // launches per captured graph
+  const int G = 32;   // attempts per captured graph
   if (n >= G) {
     if (!c->graph_exec) {
       HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < G; ++k) {
-        hipError_t e = fv_launch_step(c->d_desc, c->d_items, (int)c->items.size(), c->stream);
+        hipError_t e = launch_one(c);
         if (e != hipSuccess) {
           hipGraph_t g = nullptr;
           (void)hipStreamEndCapture(c->stream, &g);
@@ -195,7 +217,7 @@ int launch_attempts(fivonagi_ctx* c, int64_t n) {
     }
   }
   for (int64_t k = 0; k < n; ++k) {
-    hipError_t e = fv_launch_step(c->d_desc, c->d_items, (int)c->items.size(), c->stream);
+    hipError_t e = launch_one(c);
     if (e != hipSuccess) return hip_fail(e, "fv_launch_step");
   }
   return 0;
@@ -318,6 +340,8 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     d.tay_cos = std::cos(q.tay_theta); d.tay_sin = std::sin(q.tay_theta);
     d.tay_yoff = q.tay_yoff; d.tay_jc = q.tay_jc;
     d.nx = q.nx; d.ny = q.ny; d.bc = q.bc;
+    if (q.bc == FIVONAGI_BC_TAYLOR)
+      c->taylor_ring = std::max(c->taylor_ring, 4 * q.nx + 4 * (q.ny - 4));
     d.beta2m2 = (float)(2.0 * (q.beta - 1.0));
 
     auto& t = ctl[k];
@@ -330,7 +354,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * n_inst, hipMemcpyHostToDevice);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
 
-  build_items(c, auto_rows(c->params));
+  build_items(c, auto_rows(c->params, resident_waves(c->device)));
   c->d_desc = (fv::InstDesc*)c->dalloc(sizeof(fv::InstDesc) * n_inst, &e);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc desc"); }
   c->d_items = (fv::WorkItem*)c->dalloc(sizeof(fv::WorkItem) * c->items.size(), &e);

@@ -44,13 +44,13 @@ typedef __attribute__((address_space(1))) const uint32_t cguint;
 // DPP cross-lane moves over the whole wave64 (gfx9 wave_shr:1 / wave_shl:1).
 // shr1: lane l receives lane l-1's value (lane 0 receives 0).
 // shl1: lane l receives lane l+1's value (lane 63 receives 0).
+// bound_ctrl = 1: the missing source lane reads as 0, so no "old" value has
+// to be materialised first.
 __device__ __forceinline__ float shr1(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float shl1(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -76,6 +76,7 @@ struct Row {         // one cell of a row as the y sweep sees it
 
 struct YI {          // y interface with its limited flux towards the + cell
   fvna::Iface f;
+  float dl[3];       // W_p . W_p of the interface below (one per wave family)
   float P[3];        // apdq - fadd, in (density, normal, tangential) order
 };
 
@@ -133,7 +134,8 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
 
   // Ghost-cell source columns (apps/common/boundary.h): colA on rows
   // 2..ny-3, colB on ghost rows (top/bottom rules beat the right rule; the
-  // left rule beats everything and keeps its own row).
+  // left rule beats everything and keeps its own row).  Taylor ghost cells
+  // were written into the input by fv_taylor_ghost_kernel: plain copies here.
   const bool taylor = (bc == kBcTaylor);
   const bool left = !taylor && (ic <= 1);
   const int colA = taylor ? ic : (left ? 2 : (ic >= nx - 2 ? nx - 3 : ic));
@@ -159,10 +161,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
       q.q1 = drive_lane ? a : q.q1;
       q.q2 = drive_lane ? b : q.q2;
       q.q3 = drive_lane ? c : q.q3;
-    } else if (taylor) {
-      if (fvbc::is_ghost(ic, rc, nx, ny))
-        fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx,
-                     D.dy, Tnow, ic, rc, q.q1, q.q2, q.q3);
     }
     const float ir = fvna::rcp(1.0f + q.q1);
     q.un = q.q3 * ir;
@@ -194,16 +192,17 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
                   beta2m2, fn.f);
     err |= fn.f.bad && col_in && (r >= 1) && (r < ny);
     if (cfl_col && r >= y0 && r < y1 && r >= 1 && counted(r)) Sy = fmaxf(Sy, fn.f.smax);
+
+    // W(r).W(r-1): the left product of interface r, the right one of r-1
+    fn.dl[0] = fvna::dot1(fn.f.w, fc.f.w);
+    fn.dl[1] = fvna::dot2(fn.f.w, fc.f.w);
+    fn.dl[2] = fn.f.w.a3 * fc.f.w.a3;
     if (r < y0) return;
 
     // ---- y limiter at interface r-1 ----
     float M[3];
-    {
-      const float dl1 = fvna::dot1(fc.f.w, fl.f.w), dr1 = fvna::dot1(fc.f.w, fn.f.w);
-      const float dl2 = fvna::dot2(fc.f.w, fl.f.w), dr2 = fvna::dot2(fc.f.w, fn.f.w);
-      const float dl3 = fc.f.w.a3 * fl.f.w.a3, dr3 = fc.f.w.a3 * fn.f.w.a3;
-      fvna::limited_flux(fc.f, dl1, dr1, dl2, dr2, dl3, dr3, dty, fc.P, M);
-    }
+    fvna::limited_flux(fc.f, fc.dl[0], fn.dl[0], fc.dl[1], fn.dl[1], fc.dl[2], fn.dl[2], dty,
+                       fc.P, M);
     if (r < y0 + 1) return;
 
     // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2)
@@ -341,7 +340,57 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   ctl->drive_c = cos(th);
 }
 
+// Closed-form ghost ring of the Taylor-shock instances (apps/taylor-angle-xy/
+// boundary.h:38-55), written into the current input buffer before the step
+// reads it.  The values depend on the cell and on T only, never on the state,
+// so writing them in place is the reference's boundary kernel.  blockIdx.y =
+// instance; the ring (2 cells wide) is enumerated by a flat index.
+__global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __restrict__ insts) {
+  const InstDesc& D = insts[blockIdx.y];
+  if (D.bc != kBcTaylor) return;
+  const Ctl* ctl = D.ctl;
+  if (ctl->err != 0 || ctl->n_acc >= ctl->target_acc) return;
+  const int nx = D.nx, ny = D.ny;
+  const int ring = 4 * nx + 4 * (ny - 4);
+  const double T = ctl->T;
+  const int cur = ctl->cur;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ring; k += gridDim.x * blockDim.x) {
+    int i, j;
+    if (k < 4 * nx) {            // rows 0, 1, ny-2, ny-1
+      const int rr = k / nx;
+      i = k - rr * nx;
+      j = rr < 2 ? rr : ny - 4 + rr;
+    } else {                     // columns 0, 1, nx-2, nx-1 of rows 2..ny-3
+      const int kk = k - 4 * nx;
+      const int cc = kk / (ny - 4);
+      j = 2 + (kk - cc * (ny - 4));
+      i = cc < 2 ? cc : nx - 4 + cc;
+    }
+    float a, b, c;
+    fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx, D.dy,
+                 T, i, j, a, b, c);
+    const size_t o = (size_t)j * (size_t)nx + (size_t)i;
+    D.q[cur][0][o] = a;
+    D.q[cur][1][o] = b;
+    D.q[cur][2][o] = c;
+  }
+}
+
 }  // namespace fv
+
+extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
+                                              int max_ring, hipStream_t stream) {
+  if (n_inst <= 0) return hipSuccess;
+  const int blocks = (max_ring + 255) / 256;
+  hipLaunchKernelGGL(fv::fv_taylor_ghost_kernel, dim3(blocks, n_inst), dim3(256), 0, stream,
+                     insts);
+  return hipGetLastError();
+}
+
+// Resident step waves per CU (occupancy of the step kernel on this device).
+extern "C" hipError_t fv_step_waves_per_cu(int* waves) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(waves, fv::fv_step_kernel, fv::kLanes, 0);
+}
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      int n_items, hipStream_t stream) {
