@@ -100,6 +100,24 @@ int orc_source_f64(const orc_params *p, const double *in1, const double *in2,
 int orc_sweep_f64(const orc_params *p, int axis, double *q1, double *q2,
                   double *q3, double dt, double *cfl);
 
+/* ---- adjoint (fvadjoint.c) ------------------------------------------- */
+/* objective box [x0, x0+nx) x [y0, y0+ny) */
+typedef struct { int32_t x0, y0, nx, ny; } orc_box;
+
+/* One step of the line-per-sweep formulation used by the adjoint's forward
+   recompute (same arithmetic, rho'-direct), for checking it against orc_run. */
+int orc_line_step_f64(const orc_params *p, double *q1, double *q2, double *q3, double T,
+                      double dt);
+/* J = sum_n dt_n sum_box rho'_{n+1}^2 over nsteps steps with the prescribed
+   dt sequence p->dt_seq (frozen dt, no rejection); gradients with respect to
+   edge_amp[j], edge_phase[j] (ny each) and the initial state (3 grids).
+   Any gradient pointer may be NULL. */
+int orc_adjoint_f64(const orc_params *p, const double *q0, int64_t nsteps, const orc_box *box,
+                    double *J, double *grad_amp, double *grad_phase, double *grad_q0);
+/* J of the oracle forward (orc_run_f64 step by step) for finite differences */
+int orc_objective_f64(const orc_params *p, const double *q0, int64_t nsteps,
+                      const orc_box *box, double *J);
+
 #ifdef __cplusplus
 }
 #endif

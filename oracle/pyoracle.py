@@ -42,6 +42,10 @@ class State(C.Structure):
     ]
 
 
+class Box(C.Structure):
+    _fields_ = [("x0", C.c_int32), ("y0", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32)]
+
+
 def _load(name):
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
@@ -68,6 +72,13 @@ def _load(name):
     lib.orc_source_f64.restype = C.c_int
     lib.orc_sweep_f64.argtypes = [P, C.c_int, dp, dp, dp, C.c_double, dp]
     lib.orc_sweep_f64.restype = C.c_int
+    B = C.POINTER(Box)
+    lib.orc_line_step_f64.argtypes = [P, dp, dp, dp, C.c_double, C.c_double]
+    lib.orc_line_step_f64.restype = C.c_int
+    lib.orc_adjoint_f64.argtypes = [P, dp, C.c_int64, B, dp, dp, dp, dp]
+    lib.orc_adjoint_f64.restype = C.c_int
+    lib.orc_objective_f64.argtypes = [P, dp, C.c_int64, B, dp]
+    lib.orc_objective_f64.restype = C.c_int
     return lib
 
 
@@ -193,3 +204,35 @@ def sweep(prob, axis, q, dt):
     rc = lib().orc_sweep_f64(C.byref(p), int(axis), *[_ptr(a, C.c_double) for a in qs],
                              float(dt), _ptr(cfl, C.c_double))
     return np.stack(qs), cfl, rc
+
+
+def line_step(prob, q, T, dt):
+    p = prob.cparams()
+    qs = [np.ascontiguousarray(q[k], np.float64).copy() for k in range(3)]
+    rc = lib().orc_line_step_f64(C.byref(p), *[_ptr(a, C.c_double) for a in qs], float(T),
+                                 float(dt))
+    return np.stack(qs), rc
+
+
+def objective(prob, q0, nsteps, box):
+    """J = sum_n dt_n sum_box rho'^2 of the oracle forward (prescribed dt)."""
+    p = prob.cparams()
+    q = np.ascontiguousarray(q0, np.float64)
+    J = C.c_double()
+    rc = lib().orc_objective_f64(C.byref(p), _ptr(q, C.c_double), int(nsteps),
+                                 C.byref(Box(*box)), C.byref(J))
+    return J.value, rc
+
+
+def adjoint(prob, q0, nsteps, box):
+    """(J, dJ/d edge_amp, dJ/d edge_phase, dJ/dq0) with frozen (prescribed) dt."""
+    p = prob.cparams()
+    q = np.ascontiguousarray(q0, np.float64)
+    J = C.c_double()
+    ga = np.zeros(prob.ny)
+    gp = np.zeros(prob.ny)
+    gq = np.zeros((3, prob.ny, prob.nx))
+    rc = lib().orc_adjoint_f64(C.byref(p), _ptr(q, C.c_double), int(nsteps), C.byref(Box(*box)),
+                               C.byref(J), _ptr(ga, C.c_double), _ptr(gp, C.c_double),
+                               _ptr(gq, C.c_double))
+    return J.value, ga, gp, gq, rc
