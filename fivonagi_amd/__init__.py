@@ -19,7 +19,8 @@ OK, ERR_EIGEN, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 8, 9, 10
 EXPORTS = (
     "fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence", "fivonagi_step",
     "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect", "fivonagi_info",
-    "fivonagi_destroy", "fivonagi_last_error",
+    "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_destroy",
+    "fivonagi_last_error",
 )
 
 
@@ -50,6 +51,17 @@ class Info(C.Structure):
     ]
 
 
+class Objective(C.Structure):
+    """fivonagi_objective (include/fivonagi.h): the box of J."""
+    _fields_ = [("box_x0", C.c_int32), ("box_y0", C.c_int32), ("box_nx", C.c_int32),
+                ("box_ny", C.c_int32)]
+
+
+class AdjointOpts(C.Structure):
+    """fivonagi_adjoint_opts (include/fivonagi.h)."""
+    _fields_ = [("checkpoint_every", C.c_int32), ("plain_forward", C.c_int32)]
+
+
 _lib = None
 
 
@@ -76,13 +88,16 @@ def load():
     ip = C.POINTER(C.c_int64)
     lib.fivonagi_collect.argtypes = [vp, C.c_int, fp, fp, fp, dp, dp, ip, ip]
     lib.fivonagi_info.argtypes = [vp, C.c_int, C.POINTER(Info)]
+    lib.fivonagi_step_adjoint.argtypes = [vp, C.c_int64, C.POINTER(Objective),
+                                          C.POINTER(AdjointOpts), dp, dp, dp]
+    lib.fivonagi_adjoint_state.argtypes = [vp, C.c_int, fp, fp, fp]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
     lib.fivonagi_last_error.argtypes = []
     lib.fivonagi_last_error.restype = C.c_char_p
     for name in ("fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
                  "fivonagi_step", "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect",
-                 "fivonagi_info"):
+                 "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -215,3 +230,35 @@ class Context:
         i = Info()
         _check(load().fivonagi_info(self._h, inst, C.byref(i)))
         return {k: getattr(i, k) for k, _ in Info._fields_}
+
+    def step_adjoint(self, nsteps, boxes, checkpoint_every=0, plain_forward=False):
+        """Forward nsteps (recording dt), then the adjoint.  boxes: one
+        (x0, y0, nx, ny) per instance.  Returns (J, grad_amp, grad_phase) with
+        one entry per instance (arrays of ny)."""
+        n = len(self.problems)
+        if isinstance(boxes, tuple):
+            boxes = [boxes] * n
+        objs = (Objective * n)(*[Objective(*b) for b in boxes])
+        opts = AdjointOpts(int(checkpoint_every), int(bool(plain_forward)))
+        J = np.zeros(n)
+        tot = sum(p.ny for p in self.problems)
+        ga = np.zeros(tot)
+        gp = np.zeros(tot)
+        dp_ = C.POINTER(C.c_double)
+        _check(load().fivonagi_step_adjoint(self._h, int(nsteps), objs, C.byref(opts),
+                                            J.ctypes.data_as(dp_), ga.ctypes.data_as(dp_),
+                                            gp.ctypes.data_as(dp_)))
+        outa, outp, off = [], [], 0
+        for p in self.problems:
+            outa.append(ga[off:off + p.ny].copy())
+            outp.append(gp[off:off + p.ny].copy())
+            off += p.ny
+        return J, outa, outp
+
+    def adjoint_state(self, inst=0):
+        """dJ/d(state at the start of the last step_adjoint call), (3, ny, nx)."""
+        p = self.problems[inst]
+        ls = [np.zeros((p.ny, p.nx), np.float32) for _ in range(3)]
+        _check(load().fivonagi_adjoint_state(
+            self._h, inst, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in ls]))
+        return np.stack(ls)

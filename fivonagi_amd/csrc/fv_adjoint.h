@@ -1,0 +1,41 @@
+// fv_adjoint.h -- per-instance table of the adjoint passes (device memory),
+// shared by fv_adjoint.hip and the host runtime.
+#ifndef FIVONAGI_FV_ADJOINT_H
+#define FIVONAGI_FV_ADJOINT_H
+
+#include <stdint.h>
+
+namespace fv {
+
+struct AdjDesc {
+  int32_t nx, ny, bc;
+  float beta;
+  double dx, dy, delta, omega;
+  double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff;
+  int32_t tay_jc;
+  int32_t bx0, by0, bnx, bny;          // objective box
+  int32_t pad;
+  const float* edge_amp;               // ny drive amplitudes
+  const float* edge_phase;             // ny drive phases
+  float* seg;                          // (K+1) states of the current segment, 3 grids each
+  float* lam;                          // adjoint of the state (3 grids)
+  float* lam2;                         // scratch adjoints (3 grids each)
+  float* lamx;
+  float* qb;                           // recomputed q_b, q_y, q_xy (3 grids each)
+  float* qy;
+  float* qxy;
+  float* iw;                           // per interface: W (7), s (3)
+  float* pm;                           // per interface: P (3), M (3)
+  float* ig;                           // per interface: gW (7), gs (3), gdot (3)
+  int32_t* inb;                        // per interface: upwind side bits of the 3 families
+  float* iq;                           // per interface: gL (3), gR (3)
+  double* grad_amp;                    // ny, accumulated
+  double* grad_phase;                  // ny, accumulated
+  double* jstep;                       // per step: dt_n sum_box rho'_{n+1}^2
+  const double* dt_log;                // dt of accepted step n of the run
+  const double* T_log;                 // T at the start of accepted step n
+};
+
+}  // namespace fv
+
+#endif  // FIVONAGI_FV_ADJOINT_H

@@ -16,11 +16,16 @@
 #include <vector>
 
 #include "../../include/fivonagi.h"
+#include "fv_adjoint.h"
 #include "fv_types.h"
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      int n_items, hipStream_t stream);
 extern "C" hipError_t fv_step_waves_per_cu(int* waves);
+extern "C" hipError_t fv_adj_forward_step(const fv::AdjDesc* ds, int n_inst, int max_cells, int k,
+                                          int slot, int store_next, hipStream_t s);
+extern "C" hipError_t fv_adj_reverse_step(const fv::AdjDesc* ds, int n_inst, int max_cells,
+                                          int max_ny, int k, int slot, hipStream_t s);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, hipStream_t stream);
 
@@ -86,6 +91,7 @@ struct fivonagi_ctx {
   bool unlimited = false;                  // targets set to "run every attempt"
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  std::vector<float*> adj_lam;             // dJ/dq0 of the last adjoint call (per instance)
 
   void* dalloc(size_t bytes, hipError_t* e) {
     void* p = nullptr;
@@ -102,6 +108,7 @@ struct fivonagi_ctx {
   }
   ~fivonagi_ctx() {
     drop_graph();
+    for (float* q : adj_lam) (void)hipFree(q);
     if (stream) (void)hipStreamSynchronize(stream);
     for (void* p : allocs) (void)hipFree(p);
     if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -490,6 +497,212 @@ int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
   for (const auto& w : c->items)
     if (w.inst == inst) { h = w.y1 - w.y0; break; }
   info->rows_per_wave = h;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// device allocations that live for one adjoint call
+struct AdjScratch {
+  std::vector<void*> p;
+  hipError_t e = hipSuccess;
+  template <class T>
+  T* get(size_t count) {
+    if (e != hipSuccess) return nullptr;
+    void* q = nullptr;
+    e = hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1));
+    if (e == hipSuccess) p.push_back(q);
+    return (T*)q;
+  }
+  ~AdjScratch() {
+    for (void* q : p) (void)hipFree(q);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_objective* obj,
+                          const fivonagi_adjoint_opts* opts, double* J, double* grad_amp,
+                          double* grad_phase) {
+  if (!c || !obj || !J || nsteps <= 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
+  if (nsteps > (1 << 30)) return fail(FIVONAGI_ERR_ARG, "too many steps");
+  const int ni = c->n_inst;
+  for (int k = 0; k < ni; ++k) {
+    const auto& p = c->params[k];
+    const auto& b = obj[k];
+    if (b.box_x0 < 0 || b.box_y0 < 0 || b.box_nx <= 0 || b.box_ny <= 0 ||
+        b.box_x0 + b.box_nx > p.nx || b.box_y0 + b.box_ny > p.ny)
+      return fail(FIVONAGI_ERR_ARG, "objective box outside the grid");
+  }
+  int K = opts && opts->checkpoint_every > 0 ? opts->checkpoint_every
+                                             : (int)std::ceil(std::sqrt((double)nsteps));
+  K = (int)std::min<int64_t>(K, nsteps);
+  const int64_t ncp = (nsteps + K - 1) / K;
+  const bool plain = opts && opts->plain_forward;
+
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  for (int k = 0; k < ni; ++k)
+    if (ctl[k].err) return fivonagi_sync(c);
+
+  AdjScratch sc;
+  std::vector<fv::AdjDesc> ad(ni);
+  std::vector<float*> cps(ni);
+  std::vector<double*> dlog(ni), tlog(ni);
+  int max_cells = 0, max_ny = 0;
+  for (int k = 0; k < ni; ++k) {
+    const auto& p = c->params[k];
+    const size_t n = (size_t)p.nx * p.ny;
+    max_cells = std::max(max_cells, (int)n);
+    max_ny = std::max(max_ny, p.ny);
+    auto& a = ad[k];
+    std::memset(&a, 0, sizeof(a));
+    a.nx = p.nx; a.ny = p.ny; a.bc = p.bc; a.beta = (float)p.beta;
+    a.dx = p.dx; a.dy = p.dy; a.delta = p.delta; a.omega = p.omega;
+    a.tay_rhoa = p.tay_rhoa; a.tay_c = p.tay_c;
+    a.tay_cos = std::cos(p.tay_theta); a.tay_sin = std::sin(p.tay_theta);
+    a.tay_yoff = p.tay_yoff; a.tay_jc = p.tay_jc;
+    a.bx0 = obj[k].box_x0; a.by0 = obj[k].box_y0; a.bnx = obj[k].box_nx; a.bny = obj[k].box_ny;
+    float* ea = sc.get<float>(2 * (size_t)p.ny);
+    a.seg = sc.get<float>((size_t)(K + 1) * 3 * n);
+    a.lam = sc.get<float>(3 * n);
+    a.lam2 = sc.get<float>(3 * n);
+    a.lamx = sc.get<float>(3 * n);
+    a.qb = sc.get<float>(3 * n);
+    a.qy = sc.get<float>(3 * n);
+    a.qxy = sc.get<float>(3 * n);
+    a.iw = sc.get<float>(10 * n);
+    a.pm = sc.get<float>(6 * n);
+    a.ig = sc.get<float>(13 * n);
+    a.inb = sc.get<int32_t>(n);
+    a.iq = sc.get<float>(6 * n);
+    a.grad_amp = sc.get<double>(p.ny);
+    a.grad_phase = sc.get<double>(p.ny);
+    a.jstep = sc.get<double>(nsteps);
+    dlog[k] = sc.get<double>(nsteps);
+    tlog[k] = sc.get<double>(nsteps);
+    cps[k] = sc.get<float>((size_t)ncp * 3 * n);
+    if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint scratch)");
+    std::vector<float> hv(2 * (size_t)p.ny, 0.f);
+    for (int j = 0; j < p.ny; ++j) {
+      hv[j] = (float)(p.edge_amp ? p.edge_amp[j] : 0.0);
+      hv[p.ny + j] = (float)(p.edge_phase ? p.edge_phase[j] : 0.0);
+    }
+    HIP_TRY(hipMemcpy(ea, hv.data(), sizeof(float) * 2 * p.ny, hipMemcpyHostToDevice));
+    a.edge_amp = ea;
+    a.edge_phase = ea + p.ny;
+    a.dt_log = dlog[k];
+    a.T_log = tlog[k];
+    HIP_TRY(hipMemsetAsync(a.lam, 0, sizeof(float) * 3 * n, c->stream));
+    HIP_TRY(hipMemsetAsync(a.grad_amp, 0, sizeof(double) * p.ny, c->stream));
+    HIP_TRY(hipMemsetAsync(a.grad_phase, 0, sizeof(double) * p.ny, c->stream));
+  }
+  fv::AdjDesc* d_ad = sc.get<fv::AdjDesc>(ni);
+  if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint table)");
+  HIP_TRY(hipMemcpyAsync(d_ad, ad.data(), sizeof(fv::AdjDesc) * ni, hipMemcpyHostToDevice,
+                         c->stream));
+
+  // ---- forward: record dt/T of every accepted step, keep checkpoints ----
+  for (int k = 0; k < ni; ++k) {
+    c->desc[k].dt_log = dlog[k];
+    c->desc[k].T_log = tlog[k];
+    c->desc[k].log_base = ctl[k].n_acc;
+    c->desc[k].log_len = nsteps;
+  }
+  int rc = upload_tables(c);
+  for (int64_t cp = 0; cp < ncp && rc == 0; ++cp) {
+    if ((rc = read_ctls(c, ctl))) break;
+    for (int k = 0; k < ni; ++k) {
+      const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
+      for (int f = 0; f < 3; ++f) {
+        hipError_t e = hipMemcpyAsync(cps[k] + ((size_t)cp * 3 + f) * n,
+                                      c->desc[k].q[ctl[k].cur][f], sizeof(float) * n,
+                                      hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "checkpoint copy"); break; }
+      }
+    }
+    if (rc) break;
+    const int64_t len = std::min<int64_t>(K, nsteps - cp * K);
+    if (plain) {
+      // the reference's pattern: one launch per step, dt read back every step
+      for (int64_t s = 0; s < len && rc == 0; ++s) rc = fivonagi_step(c, 1);
+    } else {
+      rc = fivonagi_step(c, len);
+    }
+  }
+  for (int k = 0; k < ni; ++k) {
+    c->desc[k].dt_log = nullptr;
+    c->desc[k].T_log = nullptr;
+  }
+  int rc2 = upload_tables(c);
+  if (rc) return rc;
+  if (rc2) return rc2;
+
+  // ---- reverse, segment by segment ----
+  for (int64_t seg = ncp - 1; seg >= 0; --seg) {
+    const int64_t s0 = seg * K;
+    const int len = (int)std::min<int64_t>(K, nsteps - s0);
+    for (int k = 0; k < ni; ++k) {
+      const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
+      HIP_TRY(hipMemcpyAsync(ad[k].seg, cps[k] + (size_t)seg * 3 * n, sizeof(float) * 3 * n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int s = 0; s < len; ++s)
+      HIP_TRY(fv_adj_forward_step(d_ad, ni, max_cells, (int)(s0 + s), s, 1, c->stream));
+    for (int s = len - 1; s >= 0; --s) {
+      if (s != len - 1)
+        HIP_TRY(fv_adj_forward_step(d_ad, ni, max_cells, (int)(s0 + s), s, 0, c->stream));
+      HIP_TRY(fv_adj_reverse_step(d_ad, ni, max_cells, max_ny, (int)(s0 + s), s, c->stream));
+    }
+  }
+
+  // ---- results ----
+  size_t goff = 0;
+  for (int k = 0; k < ni; ++k) {
+    const auto& p = c->params[k];
+    std::vector<double> js(nsteps), ga(p.ny), gp(p.ny);
+    HIP_TRY(hipMemcpyAsync(js.data(), ad[k].jstep, sizeof(double) * nsteps,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(ga.data(), ad[k].grad_amp, sizeof(double) * p.ny,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(gp.data(), ad[k].grad_phase, sizeof(double) * p.ny,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double acc = 0.0;
+    for (double v : js) acc += v;
+    J[k] = acc;
+    if (grad_amp) std::memcpy(grad_amp + goff, ga.data(), sizeof(double) * p.ny);
+    if (grad_phase) std::memcpy(grad_phase + goff, gp.data(), sizeof(double) * p.ny);
+    goff += p.ny;
+  }
+  // keep dJ/dq0 for fivonagi_adjoint_state
+  for (float* q : c->adj_lam) (void)hipFree(q);
+  c->adj_lam.assign(ni, nullptr);
+  for (int k = 0; k < ni; ++k) {
+    const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
+    hipError_t e = hipMalloc(&c->adj_lam[k], sizeof(float) * 3 * n);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint state)");
+    HIP_TRY(hipMemcpyAsync(c->adj_lam[k], ad[k].lam, sizeof(float) * 3 * n,
+                           hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fivonagi_adjoint_state(fivonagi_ctx* c, int inst, float* l1, float* l2, float* l3) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  if (c->adj_lam.empty() || !c->adj_lam[inst])
+    return fail(FIVONAGI_ERR_ARG, "no adjoint has been run on this context");
+  const size_t n = (size_t)c->params[inst].nx * c->params[inst].ny;
+  float* dst[3] = {l1, l2, l3};
+  for (int f = 0; f < 3; ++f)
+    if (dst[f])
+      HIP_TRY(hipMemcpy(dst[f], c->adj_lam[inst] + (size_t)f * n, sizeof(float) * n,
+                        hipMemcpyDeviceToHost));
   return 0;
 }
 

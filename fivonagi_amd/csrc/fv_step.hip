@@ -315,6 +315,13 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   ctl->last_cfl = cfl;
   double T = Tnow;
   if (acc) {
+    if (D.dt_log) {   // record the executed dt/T sequence (adjoint runs)
+      const int64_t idx = ctl->n_acc - D.log_base;
+      if (idx >= 0 && idx < D.log_len) {
+        D.dt_log[idx] = dt;
+        D.T_log[idx] = Tnow;
+      }
+    }
     T = Tnow + dt;
     ctl->T = T;
     ctl->n_acc = ctl->n_acc + 1;

@@ -59,6 +59,9 @@ struct InstDesc {
   const uint32_t* cfl_bits;  // kCflBitmap: bit (i + j*nx) set = cell counted
   const double* dt_seq; // prescribed dt per attempt, or null
   int64_t dt_seq_len;
+  double* dt_log;       // when set: dt and T of accepted step n_acc - log_base
+  double* T_log;
+  int64_t log_base, log_len;
   double dx, dy, delta, cfl_max, cfl_wish, omega;
   double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff;
   int32_t nx, ny;

@@ -98,6 +98,34 @@ int fivonagi_collect(fivonagi_ctx *ctx, int inst, float *q1, float *q2, float *q
                      double *T, double *dt, int64_t *n_accepted, int64_t *n_rejected);
 int fivonagi_info(fivonagi_ctx *ctx, int inst, fivonagi_info_t *info);
 
+/* ---- discrete adjoint (no counterpart in the reference) -------------------
+   Objective per instance: J = sum_n dt_n * sum_{box} rho'_{n+1}^2 over the
+   nsteps accepted steps of the call.  Parameters: the left-edge drive
+   amplitude and phase of every row (FIVONAGI_BC_EDGE_DRIVE), and the state at
+   the start of the call.  The gradient is that of the executed branch path
+   with the dt sequence frozen (dt of each accepted step taken as given). */
+typedef struct {
+  int32_t box_x0, box_y0, box_nx, box_ny;  /* cells [x0, x0+nx) x [y0, y0+ny) */
+} fivonagi_objective;
+
+typedef struct {
+  int32_t checkpoint_every;  /* states kept every K steps (0 = ceil(sqrt(nsteps)));
+                                segments of K steps are recomputed backwards  */
+  int32_t plain_forward;     /* 1 = forward with one launch per step and a host
+                                readback of dt after each (no graph replay)   */
+} fivonagi_adjoint_opts;
+
+/* Advance every instance nsteps accepted steps (as fivonagi_step, recording
+   dt and T), then run the adjoint backwards.  obj: n_inst entries.  J: n_inst
+   values.  grad_amp / grad_phase: dJ/d edge_amp[j], dJ/d edge_phase[j], the
+   instances' ny values concatenated (may be NULL).  The state is left at the
+   end of the forward run. */
+int fivonagi_step_adjoint(fivonagi_ctx *ctx, int64_t nsteps, const fivonagi_objective *obj,
+                          const fivonagi_adjoint_opts *opts, double *J, double *grad_amp,
+                          double *grad_phase);
+/* dJ/d(state at the start of the last fivonagi_step_adjoint call), 3 grids. */
+int fivonagi_adjoint_state(fivonagi_ctx *ctx, int inst, float *l1, float *l2, float *l3);
+
 /* clean_gpu (common/data_definitions.h:415-517) */
 void fivonagi_destroy(fivonagi_ctx *ctx);
 

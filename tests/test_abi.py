@@ -33,12 +33,16 @@ def test_library_has_gfx950_code_object():
     assert b"gfx950" in data
 
 
+STRUCTS = (("fivonagi_params", fa.Params), ("fivonagi_info_t", fa.Info),
+           ("fivonagi_objective", fa.Objective), ("fivonagi_adjoint_opts", fa.AdjointOpts))
+
+
 def test_struct_layout_matches_header(tmp_path):
     """ctypes mirrors of fivonagi_params / fivonagi_info_t agree with the C
     compiler's layout of include/fivonagi.h (sizes and every field offset)."""
     import subprocess
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "fivonagi.h"', "int main(void){"]
-    for cname, cls in (("fivonagi_params", fa.Params), ("fivonagi_info_t", fa.Info)):
+    for cname, cls in STRUCTS:
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in cls._fields_:
             lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
@@ -50,7 +54,7 @@ def test_struct_layout_matches_header(tmp_path):
                     str(exe)], check=True)
     out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
                                                   check=True).stdout.splitlines())
-    for cname, cls in (("fivonagi_params", fa.Params), ("fivonagi_info_t", fa.Info)):
+    for cname, cls in STRUCTS:
         assert int(out[cname]) == ctypes.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"
