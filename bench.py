@@ -127,6 +127,114 @@ def copy_probe(nbytes=256 << 20, reps=5):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def bench_batched(args, hip):
+    """64 uneven instances (nx, ny in 256..2048 step 64), one launch per step
+    for all of them, work split chosen by the autotuner."""
+    import fivonagi_amd as fa
+    from fivonagi_amd import workloads as wl
+    probs = wl.batched(64)
+    stream = hip.stream()
+    ctx = fa.Context(probs)
+    ctx.set_stream(stream.value)
+    rows = ctx.autotune(10)
+    ctx.enqueue(args.warmup)
+    if ctx.sync() != 0:
+        raise RuntimeError("warmup failed")
+    before = [ctx.info(k) for k in range(len(probs))]
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    ctx.enqueue(args.steps)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1)
+    if ctx.sync() != 0:
+        raise RuntimeError("timed steps failed")
+    after = [ctx.info(k) for k in range(len(probs))]
+    cells = sum(p.nx * p.ny * (a["n_accepted"] - b["n_accepted"])
+                for p, a, b in zip(probs, after, before))
+    total = sum(p.nx * p.ny for p in probs)
+    value = cells / (ms / 1e3) / 1e9
+    achieved = BYTES_PER_CELL * total / (ms / args.steps / 1e3) / 1e9
+    ctx.close()
+    return {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched forward: 64 instances, {total} cells, per-instance dt",
+                   "rows_per_wave": rows, "accepted_cell_updates": cells},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": KERNEL},
+        "cpu_baseline": None,
+    }
+
+
+def bench_adjoint(args, hip):
+    """One gradient of J (time-integrated rho'^2 over a 64x64 box at x=1500)
+    with respect to 512 edge parameters on 2048x1024: forward (recording dt)
+    + discrete adjoint with checkpoints every 64 steps."""
+    import fivonagi_amd as fa
+    from fivonagi_amd import workloads as wl
+    prob, box, rows = wl.adjoint_single()
+    nsteps = args.steps
+    ctx = fa.Context(prob)
+    t0 = time.perf_counter()
+    J, ga, gp = ctx.step_adjoint(nsteps, box, checkpoint_every=64)
+    el = time.perf_counter() - t0
+    ctx.close()
+    g = np.concatenate([ga[0][rows], gp[0][rows]])
+    return {
+        "metric": METRIC, "value": round(1.0 / el, 4), "unit": "gradients/s", "n_gpus": 1,
+        "steps": nsteps, "warmup": 0, "ms_per_step": round(el * 1e3 / nsteps, 4),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"adjoint 2048x1024, 512 edge parameters, {nsteps} steps, "
+                               "checkpoint every 64", "J": J[0],
+                   "grad_norm": float(np.linalg.norm(g)), "seconds": round(el, 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def bench_batched_adjoint(args, hip):
+    """32 instances of 1024x1024 (own edge profiles, wavelengths and focal
+    boxes): batched forward+adjoint (one launch per pass for all instances,
+    graph-replayed forward) against 32 sequential plain-form solves (one
+    instance at a time, one launch per forward step with a host readback of dt)."""
+    import fivonagi_amd as fa
+    from fivonagi_amd import workloads as wl
+    probs, boxes = wl.adjoint_batch(32)
+    nsteps = args.steps
+    ctx = fa.Context(probs)
+    t0 = time.perf_counter()
+    J, ga, gp = ctx.step_adjoint(nsteps, boxes)
+    tb = time.perf_counter() - t0
+    ctx.close()
+    n_plain = args.plain_instances
+    t0 = time.perf_counter()
+    Jp = []
+    for k in range(n_plain):
+        c = fa.Context(probs[k])
+        jk, _, _ = c.step_adjoint(nsteps, boxes[k], plain_forward=True)
+        Jp.append(jk[0])
+        c.close()
+        print(f"plain solve {k + 1}/{n_plain} done", file=sys.stderr, flush=True)
+    tp = (time.perf_counter() - t0) * len(probs) / n_plain
+    rel = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(J[:n_plain], Jp))
+    return {
+        "metric": METRIC, "value": round(len(probs) / tb, 4), "unit": "gradients/s",
+        "n_gpus": 1, "steps": nsteps, "warmup": 0, "ms_per_step": round(tb * 1e3 / nsteps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched forward+adjoint: 32 x 1024x1024, {nsteps} steps",
+                   "batched_seconds": round(tb, 3),
+                   "plain_seconds_for_32": round(tp, 3),
+                   "plain_instances_timed": n_plain,
+                   "speedup_vs_plain": round(tp / tb, 3),
+                   "J_rel_diff_batched_vs_plain": rel},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,9 +247,22 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain copy of known size (counter calibration)")
+    ap.add_argument("--workload", default="edge_drive",
+                    choices=["edge_drive", "batched", "adjoint", "batched_adjoint"])
+    ap.add_argument("--plain-instances", type=int, default=32,
+                    help="batched_adjoint: plain solves timed (scaled to 32)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.workload != "edge_drive":
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        hip = Hip()
+        fn = {"batched": bench_batched, "adjoint": bench_adjoint,
+              "batched_adjoint": bench_batched_adjoint}[args.workload]
+        print(json.dumps(fn(args, hip)), flush=True)
         return
 
     if args.gpus != 1:

@@ -19,7 +19,7 @@ OK, ERR_EIGEN, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 8, 9, 10
 EXPORTS = (
     "fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence", "fivonagi_step",
     "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect", "fivonagi_info",
-    "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_destroy",
+    "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_destroy",
     "fivonagi_last_error",
 )
 
@@ -91,13 +91,15 @@ def load():
     lib.fivonagi_step_adjoint.argtypes = [vp, C.c_int64, C.POINTER(Objective),
                                           C.POINTER(AdjointOpts), dp, dp, dp]
     lib.fivonagi_adjoint_state.argtypes = [vp, C.c_int, fp, fp, fp]
+    lib.fivonagi_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int32)]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
     lib.fivonagi_last_error.argtypes = []
     lib.fivonagi_last_error.restype = C.c_char_p
     for name in ("fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
                  "fivonagi_step", "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect",
-                 "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state"):
+                 "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state",
+                 "fivonagi_autotune"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -262,3 +264,9 @@ class Context:
         _check(load().fivonagi_adjoint_state(
             self._h, inst, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in ls]))
         return np.stack(ls)
+
+    def autotune(self, trial_steps=20):
+        """Pick the rows per wave with the fastest step; state is unchanged."""
+        h = C.c_int32()
+        _check(load().fivonagi_autotune(self._h, int(trial_steps), C.byref(h)))
+        return h.value

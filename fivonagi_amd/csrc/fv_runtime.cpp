@@ -110,6 +110,7 @@ struct fivonagi_ctx {
     drop_graph();
     for (float* q : adj_lam) (void)hipFree(q);
     if (stream) (void)hipStreamSynchronize(stream);
+    if (d_items) (void)hipFree(d_items);
     for (void* p : allocs) (void)hipFree(p);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
@@ -163,6 +164,21 @@ void build_items(fivonagi_ctx* c, int auto_h) {
     c->desc[k].n_items = n;
     c->desc[k].first_item = (int)c->items.size() - n;
   }
+}
+
+// (re)upload the work list after build_items; the captured graph holds the
+// old grid size, so it is dropped
+int upload_items(fivonagi_ctx* c) {
+  c->drop_graph();
+  if (c->d_items) (void)hipFree(c->d_items);
+  c->d_items = nullptr;
+  HIP_TRY(hipMalloc(&c->d_items, sizeof(fv::WorkItem) * c->items.size()));
+  HIP_TRY(hipMemcpyAsync(c->d_items, c->items.data(), sizeof(fv::WorkItem) * c->items.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_desc, c->desc.data(), sizeof(fv::InstDesc) * c->n_inst,
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int upload_tables(fivonagi_ctx* c) {
@@ -364,12 +380,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   build_items(c, auto_rows(c->params, resident_waves(c->device)));
   c->d_desc = (fv::InstDesc*)c->dalloc(sizeof(fv::InstDesc) * n_inst, &e);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc desc"); }
-  c->d_items = (fv::WorkItem*)c->dalloc(sizeof(fv::WorkItem) * c->items.size(), &e);
-  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc items"); }
-  e = hipMemcpy(c->d_items, c->items.data(), sizeof(fv::WorkItem) * c->items.size(),
-                hipMemcpyHostToDevice);
-  if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy items"); }
-  int rc = upload_tables(c);
+  int rc = upload_items(c);
   if (rc) { delete c; return rc; }
   *out = c;
   return 0;
@@ -410,6 +421,75 @@ int fivonagi_set_dt_sequence(fivonagi_ctx* c, int inst, const double* dt, int64_
     d.dt_seq_len = 0;
   }
   return upload_tables(c);
+}
+
+int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
+  if (!c || trial_steps <= 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
+  std::vector<fv::Ctl> saved;
+  if (int rc = read_ctls(c, saved)) return rc;
+  for (int k = 0; k < c->n_inst; ++k)
+    if (saved[k].err) return fivonagi_sync(c);
+  // keep every instance's current state
+  std::vector<float*> keep(c->n_inst, nullptr);
+  auto release = [&]() {
+    for (float* p : keep) (void)hipFree(p);
+  };
+  for (int k = 0; k < c->n_inst; ++k) {
+    const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
+    hipError_t e = hipMalloc(&keep[k], sizeof(float) * 3 * n);
+    if (e != hipSuccess) { release(); return hip_fail(e, "hipMalloc (autotune)"); }
+    for (int f = 0; f < 3; ++f)
+      (void)hipMemcpyAsync(keep[k] + (size_t)f * n, c->desc[k].q[saved[k].cur][f],
+                           sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
+  }
+  const int auto_h = auto_rows(c->params, resident_waves(c->device));
+  std::vector<int> cands = {auto_h, 8, 12, 16, 20, 24, 32, 40, 48, 64, 96, 128};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float best_ms = 1e30f;
+  int best_h = auto_h;
+  int rc = 0;
+  std::vector<fivonagi_params> orig = c->params;
+  for (int h : cands) {
+    for (auto& p : c->params) p.rows_per_wave = h;
+    build_items(c, h);
+    if ((rc = upload_items(c))) break;
+    std::vector<int64_t> tgt(c->n_inst, INT64_MAX);
+    if ((rc = write_targets(c, tgt))) break;
+    if ((rc = launch_attempts(c, 2))) break;   // warm-up
+    (void)hipEventRecord(e0, c->stream);
+    if ((rc = launch_attempts(c, trial_steps))) break;
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) { best_ms = ms; best_h = h; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // restore: chosen work split, the saved state in the saved buffer, counters
+  c->params = orig;
+  for (auto& p : c->params) p.rows_per_wave = best_h;
+  build_items(c, best_h);
+  int rc2 = upload_items(c);
+  for (int k = 0; k < c->n_inst; ++k) {
+    const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
+    for (int f = 0; f < 3; ++f)
+      (void)hipMemcpyAsync(c->desc[k].q[saved[k].cur][f], keep[k] + (size_t)f * n,
+                           sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
+  }
+  hipError_t e = hipMemcpyAsync(c->d_ctl, saved.data(), sizeof(fv::Ctl) * c->n_inst,
+                                hipMemcpyHostToDevice, c->stream);
+  hipError_t es = hipStreamSynchronize(c->stream);
+  release();
+  c->unlimited = false;
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (e != hipSuccess) return hip_fail(e, "restore (autotune)");
+  if (es != hipSuccess) return hip_fail(es, "hipStreamSynchronize (autotune)");
+  if (rows_out) *rows_out = best_h;
+  return 0;
 }
 
 int fivonagi_enqueue(fivonagi_ctx* c, int64_t n) {

@@ -119,3 +119,46 @@ def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_
     t = dict(rhoa=rhoa, c=TAYLOR_C, theta=theta, yoff=yoff, jc=jc)
     return Problem(nx, ny, dx, dy, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=0.005,
                    T0=T0, bc=BC_TAYLOR, cfl_mode=cfl_mode, taylor=t)
+
+
+def aperture_params(ny, n_rows, rng, amp=0.02, halfwidth_rows=None, noise=0.05):
+    """Edge-drive parameters on an aperture of n_rows rows centred on the edge:
+    amp_j = amp * (g_j + noise * N(0,1)), phase_j = noise * N(0,1), zero
+    outside the aperture; g is a Gaussian of half-width n_rows/4."""
+    hw = n_rows / 4.0 if halfwidth_rows is None else halfwidth_rows
+    j0 = (ny - n_rows) // 2
+    rows = np.arange(j0, j0 + n_rows)
+    jc = (ny - 1) / 2.0
+    a = np.zeros(ny)
+    ph = np.zeros(ny)
+    a[rows] = amp * (np.exp(-((rows - jc) / hw) ** 2) + noise * rng.standard_normal(n_rows))
+    ph[rows] = noise * rng.standard_normal(n_rows)
+    return a, ph, rows
+
+
+def adjoint_single(seed=4, nx=2048, ny=1024):
+    """The single-instance adjoint workload: 2048x1024, 256 aperture rows x
+    (amplitude, phase) = 512 parameters, 20 cells per wavelength; J over a 64x64
+    box at x = 1500 (centred in y), 4000 steps, checkpoints every 64 steps."""
+    rng = np.random.default_rng(seed)
+    a, ph, rows = aperture_params(ny, 256, rng)
+    prob = edge_drive(nx, ny, edge_amp=a, edge_phase=ph, beta=3.5, delta=5e-5)
+    box = (1500, ny // 2 - 32, 64, 64)
+    return prob, box, rows
+
+
+def adjoint_batch(n=32, seed=5, size=1024):
+    """The batched forward+adjoint workload: n instances of size x size, each
+    with a 256-row edge profile, its own drive wavelength (10-40 cells) and its
+    own 64x64 focal box; beta = 3.5, delta = 1e-4."""
+    rng = np.random.default_rng(seed)
+    probs, boxes = [], []
+    for _ in range(n):
+        a, ph, rows = aperture_params(size, 256, rng)
+        lam = float(rng.uniform(10.0, 40.0))
+        probs.append(edge_drive(size, size, edge_amp=a, edge_phase=ph, cells_per_wavelength=lam,
+                                beta=3.5, delta=1e-4))
+        bx = int(rng.integers(size // 2, size - 64 - 2))
+        by = int(rng.integers(size // 4, 3 * size // 4 - 64))
+        boxes.append((bx, by, 64, 64))
+    return probs, boxes

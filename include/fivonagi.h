@@ -92,6 +92,12 @@ int fivonagi_enqueue(fivonagi_ctx *ctx, int64_t nattempts);
 /* Wait for the stream; return the first instance error code (0 if none). */
 int fivonagi_sync(fivonagi_ctx *ctx);
 
+/* Autotuner (no counterpart in the reference): time trial_steps step
+   attempts for each candidate height of the work items (rows per wave),
+   keep the fastest, and restore the state and counters as they were.
+   rows_per_wave_out (may be NULL) receives the choice. */
+int fivonagi_autotune(fivonagi_ctx *ctx, int trial_steps, int32_t *rows_per_wave_out);
+
 /* dataCollect/dataExport stand-in (apps/hifu-beam/data_export.h:29-75):
    copy instance inst's state and counters to the host.  Any pointer may be NULL. */
 int fivonagi_collect(fivonagi_ctx *ctx, int inst, float *q1, float *q2, float *q3,

@@ -158,3 +158,22 @@ def test_imaginary_eigenvalue_reports_2(oracle):
     op = to_oracle(prob)
     _, _, orc = oracle.run(op, prob.q0.astype(np.float64), 5)
     assert orc == oracle.ORC_ERR_EIGEN if hasattr(oracle, "ORC_ERR_EIGEN") else orc == 2
+
+
+def test_autotune_leaves_state_and_results_unchanged():
+    probs = [small_pulse(64, 64), wl.edge_drive(200, 96, halfwidth_cells=20.0)]
+    with fa.Context(probs) as ctx:
+        ctx.step(5)
+        before = [ctx.collect(k) for k in range(2)]
+        h = ctx.autotune(4)
+        after = [ctx.collect(k) for k in range(2)]
+        for (qa, sa), (qb, sb) in zip(before, after):
+            assert np.array_equal(qa, qb) and sa == sb
+        ctx.step(20)
+        tuned = [ctx.collect(k)[0] for k in range(2)]
+    assert h > 0
+    with fa.Context(probs) as ctx:
+        ctx.step(25)
+        plain = [ctx.collect(k)[0] for k in range(2)]
+    for a, b in zip(tuned, plain):
+        assert np.array_equal(a, b)
