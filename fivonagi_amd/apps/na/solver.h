@@ -30,6 +30,15 @@ namespace fvna {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// max(a, b) of finite values as one v_med3_f32 (median of a, b, FLT_MAX;
+// with +inf the intrinsic is folded back into a maxnum).
+// fmaxf must quiet signalling NaNs in IEEE mode, so on values the compiler
+// cannot prove canonical (registers carried around the march loop) it adds a
+// canonicalising v_max per operand; the med3 intrinsic is taken as is.
+__device__ __forceinline__ float fmax2(float a, float b) {
+  return __builtin_amdgcn_fmed3f(a, b, 3.4028234663852886e38f);
+}
+
 // Waves of one interface in (density, normal, tangential) component order.
 // W1 = a1*(1, L1, A1), W2 = a2*(1, L2, A2), W3 = (0, 0, a3).
 struct Waves {
@@ -78,7 +87,7 @@ __device__ __forceinline__ void riemann(float r1L, float nL, float tL, float unL
   f.w.a1 = a1; f.w.w21 = a1 * L1; f.w.w31 = a1 * A1;
   f.w.a2 = a2; f.w.w22 = a2 * L2; f.w.w32 = a2 * A2;
   f.w.a3 = dtg - (f.w.w31 + f.w.w32);                           // fv.h:119-125
-  f.smax = fmaxf(fabsf(L1), fabsf(L2));
+  f.smax = fmax2(fabsf(L1), fabsf(L2));
 }
 
 // MC limiter phi(r) = max(0, min((1+r)/2, 2, 2r)) (fv.h:587-605).  The
@@ -120,9 +129,10 @@ __device__ __forceinline__ void limited_flux(const Iface& f, float dotl1, float 
   const float c1 = 0.5f * s1 * fmaf(-s1, dtd, 1.0f) * phi1;
   const float c2 = 0.5f * s2 * fmaf(-s2, dtd, 1.0f) * phi2;
   const float c3 = 0.5f * s3 * fmaf(-s3, dtd, 1.0f) * phi3;
-  const float al1 = fmaxf(f.L1, 0.0f) - c1, be1 = fminf(f.L1, 0.0f) + c1;
-  const float al2 = fmaxf(f.L2, 0.0f) - c2, be2 = fminf(f.L2, 0.0f) + c2;
-  const float al3 = fmaxf(f.L3, 0.0f) - c3, be3 = fminf(f.L3, 0.0f) + c3;
+  // al_p = max(s_p, 0) - c_p; be_p = min(s_p, 0) + c_p = s_p - al_p
+  const float al1 = fmax2(f.L1, 0.0f) - c1, be1 = f.L1 - al1;
+  const float al2 = fmax2(f.L2, 0.0f) - c2, be2 = f.L2 - al2;
+  const float al3 = fmax2(f.L3, 0.0f) - c3, be3 = f.L3 - al3;
   P[0] = fmaf(al1, w.a1, al2 * w.a2);
   P[1] = fmaf(al1, w.w21, al2 * w.w22);
   P[2] = fmaf(al3, w.a3, fmaf(al1, w.w31, al2 * w.w32));

@@ -20,7 +20,7 @@
 #include "fv_types.h"
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
-                                     int n_items, hipStream_t stream);
+                                     const uint64_t* rowmask, int n_items, hipStream_t stream);
 extern "C" hipError_t fv_step_waves_per_cu(int* waves);
 extern "C" hipError_t fv_adj_forward_step(const fv::AdjDesc* ds, int n_inst, int max_cells, int k,
                                           int slot, int store_next, hipStream_t s);
@@ -81,10 +81,13 @@ struct fivonagi_ctx {
   std::vector<fivonagi_params> params;
   std::vector<fv::InstDesc> desc;          // host copy of the instance table
   std::vector<fv::WorkItem> items;
+  std::vector<uint64_t> rowmask;           // CFL lane mask per item row (fv::WorkItem::mrow)
+  std::vector<std::vector<uint32_t>> fold_bits;  // kCflBitmap: bit i + j*nx = cell counted
   std::vector<std::vector<double>> dt_seq;
   std::vector<void*> allocs;               // every device allocation
   fv::InstDesc* d_desc = nullptr;
   fv::WorkItem* d_items = nullptr;
+  uint64_t* d_rowmask = nullptr;
   fv::Ctl* d_ctl = nullptr;
   int graph_len = 0;
   int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
@@ -111,6 +114,7 @@ struct fivonagi_ctx {
     for (float* q : adj_lam) (void)hipFree(q);
     if (stream) (void)hipStreamSynchronize(stream);
     if (d_items) (void)hipFree(d_items);
+    if (d_rowmask) (void)hipFree(d_rowmask);
     for (void* p : allocs) (void)hipFree(p);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
@@ -142,8 +146,18 @@ long resident_waves(int device) {
   return (long)cus * per_cu;
 }
 
+// Is cell (i, j) of instance k one the CFL reduction counts?
+bool cfl_counted(const fivonagi_ctx* c, int k, int i, int j) {
+  const auto& d = c->desc[k];
+  const size_t o = (size_t)i + (size_t)j * (size_t)d.nx;
+  if (d.cfl_mask == fv::kCflEven) return (o & 1u) == 0u;
+  if (d.cfl_mask == fv::kCflBitmap) return (c->fold_bits[k][o >> 5] >> (o & 31u)) & 1u;
+  return true;
+}
+
 void build_items(fivonagi_ctx* c, int auto_h) {
   c->items.clear();
+  c->rowmask.clear();
   for (int k = 0; k < c->n_inst; ++k) {
     const auto& p = c->params[k];
     const int H = p.rows_per_wave > 0 ? p.rows_per_wave : auto_h;
@@ -157,6 +171,16 @@ void build_items(fivonagi_ctx* c, int auto_h) {
         w.x0 = s * fv::kStripCols;
         w.y0 = (int)((long)p.ny * ch / chunks);
         w.y1 = (int)((long)p.ny * (ch + 1) / chunks);
+        w.mrow = (int)c->rowmask.size();
+        w.pad[0] = w.pad[1] = w.pad[2] = 0;
+        for (int j = w.y0; j < w.y1; ++j) {
+          uint64_t m = 0;
+          for (int l = fv::kHalo; l < fv::kLanes - fv::kHalo; ++l) {
+            const int i = w.x0 - fv::kHalo + l;
+            if (i >= 1 && i < p.nx && cfl_counted(c, k, i, j)) m |= (uint64_t)1 << l;
+          }
+          c->rowmask.push_back(m);
+        }
         c->items.push_back(w);
         ++n;
       }
@@ -172,8 +196,13 @@ int upload_items(fivonagi_ctx* c) {
   c->drop_graph();
   if (c->d_items) (void)hipFree(c->d_items);
   c->d_items = nullptr;
+  if (c->d_rowmask) (void)hipFree(c->d_rowmask);
+  c->d_rowmask = nullptr;
   HIP_TRY(hipMalloc(&c->d_items, sizeof(fv::WorkItem) * c->items.size()));
   HIP_TRY(hipMemcpyAsync(c->d_items, c->items.data(), sizeof(fv::WorkItem) * c->items.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMalloc(&c->d_rowmask, sizeof(uint64_t) * c->rowmask.size()));
+  HIP_TRY(hipMemcpyAsync(c->d_rowmask, c->rowmask.data(), sizeof(uint64_t) * c->rowmask.size(),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->d_desc, c->desc.data(), sizeof(fv::InstDesc) * c->n_inst,
                          hipMemcpyHostToDevice, c->stream));
@@ -213,7 +242,7 @@ hipError_t launch_one(fivonagi_ctx* c) {
     hipError_t e = fv_launch_taylor_ghosts(c->d_desc, c->n_inst, c->taylor_ring, c->stream);
     if (e != hipSuccess) return e;
   }
-  return fv_launch_step(c->d_desc, c->d_items, (int)c->items.size(), c->stream);
+  return fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(), c->stream);
 }
 
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
@@ -288,6 +317,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   c->params.assign(p, p + n_inst);
   c->desc.resize(n_inst);
   c->dt_seq.resize(n_inst);
+  c->fold_bits.resize(n_inst);
   hipError_t e;
   e = hipGetDevice(&c->device);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipGetDevice"); }
@@ -342,14 +372,10 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
       if (even) {
         d.cfl_mask = fv::kCflEven;
       } else {
-        std::vector<uint32_t> bits((n + 31) / 32, 0u);
+        std::vector<uint32_t>& bits = c->fold_bits[k];
+        bits.assign((n + 31) / 32, 0u);
         for (size_t o = 0; o < n; ++o)
           if (m[o]) bits[o >> 5] |= 1u << (o & 31);
-        uint32_t* db = (uint32_t*)c->dalloc(sizeof(uint32_t) * bits.size(), &e);
-        if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc cfl bits"); }
-        e = hipMemcpy(db, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice);
-        if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy cfl bits"); }
-        d.cfl_bits = db;
         d.cfl_mask = fv::kCflBitmap;
       }
     }

@@ -39,7 +39,7 @@ namespace fv {
 
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) const float cgfloat;
-typedef __attribute__((address_space(1))) const uint32_t cguint;
+typedef __attribute__((address_space(4))) const float kfloat;   // read-only: scalar loads
 
 // DPP cross-lane moves over the whole wave64 (gfx9 wave_shr:1 / wave_shl:1).
 // shr1: lane l receives lane l-1's value (lane 0 receives 0).
@@ -51,6 +51,35 @@ __device__ __forceinline__ float shr1(float v) {
 }
 __device__ __forceinline__ float shl1(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+typedef __attribute__((address_space(1))) char gchar;
+typedef __attribute__((address_space(1))) const char cgchar;
+
+// Loads/stores at a 32-bit byte offset from a uniform base: the base stays in
+// scalar registers (global_load/store with saddr) and each lane carries one
+// 32-bit offset instead of a 64-bit address per field.
+__device__ __forceinline__ float ldg(cgfloat* base, uint32_t off) {
+  return *(cgfloat*)((cgchar*)base + off);
+}
+__device__ __forceinline__ void stg(gfloat* base, uint32_t off, float v) {
+  *(gfloat*)((gchar*)base + off) = v;
+}
+
+// s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding)
+constexpr int kWaitVm0 = 0x0F70;
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// v on the lanes whose bit of the uniform 64-bit mask is set, 0 elsewhere
+// (one v_cndmask with the mask as the lane condition).
+__device__ __forceinline__ float lane_sel(uint64_t mask, float v) {
+  float d;
+  asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(d) : "v"(v), "s"(mask));
+  return d;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -86,6 +115,7 @@ struct XY {          // one cell of q_xy, with its velocities for the source
 
 __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
                                                      const WorkItem* __restrict__ items,
+                                                     const uint64_t* __restrict__ rowmask,
                                                      int n_items) {
   // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
   // run of work items so vertically adjacent chunks of a strip (which share
@@ -101,17 +131,19 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   Ctl* ctl = D.ctl;
 
   // ---- per-step scalars (written by the previous step's finalizer) ----
-  if (ctl->err != 0 || ctl->n_acc >= ctl->target_acc) return;
-  const int cur = ctl->cur;
+  // The control block is read through vector loads (it is written in this
+  // launch); readfirstlane moves the uniform values to scalar registers so
+  // the state pointers stay scalar and free VGPRs.
+  if (uni(ctl->err) != 0 || ctl->n_acc >= ctl->target_acc) return;
+  const int cur = uni(ctl->cur);
   const double dt = ctl->dt;
   const double Tnow = ctl->T;
-  const float dtx = ctl->dtx, dty = ctl->dty;
-  const float ddx2 = ctl->ddx2, ddy2 = ctl->ddy2;
-  const float sin_t = (float)ctl->drive_s, cos_t = (float)ctl->drive_c;
+  const float dtx = unif(ctl->dtx), dty = unif(ctl->dty);
+  const float ddx2 = unif(ctl->ddx2), ddy2 = unif(ctl->ddy2);
+  const float sin_t = unif((float)ctl->drive_s), cos_t = unif((float)ctl->drive_c);
   const float beta2m2 = D.beta2m2;
   const int nx = D.nx, ny = D.ny;
   const int bc = D.bc;
-  const int cfl_mask = D.cfl_mask;
 
   cgfloat* __restrict__ in1 = (cgfloat*)D.q[cur][0];
   cgfloat* __restrict__ in2 = (cgfloat*)D.q[cur][1];
@@ -119,18 +151,19 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   gfloat* __restrict__ o1 = (gfloat*)D.q[cur ^ 1][0];
   gfloat* __restrict__ o2 = (gfloat*)D.q[cur ^ 1][1];
   gfloat* __restrict__ o3 = (gfloat*)D.q[cur ^ 1][2];
-  cgfloat* __restrict__ edge_c = (cgfloat*)D.edge_c;
-  cgfloat* __restrict__ edge_s = (cgfloat*)D.edge_s;
-  cguint* __restrict__ cbits = (cguint*)D.cfl_bits;
+  kfloat* __restrict__ edge_c = (kfloat*)D.edge_c;
+  kfloat* __restrict__ edge_s = (kfloat*)D.edge_s;
 
   const int lane = threadIdx.x;
   const int i = it.x0 - kHalo + lane;
   const bool col_in = (i >= 0) && (i < nx);
+  const bool col_x = col_in && (i >= 1);     // lane holds an x interface of the grid
   const int ic = min(max(i, 0), nx - 1);
   const bool col_int = (i >= 2) && (i <= nx - 3);
   const bool out_lane = (lane >= kHalo) && (lane < kLanes - kHalo) && (i < nx);
-  const bool cfl_col = out_lane && (i >= 1);
   const int y0 = it.y0, y1 = it.y1;
+  // CFL lane masks of the item's rows: cm[j], j in [y0, y1)
+  const uint64_t* __restrict__ cm = rowmask + (it.mrow - y0);
 
   // Ghost-cell source columns (apps/common/boundary.h): colA on rows
   // 2..ny-3, colB on ghost rows (top/bottom rules beat the right rule; the
@@ -140,24 +173,33 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   const bool left = !taylor && (ic <= 1);
   const int colA = taylor ? ic : (left ? 2 : (ic >= nx - 2 ? nx - 3 : ic));
   const int colB = taylor ? ic : (left ? 2 : ic);
+  const uint32_t bA = 4u * (uint32_t)colA, bB = 4u * (uint32_t)colB;
   const bool drive_lane = left && (bc == kBcEdgeDrive);
+  const uint32_t row_bytes = 4u * (uint32_t)nx;
 
-  // raw (unpatched) load of row r; the boundary patch is applied when the row
-  // is consumed one iteration later, so the load latency overlaps a full step
-  auto load_raw = [&](int r, float& a, float& b, float& c) {
+  // raw (unpatched) load of row r, with the row's drive coefficients; the
+  // boundary patch is applied when the row is consumed one iteration later,
+  // so the load latency overlaps a full step
+  auto load_raw = [&](int r, float& a, float& b, float& c, float& ec, float& es) {
     const int rc = min(max(r, 0), ny - 1);
     const bool grow = (rc <= 1) || (rc >= ny - 2);
     const int jm = taylor ? rc : (rc >= ny - 2 ? ny - 3 : (rc <= 1 ? 2 : rc));
-    const int o = (left ? rc : jm) * nx + (grow ? colB : colA);
-    a = in1[o];
-    b = in2[o];
-    c = in3[o];
+    const uint32_t rowL = (uint32_t)rc * row_bytes, rowM = (uint32_t)jm * row_bytes;
+    uint32_t o;
+    if (grow) o = (left ? rowL : rowM) + bB;   // ghost rows
+    else o = rowM + bA;                        // rows 2..ny-3: rc == jm
+    a = ldg(in1, o);
+    b = ldg(in2, o);
+    c = ldg(in3, o);
+    if (bc == kBcEdgeDrive) {
+      ec = edge_c[rc];
+      es = edge_s[rc];
+    }
   };
-  auto patch = [&](int r, Row& q) {
-    const int rc = min(max(r, 0), ny - 1);
+  auto patch = [&](Row& q, float ec, float es) {
     if (bc == kBcEdgeDrive) {
       float a, b, c;
-      fvbc::drive(edge_c[rc], edge_s[rc], sin_t, cos_t, a, b, c);
+      fvbc::drive(ec, es, sin_t, cos_t, a, b, c);
       q.q1 = drive_lane ? a : q.q1;
       q.q2 = drive_lane ? b : q.q2;
       q.q3 = drive_lane ? c : q.q3;
@@ -166,32 +208,61 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     q.un = q.q3 * ir;
     q.ut = q.q2 * ir;
   };
-  auto counted = [&](int j) -> bool {
-    if (cfl_mask == kCflAll) return true;
-    const uint32_t o = (uint32_t)i + (uint32_t)j * (uint32_t)nx;
-    if (cfl_mask == kCflEven) return (o & 1u) == 0u;
-    return (cbits[o >> 5] >> (o & 31u)) & 1u;
+  // source on row s (apps/na/source.h:86-104) from q_xy rows s-1, s, s+1,
+  // then the store of row s (lanes 3..60)
+  auto source_store = [&](int s, const XY& lo, const XY& mid, const XY& hi) {
+    const bool s_int = col_int && (s >= 2) && (s <= ny - 3);
+    const float uw = shr1(mid.u), ue = shl1(mid.u);
+    const float vw = shr1(mid.v), ve = shl1(mid.v);
+    const float r2 = (mid.q2 + ddx2 * (fmaf(-2.0f, mid.u, uw) + ue))
+                     + ddy2 * (fmaf(-2.0f, mid.u, lo.u) + hi.u);
+    const float r3 = (mid.q3 + ddx2 * (fmaf(-2.0f, mid.v, vw) + ve))
+                     + ddy2 * (fmaf(-2.0f, mid.v, lo.v) + hi.v);
+    if (out_lane) {
+      const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
+      stg(o1, o, mid.q1);            // q1 passes the source unchanged
+      stg(o2, o, s_int ? r2 : mid.q2);
+      stg(o3, o, s_int ? r3 : mid.q3);
+    }
   };
 
   float Sx = 0.0f, Sy = 0.0f;
-  bool err = false;
+  uint64_t err = 0;                          // lanes that met tmp3 < 0 (or NaN)
+  const uint64_t in_mask = __builtin_amdgcn_ballot_w64(col_in);
+  const uint64_t x_mask = __builtin_amdgcn_ballot_w64(col_x);
 
-  // One march step.  bn/bm/bo: q_b rows r, r-1, r-2; fn/fc/fl: y interfaces
-  // r, r-1, r-2; xn/xm/xo: q_xy rows r-2, r-3, r-4.  n1..n3 hold the raw
-  // prefetched row r on entry and row r+1 on exit.
-  float n1, n2, n3;
+  // One march step, r <= y1 + 2.  bn/bm/bo: q_b rows r, r-1, r-2; fn/fc/fl:
+  // y interfaces r, r-1, r-2.  xn/xo/xm hold q_xy rows r-5, r-4, r-3 on
+  // entry; the x sweep overwrites xn with row r-2.  n1..n3 (nec, nes) hold the
+  // raw prefetched row r on entry and row r+1 on exit.  Order: consume the
+  // prefetch, store row r-4, issue the next prefetch -- so each store has a
+  // whole step to drain before the next step waits for its prefetch (stores
+  // and loads share one in-order counter).
+  float n1, n2, n3, nec = 0.0f, nes = 0.0f;
   auto body = [&](int r, Row& bn, Row& bm, Row& bo, YI& fn, YI& fc, YI& fl, XY& xn, XY& xm,
                   XY& xo) {
+    // every load of the prefetch (and every store before it) is done: an
+    // explicit vmcnt(0) here keeps the compiler's own wait for the prefetched
+    // registers from landing after this step's stores
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
     bn.q1 = n1; bn.q2 = n2; bn.q3 = n3;
-    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3);
-    patch(r, bn);
+    patch(bn, nec, nes);
+    if (r >= y0 + 4) source_store(r - 4, xn, xo, xm);
+    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3, nec, nes);
     if (r < y0 - 2) return;
+    const int m = r - 2;
+    // CFL lane masks of rows r (y interfaces) and m (x interfaces); the index
+    // is clamped so the loads are unconditional, the row test selects 0
+    const uint64_t mYr = cm[min(max(r, y0), y1 - 1)];
+    const uint64_t mXr = cm[min(max(m, y0), y1 - 1)];
+    const uint64_t mY = (r >= y0 && r < y1 && r >= 1) ? mYr : 0;
+    const uint64_t mX = (m >= y0 && m < y1) ? mXr : 0;
 
     // ---- y interface r (rows r-1 | r): normal = q3, tangential = q2 ----
     fvna::riemann(bm.q1, bm.q3, bm.q2, bm.un, bm.ut, bn.q1, bn.q3, bn.q2, bn.un, bn.ut,
                   beta2m2, fn.f);
-    err |= fn.f.bad && col_in && (r >= 1) && (r < ny);
-    if (cfl_col && r >= y0 && r < y1 && r >= 1 && counted(r)) Sy = fmaxf(Sy, fn.f.smax);
+    if (r >= 1 && r < ny) err |= __builtin_amdgcn_ballot_w64(fn.f.bad) & in_mask;
+    Sy = fvna::fmax2(Sy, lane_sel(mY, fn.f.smax));
 
     // W(r).W(r-1): the left product of interface r, the right one of r-1
     fn.dl[0] = fvna::dot1(fn.f.w, fc.f.w);
@@ -206,7 +277,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     if (r < y0 + 1) return;
 
     // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2)
-    const int m = r - 2;
     const bool row_int = (m >= 2) && (m <= ny - 3);
     const bool upd = col_int && row_int;
     const float y1q = upd ? bo.q1 - dty * (fl.P[0] + M[0]) : bo.q1;
@@ -214,57 +284,38 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     const float y2q = upd ? bo.q2 - dty * (fl.P[2] + M[2]) : bo.q2;
 
     // ---- x sweep on row m: normal = q2, tangential = q3 ----
-    {
-      const float ir = fvna::rcp(1.0f + y1q);
-      const float un = y2q * ir, ut = y3q * ir;
-      const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
-      const float lun = shr1(un), lut = shr1(ut);
-      fvna::Iface fx;
-      fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, beta2m2, fx);
-      err |= fx.bad && col_in && (i >= 1) && (m >= 0) && (m < ny);
-      if (cfl_col && m >= y0 && m < y1 && counted(m)) Sx = fmaxf(Sx, fx.smax);
-      // waves of the interface to the left (k-1)
-      fvna::Waves wl;
-      wl.a1 = shr1(fx.w.a1); wl.w21 = shr1(fx.w.w21); wl.w31 = shr1(fx.w.w31);
-      wl.a2 = shr1(fx.w.a2); wl.w22 = shr1(fx.w.w22); wl.w32 = shr1(fx.w.w32);
-      wl.a3 = shr1(fx.w.a3);
-      const float dl1 = fvna::dot1(fx.w, wl), dl2 = fvna::dot2(fx.w, wl);
-      const float dl3 = fx.w.a3 * wl.a3;
-      // W(k).W(k+1) is the right neighbour's W(k+1).W(k)
-      const float dr1 = shl1(dl1), dr2 = shl1(dl2), dr3 = shl1(dl3);
-      float PX[3], MX[3];
-      fvna::limited_flux(fx, dl1, dr1, dl2, dr2, dl3, dr3, dtx, PX, MX);
-      const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
-      xn.q1 = upd ? y1q - dtx * (PX[0] + mr0) : y1q;
-      xn.q2 = upd ? y2q - dtx * (PX[1] + mr1) : y2q;
-      xn.q3 = upd ? y3q - dtx * (PX[2] + mr2) : y3q;
-      const float ir2 = fvna::rcp(1.0f + xn.q1);
-      xn.u = xn.q2 * ir2;
-      xn.v = xn.q3 * ir2;
-    }
-    if (r < y0 + 3) return;
-
-    // ---- source on row s = r-3 (apps/na/source.h:86-104), then store ----
-    const int s = r - 3;
-    const bool s_int = col_int && (s >= 2) && (s <= ny - 3);
-    const float uw = shr1(xm.u), ue = shl1(xm.u);
-    const float vw = shr1(xm.v), ve = shl1(xm.v);
-    const float r2 = (xm.q2 + ddx2 * (fmaf(-2.0f, xm.u, uw) + ue))
-                     + ddy2 * (fmaf(-2.0f, xm.u, xo.u) + xn.u);
-    const float r3 = (xm.q3 + ddx2 * (fmaf(-2.0f, xm.v, vw) + ve))
-                     + ddy2 * (fmaf(-2.0f, xm.v, xo.v) + xn.v);
-    if (out_lane) {
-      const int o = s * nx + i;
-      o1[o] = xm.q1;            // q1 passes the source unchanged
-      o2[o] = s_int ? r2 : xm.q2;
-      o3[o] = s_int ? r3 : xm.q3;
-    }
+    const float ir = fvna::rcp(1.0f + y1q);
+    const float un = y2q * ir, ut = y3q * ir;
+    const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
+    const float lun = shr1(un), lut = shr1(ut);
+    fvna::Iface fx;
+    fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, beta2m2, fx);
+    if (m >= 0 && m < ny) err |= __builtin_amdgcn_ballot_w64(fx.bad) & x_mask;
+    Sx = fvna::fmax2(Sx, lane_sel(mX, fx.smax));
+    // waves of the interface to the left (k-1)
+    fvna::Waves wl;
+    wl.a1 = shr1(fx.w.a1); wl.w21 = shr1(fx.w.w21); wl.w31 = shr1(fx.w.w31);
+    wl.a2 = shr1(fx.w.a2); wl.w22 = shr1(fx.w.w22); wl.w32 = shr1(fx.w.w32);
+    wl.a3 = shr1(fx.w.a3);
+    const float dl1 = fvna::dot1(fx.w, wl), dl2 = fvna::dot2(fx.w, wl);
+    const float dl3 = fx.w.a3 * wl.a3;
+    // W(k).W(k+1) is the right neighbour's W(k+1).W(k)
+    const float dr1 = shl1(dl1), dr2 = shl1(dl2), dr3 = shl1(dl3);
+    float PX[3], MX[3];
+    fvna::limited_flux(fx, dl1, dr1, dl2, dr2, dl3, dr3, dtx, PX, MX);
+    const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
+    xn.q1 = upd ? y1q - dtx * (PX[0] + mr0) : y1q;
+    xn.q2 = upd ? y2q - dtx * (PX[1] + mr1) : y2q;
+    xn.q3 = upd ? y3q - dtx * (PX[2] + mr2) : y3q;
+    const float ir2 = fvna::rcp(1.0f + xn.q1);
+    xn.u = xn.q2 * ir2;
+    xn.v = xn.q3 * ir2;
   };
 
   Row B0{}, B1{}, B2{};
   YI F0{}, F1{}, F2{};
   XY X0{}, X1{}, X2{};
-  load_raw(y0 - kHalo, n1, n2, n3);
+  load_raw(y0 - kHalo, n1, n2, n3, nec, nes);
   for (int r = y0 - kHalo; r <= y1 + 2; r += 3) {
     body(r, B0, B2, B1, F0, F2, F1, X0, X2, X1);
     if (r + 1 > y1 + 2) break;
@@ -272,11 +323,17 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     if (r + 2 > y1 + 2) break;
     body(r + 2, B2, B1, B0, F2, F1, F0, X2, X1, X0);
   }
+  // the last row, y1 - 1, with the window of the step r = y1 + 3 would see
+  switch ((y1 - y0) % 3) {
+    case 0: source_store(y1 - 1, X0, X1, X2); break;
+    case 1: source_store(y1 - 1, X1, X2, X0); break;
+    default: source_store(y1 - 1, X2, X0, X1); break;
+  }
 
   // ---- CFL / error reduction, then the instance finalizer ----
   // per-cell cfl = |s| * dt/d (fv.h:148-180); max over the wave's cells
   const float cflw = wave_max(fmaxf(Sy * dty, Sx * dtx));
-  const bool any_err = __any(err);
+  const bool any_err = err != 0;
   if (lane != 0) return;
   SyncSlots* sy = D.sync;
   const int local = item - D.first_item;
@@ -400,9 +457,9 @@ extern "C" hipError_t fv_step_waves_per_cu(int* waves) {
 }
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
-                                     int n_items, hipStream_t stream) {
+                                     const uint64_t* rowmask, int n_items, hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
   hipLaunchKernelGGL(fv::fv_step_kernel, dim3(n_items), dim3(fv::kLanes), 0, stream, insts,
-                     items, n_items);
+                     items, rowmask, n_items);
   return hipGetLastError();
 }

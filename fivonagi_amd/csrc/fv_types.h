@@ -56,7 +56,6 @@ struct InstDesc {
   Ctl* ctl;
   const float* edge_c;  // amp_j * cos(phase_j), length ny (kBcEdgeDrive)
   const float* edge_s;  // amp_j * sin(phase_j), length ny
-  const uint32_t* cfl_bits;  // kCflBitmap: bit (i + j*nx) set = cell counted
   const double* dt_seq; // prescribed dt per attempt, or null
   int64_t dt_seq_len;
   double* dt_log;       // when set: dt and T of accepted step n_acc - log_base
@@ -65,7 +64,7 @@ struct InstDesc {
   double dx, dy, delta, cfl_max, cfl_wish, omega;
   double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff;
   int32_t nx, ny;
-  int32_t bc, cfl_mask;
+  int32_t bc, cfl_mask;  // cfl_mask: host-side record of the fold subset kind
   SyncSlots* sync;
   int32_t n_items;      // work items (waves) of this instance per step
   int32_t first_item;   // index of the instance's first work item
@@ -76,10 +75,15 @@ struct InstDesc {
 };
 
 // One wave's work: a 58-column strip of one instance, rows [y0, y1).
+// mrow: index of the item's first row in the CFL lane-mask table; entry
+// mrow + (j - y0) has bit l set when lane l's cell (x0 - 3 + l, j) is an
+// output cell with i >= 1 that the CFL reduction counts.
 struct WorkItem {
   int32_t inst;
   int32_t x0;           // first output column of the strip
   int32_t y0, y1;
+  int32_t mrow;
+  int32_t pad[3];
 };
 
 }  // namespace fv
