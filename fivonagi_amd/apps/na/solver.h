@@ -30,6 +30,13 @@ namespace fvna {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// a + b never contracted with a product feeding it, so that when a is the
+// neighbouring lane's value its cross-lane move can fold into this add
+__device__ __forceinline__ float add_nc(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
 // max(a, b) of finite values as one v_med3_f32 (median of a, b, FLT_MAX;
 // with +inf the intrinsic is folded back into a maxnum).
 // fmaxf must quiet signalling NaNs in IEEE mode, so on values the compiler
@@ -57,31 +64,34 @@ struct Iface {
 // Riemann solve at the interface between cell L and cell R.
 // r1 = rho', n = normal momentum, t = tangential momentum,
 // un/ut = n/rho, t/rho of the same cell (precomputed once per cell).
-// beta2m2 = 2*(BETA - 1).  The fluctuations amdq/apdq (fv.h:141-180) are not
+// hbeta = BETA - 1.  The fluctuations amdq/apdq (fv.h:141-180) are not
 // formed here: limited_flux() folds them with the correction flux.
+// Every L operand enters only sums and differences with its R partner, so
+// when L is the neighbouring lane's value the cross-lane move folds into
+// the add/subtract (DPP operand).
 __device__ __forceinline__ void riemann(float r1L, float nL, float tL, float unL,
                                         float utL, float r1R, float nR, float tR,
-                                        float unR, float utR, float beta2m2,
+                                        float unR, float utR, float hbeta,
                                         Iface& f) {
-  const float Z1m1 = 0.5f * (r1R + r1L);            // fv.h:91, minus 1
-  const float Z1 = 1.0f + Z1m1;
-  const float Z2 = 0.5f * (unR + unL);              // fv.h:92
-  const float Z3 = 0.5f * (utR + utL);              // fv.h:93
-  const float tmp1 = Z2 * rcp(Z1);                  // fv.h:96
-  const float tmp2 = Z2 * tmp1;                     // fv.h:97
-  const float tmp3 = fmaf(tmp1, tmp1, -2.0f * tmp2) + fmaf(beta2m2, Z1m1, 1.0f);
+  const float s1 = add_nc(r1L, r1R);                // 2 (Z1 - 1)   fv.h:91
+  const float s2 = add_nc(unL, unR);                // 2 Z2         fv.h:92
+  const float s3 = add_nc(utL, utR);                // 2 Z3         fv.h:93
+  const float drho = r1R - r1L, dn = nR - nL, dtg = tR - tL;
+  const float S1 = s1 + 2.0f;                       // 2 Z1
+  const float tmp1 = s2 * rcp(S1);                  // Z2/Z1        fv.h:96
+  // tmp1^2 - 2 Z2 tmp1 + 1 + 2 (BETA-1)(Z1-1)                      fv.h:97-101
+  const float tmp3 = fmaf(tmp1, tmp1 - s2, fmaf(hbeta, s1, 1.0f));
   f.bad = !(tmp3 >= 0.0f);                          // fv.h:102
   const float rs = rsq(tmp3);
   const float sq = tmp3 * rs;                       // sqrt(tmp3)
   const float L1 = tmp1 + sq, L2 = tmp1 - sq;       // fv.h:105-107
   f.L1 = L1; f.L2 = L2; f.L3 = tmp1;
-  const float t2 = 2.0f * Z2 * Z3;
-  // fv.h:110-113 with one reciprocal for both denominators (each is ~ +-1)
-  const float d1 = fmaf(L1, Z1, -Z3), d2 = fmaf(L2, Z1, -Z3);
-  const float rd = rcp(d1 * d2);
-  const float A1 = fmaf(L1, Z3, -t2) * (d2 * rd);
-  const float A2 = fmaf(L2, Z3, -t2) * (d1 * rd);
-  const float drho = r1R - r1L, dn = nR - nL, dtg = tR - tL;
+  // fv.h:110-113: A_p = (L_p Z3 - 2 Z2 Z3)/(L_p Z1 - Z3) = s3 (L_p - s2)/(L_p S1 - s3),
+  // with one reciprocal for both denominators (each is ~ +-2)
+  const float d1 = fmaf(L1, S1, -s3), d2 = fmaf(L2, S1, -s3);
+  const float srd = s3 * rcp(d1 * d2);
+  const float A1 = (L1 - s2) * (d2 * srd);
+  const float A2 = (L2 - s2) * (d1 * srd);
   const float a1 = fmaf(-drho, L2, dn) * (0.5f * rs);         // fv.h:116-117
   const float a2 = drho - a1;                                   // fv.h:118
   f.w.a1 = a1; f.w.w21 = a1 * L1; f.w.w31 = a1 * A1;
@@ -90,28 +100,31 @@ __device__ __forceinline__ void riemann(float r1L, float nL, float tL, float unL
   f.smax = fmax2(fabsf(L1), fabsf(L2));
 }
 
-// MC limiter phi(r) = max(0, min((1+r)/2, 2, 2r)) (fv.h:587-605).  The
-// reference sets phi = 1 where wnorm2 == 0; such a wave is the zero vector,
-// so its correction phi*W is 0 for any finite phi.  Here wnorm2 == 0 gives
-// r = dot*inf (NaN or inf), which fminf/fmaxf turn into phi = 2 (finite), and
-// the select is skipped.
-__device__ __forceinline__ float philim(float wnorm2, float dot) {
+// MC limiter phi(r) = max(0, min((1+r)/2, 2, 2r)) (fv.h:587-605), returned
+// halved: phi/2 = clamp(min((1+r)/4, r), 0, 1), where the clamp is the VOP3
+// output modifier of the min.  The reference sets phi = 1 where wnorm2 == 0;
+// such a wave is the zero vector, so its correction phi*W is 0 for any
+// finite phi.  Here wnorm2 == 0 gives r = NaN (0 * inf), which the clamp
+// turns into 0 (finite), and the select is skipped.
+__device__ __forceinline__ float philim_half(float wnorm2, float dot) {
   const float r = dot * rcp(wnorm2);
-  return fmaxf(fminf(fminf(fmaf(0.5f, r, 0.5f), 2.0f), 2.0f * r), 0.0f);
+  return __builtin_amdgcn_fmed3f(fminf(fmaf(0.25f, r, 0.25f), r), 0.0f, 1.0f);
 }
 
+// W_p(a) . W_p(b); b is the operand that may come from a neighbouring lane
+// (mul, then multiply-accumulates with b as the plain multiplicand)
 __device__ __forceinline__ float dot1(const Waves& a, const Waves& b) {
-  return fmaf(a.a1, b.a1, fmaf(a.w21, b.w21, a.w31 * b.w31));
+  return fmaf(b.a1, a.a1, fmaf(b.w21, a.w21, b.w31 * a.w31));
 }
 __device__ __forceinline__ float dot2(const Waves& a, const Waves& b) {
-  return fmaf(a.a2, b.a2, fmaf(a.w22, b.w22, a.w32 * b.w32));
+  return fmaf(b.a2, a.a2, fmaf(b.w22, a.w22, b.w32 * a.w32));
 }
 
 // Fluctuations (fv.h:141-180) and correction flux fadd (fv.h:666-711) of one
 // interface, folded into what the two adjacent cells need:
 //   P = apdq - fadd = sum_p (max(s_p,0) - c_p) W_p   (cell on the + side)
 //   M = amdq + fadd = sum_p (min(s_p,0) + c_p) W_p   (cell on the - side)
-// with c_p = |s_p| (1 - |s_p| dt/d) phi_p / 2.  A wave with s_p = 0 enters
+// with c_p = |s_p| (1 - |s_p| dt/d) phi_p / 2 = |s_p| (1 - |s_p| dt/d) philim_half.  A wave with s_p = 0 enters
 // neither fluctuation, as in the reference.
 // dotl_p = W_p . W_p(k-1), dotr_p = W_p . W_p(k+1); upwind by sign(s_p).
 __device__ __forceinline__ void limited_flux(const Iface& f, float dotl1, float dotr1,
@@ -122,13 +135,13 @@ __device__ __forceinline__ void limited_flux(const Iface& f, float dotl1, float 
   const float n1 = fmaf(w.a1, w.a1, fmaf(w.w21, w.w21, w.w31 * w.w31));
   const float n2 = fmaf(w.a2, w.a2, fmaf(w.w22, w.w22, w.w32 * w.w32));
   const float n3 = w.a3 * w.a3;
-  const float phi1 = philim(n1, f.L1 > 0.0f ? dotl1 : dotr1);
-  const float phi2 = philim(n2, f.L2 > 0.0f ? dotl2 : dotr2);
-  const float phi3 = philim(n3, f.L3 > 0.0f ? dotl3 : dotr3);
+  const float h1 = philim_half(n1, f.L1 > 0.0f ? dotl1 : dotr1);
+  const float h2 = philim_half(n2, f.L2 > 0.0f ? dotl2 : dotr2);
+  const float h3 = philim_half(n3, f.L3 > 0.0f ? dotl3 : dotr3);
   const float s1 = fabsf(f.L1), s2 = fabsf(f.L2), s3 = fabsf(f.L3);
-  const float c1 = 0.5f * s1 * fmaf(-s1, dtd, 1.0f) * phi1;
-  const float c2 = 0.5f * s2 * fmaf(-s2, dtd, 1.0f) * phi2;
-  const float c3 = 0.5f * s3 * fmaf(-s3, dtd, 1.0f) * phi3;
+  const float c1 = s1 * fmaf(-s1, dtd, 1.0f) * h1;
+  const float c2 = s2 * fmaf(-s2, dtd, 1.0f) * h2;
+  const float c3 = s3 * fmaf(-s3, dtd, 1.0f) * h3;
   // al_p = max(s_p, 0) - c_p; be_p = min(s_p, 0) + c_p = s_p - al_p
   const float al1 = fmax2(f.L1, 0.0f) - c1, be1 = f.L1 - al1;
   const float al2 = fmax2(f.L2, 0.0f) - c2, be2 = f.L2 - al2;

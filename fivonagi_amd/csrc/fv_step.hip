@@ -82,6 +82,11 @@ __device__ __forceinline__ float lane_sel(uint64_t mask, float v) {
   return d;
 }
 
+// Pins a value in a register here: its computation cannot sink into a
+// branch that follows, so a cross-lane move feeding it stays in its block
+// and folds into its user.
+__device__ __forceinline__ void pin(float& v) { asm volatile("" : "+v"(v)); }
+
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -141,7 +146,7 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   const float dtx = unif(ctl->dtx), dty = unif(ctl->dty);
   const float ddx2 = unif(ctl->ddx2), ddy2 = unif(ctl->ddy2);
   const float sin_t = unif((float)ctl->drive_s), cos_t = unif((float)ctl->drive_c);
-  const float beta2m2 = D.beta2m2;
+  const float hbeta = 0.5f * D.beta2m2;     // BETA - 1
   const int nx = D.nx, ny = D.ny;
   const int bc = D.bc;
 
@@ -212,17 +217,22 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // then the store of row s (lanes 3..60)
   auto source_store = [&](int s, const XY& lo, const XY& mid, const XY& hi) {
     const bool s_int = col_int && (s >= 2) && (s <= ny - 3);
-    const float uw = shr1(mid.u), ue = shl1(mid.u);
-    const float vw = shr1(mid.v), ve = shl1(mid.v);
-    const float r2 = (mid.q2 + ddx2 * (fmaf(-2.0f, mid.u, uw) + ue))
-                     + ddy2 * (fmaf(-2.0f, mid.u, lo.u) + hi.u);
-    const float r3 = (mid.q3 + ddx2 * (fmaf(-2.0f, mid.v, vw) + ve))
-                     + ddy2 * (fmaf(-2.0f, mid.v, lo.v) + hi.v);
+    // u(i-1) + u(i+1): one cross-lane move, the other folds into the add
+    const float uwe = shr1(mid.u) + shl1(mid.u);
+    const float vwe = shr1(mid.v) + shl1(mid.v);
+    float r2 = (mid.q2 + ddx2 * fmaf(-2.0f, mid.u, uwe))
+               + ddy2 * (fmaf(-2.0f, mid.u, lo.u) + hi.u);
+    float r3 = (mid.q3 + ddx2 * fmaf(-2.0f, mid.v, vwe))
+               + ddy2 * (fmaf(-2.0f, mid.v, lo.v) + hi.v);
+    r2 = s_int ? r2 : mid.q2;
+    r3 = s_int ? r3 : mid.q3;
+    pin(r2);
+    pin(r3);
     if (out_lane) {
       const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
       stg(o1, o, mid.q1);            // q1 passes the source unchanged
-      stg(o2, o, s_int ? r2 : mid.q2);
-      stg(o3, o, s_int ? r3 : mid.q3);
+      stg(o2, o, r2);
+      stg(o3, o, r3);
     }
   };
 
@@ -260,8 +270,10 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
 
     // ---- y interface r (rows r-1 | r): normal = q3, tangential = q2 ----
     fvna::riemann(bm.q1, bm.q3, bm.q2, bm.un, bm.ut, bn.q1, bn.q3, bn.q2, bn.un, bn.ut,
-                  beta2m2, fn.f);
-    if (r >= 1 && r < ny) err |= __builtin_amdgcn_ballot_w64(fn.f.bad) & in_mask;
+                  hbeta, fn.f);
+    // branch-free (a branch here would split the block and stop the
+    // cross-lane moves of the x sweep from folding into their users)
+    err |= __builtin_amdgcn_ballot_w64(fn.f.bad) & ((r >= 1 && r < ny) ? in_mask : 0);
     Sy = fvna::fmax2(Sy, lane_sel(mY, fn.f.smax));
 
     // W(r).W(r-1): the left product of interface r, the right one of r-1
@@ -289,8 +301,8 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
     const float lun = shr1(un), lut = shr1(ut);
     fvna::Iface fx;
-    fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, beta2m2, fx);
-    if (m >= 0 && m < ny) err |= __builtin_amdgcn_ballot_w64(fx.bad) & x_mask;
+    fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, hbeta, fx);
+    err |= __builtin_amdgcn_ballot_w64(fx.bad) & ((m >= 0 && m < ny) ? x_mask : 0);
     Sx = fvna::fmax2(Sx, lane_sel(mX, fx.smax));
     // waves of the interface to the left (k-1)
     fvna::Waves wl;
@@ -304,9 +316,17 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     float PX[3], MX[3];
     fvna::limited_flux(fx, dl1, dr1, dl2, dr2, dl3, dr3, dtx, PX, MX);
     const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
-    xn.q1 = upd ? y1q - dtx * (PX[0] + mr0) : y1q;
-    xn.q2 = upd ? y2q - dtx * (PX[1] + mr1) : y2q;
-    xn.q3 = upd ? y3q - dtx * (PX[2] + mr2) : y3q;
+    // computed on every lane (the non-updated lanes are few; a branch would
+    // save nothing and would keep the moves above from folding)
+    float nq1 = y1q - dtx * (mr0 + PX[0]);
+    float nq2 = y2q - dtx * (mr1 + PX[1]);
+    float nq3 = y3q - dtx * (mr2 + PX[2]);
+    pin(nq1);
+    pin(nq2);
+    pin(nq3);
+    xn.q1 = upd ? nq1 : y1q;
+    xn.q2 = upd ? nq2 : y2q;
+    xn.q3 = upd ? nq3 : y3q;
     const float ir2 = fvna::rcp(1.0f + xn.q1);
     xn.u = xn.q2 * ir2;
     xn.v = xn.q3 * ir2;
