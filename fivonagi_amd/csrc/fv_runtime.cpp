@@ -217,12 +217,14 @@ int upload_tables(fivonagi_ctx* c) {
   return 0;
 }
 
-void set_ctl_derived(fv::Ctl& t, const fivonagi_params& p) {
-  t.dtx = (float)(t.dt / p.dx);
-  t.dty = (float)(t.dt / p.dy);
-  t.ddx2 = (float)(p.delta * t.dt / (p.dx * p.dx));
-  t.ddy2 = (float)(p.delta * t.dt / (p.dy * p.dy));
-  const double th = std::fmod(p.omega * t.T, 6.283185307179586476925286766559);
+// dt-derived step scalars, as the step kernel's finalizer forms them
+void set_ctl_derived(fv::Ctl& t, const fv::InstDesc& d) {
+  t.dtx = (float)(t.dt * d.inv_dx);
+  t.dty = (float)(t.dt * d.inv_dy);
+  t.ddx2 = (float)(t.dt * d.dif_x);
+  t.ddy2 = (float)(t.dt * d.dif_y);
+  const double turns = d.omega_turns * t.T;
+  const double th = 6.283185307179586476925286766559 * (turns - std::floor(turns));
   t.drive_s = std::sin(th);
   t.drive_c = std::cos(th);
 }
@@ -385,6 +387,9 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     d.dt_seq_len = 0;
     d.dx = q.dx; d.dy = q.dy; d.delta = q.delta;
     d.cfl_max = q.cfl_max; d.cfl_wish = q.cfl_wish; d.omega = q.omega;
+    d.inv_dx = 1.0 / q.dx; d.inv_dy = 1.0 / q.dy;
+    d.dif_x = q.delta / (q.dx * q.dx); d.dif_y = q.delta / (q.dy * q.dy);
+    d.omega_turns = q.omega / 6.283185307179586476925286766559;
     d.tay_rhoa = q.tay_rhoa; d.tay_c = q.tay_c;
     d.tay_cos = std::cos(q.tay_theta); d.tay_sin = std::sin(q.tay_theta);
     d.tay_yoff = q.tay_yoff; d.tay_jc = q.tay_jc;
@@ -398,7 +403,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     t.dt = q.dt0;
     t.T = q.T0;
     t.target_acc = 0;
-    set_ctl_derived(t, q);
+    set_ctl_derived(t, d);
   }
   e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * n_inst, hipMemcpyHostToDevice);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
@@ -440,7 +445,7 @@ int fivonagi_set_dt_sequence(fivonagi_ctx* c, int inst, const double* dt, int64_
     auto& t = ctl[inst];
     if (t.n_att >= n) return fail(FIVONAGI_ERR_ARG, "dt sequence shorter than attempts made");
     t.dt = dt[t.n_att];
-    set_ctl_derived(t, c->params[inst]);
+    set_ctl_derived(t, d);
     HIP_TRY(hipMemcpy(c->d_ctl + inst, &t, sizeof(fv::Ctl), hipMemcpyHostToDevice));
   } else {
     d.dt_seq = nullptr;

@@ -70,6 +70,12 @@ __device__ __forceinline__ void stg(gfloat* base, uint32_t off, float v) {
 constexpr int kWaitVm0 = 0x0F70;
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+  const uint64_t u = (uint64_t)v;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ float unif(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
@@ -139,7 +145,10 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // The control block is read through vector loads (it is written in this
   // launch); readfirstlane moves the uniform values to scalar registers so
   // the state pointers stay scalar and free VGPRs.
-  if (uni(ctl->err) != 0 || ctl->n_acc >= ctl->target_acc) return;
+  const int64_t n_acc = uni64(ctl->n_acc);
+  if (uni(ctl->err) != 0 || n_acc >= uni64(ctl->target_acc)) return;
+  // counters the finalizer updates, read now so its tail has no loads
+  const int64_t n_att = uni64(ctl->n_att), n_rej = uni64(ctl->n_rej);
   const int cur = uni(ctl->cur);
   const double dt = ctl->dt;
   const double Tnow = ctl->T;
@@ -291,9 +300,12 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2)
     const bool row_int = (m >= 2) && (m <= ny - 3);
     const bool upd = col_int && row_int;
-    const float y1q = upd ? bo.q1 - dty * (fl.P[0] + M[0]) : bo.q1;
-    const float y3q = upd ? bo.q3 - dty * (fl.P[1] + M[1]) : bo.q3;
-    const float y2q = upd ? bo.q2 - dty * (fl.P[2] + M[2]) : bo.q2;
+    // ghost cells keep their value: dt/d is 0 there (the fluxes are finite
+    // wherever the attempt is valid, so q - 0*flux == q exactly)
+    const float dtyu = upd ? dty : 0.0f, dtxu = upd ? dtx : 0.0f;
+    const float y1q = bo.q1 - dtyu * (fl.P[0] + M[0]);
+    const float y3q = bo.q3 - dtyu * (fl.P[1] + M[1]);
+    const float y2q = bo.q2 - dtyu * (fl.P[2] + M[2]);
 
     // ---- x sweep on row m: normal = q2, tangential = q3 ----
     const float ir = fvna::rcp(1.0f + y1q);
@@ -316,17 +328,9 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     float PX[3], MX[3];
     fvna::limited_flux(fx, dl1, dr1, dl2, dr2, dl3, dr3, dtx, PX, MX);
     const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
-    // computed on every lane (the non-updated lanes are few; a branch would
-    // save nothing and would keep the moves above from folding)
-    float nq1 = y1q - dtx * (mr0 + PX[0]);
-    float nq2 = y2q - dtx * (mr1 + PX[1]);
-    float nq3 = y3q - dtx * (mr2 + PX[2]);
-    pin(nq1);
-    pin(nq2);
-    pin(nq3);
-    xn.q1 = upd ? nq1 : y1q;
-    xn.q2 = upd ? nq2 : y2q;
-    xn.q3 = upd ? nq3 : y3q;
+    xn.q1 = y1q - dtxu * (mr0 + PX[0]);
+    xn.q2 = y2q - dtxu * (mr1 + PX[1]);
+    xn.q3 = y3q - dtxu * (mr2 + PX[2]);
     const float ir2 = fvna::rcp(1.0f + xn.q1);
     xn.u = xn.q2 * ir2;
     xn.v = xn.q3 * ir2;
@@ -368,17 +372,22 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   asm volatile("" ::"v"(prev));
   __builtin_amdgcn_s_waitcnt(0);
   if (aadd_relaxed(&sy->grp[g].ticket, 1u) != (uint32_t)(gsize - 1)) return;
-  __builtin_amdgcn_s_waitcnt(0);
   if (aadd_relaxed(&sy->top_ticket, 1u) != (uint32_t)(ngroups - 1)) return;
 
-  // finalizer: every wave of the instance has taken its ticket
-  float cfl32 = 0.0f;
-  for (int k = 0; k < ngroups; ++k) {
-    cfl32 = fmaxf(cfl32, __uint_as_float(axchg_relaxed(&sy->grp[k].cfl_bits, 0u)));
-    axchg_relaxed(&sy->grp[k].ticket, 0u);
-  }
-  axchg_relaxed(&sy->top_ticket, 0u);
+  // finalizer: every wave of the instance has taken its ticket.  The group
+  // slots are read and cleared by independent exchanges/stores issued back
+  // to back (slots of groups an instance never uses stay 0).
+  uint32_t gm[kGroups];
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) gm[k] = axchg_relaxed(&sy->grp[k].cfl_bits, 0u);
   const uint32_t e = axchg_relaxed(&sy->err_flag, 0u);
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k)
+    __hip_atomic_store(&sy->grp[k].ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&sy->top_ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  float cfl32 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) cfl32 = fmaxf(cfl32, __uint_as_float(gm[k]));
   if (e) {
     ctl->err = 2;   // imaginary eigenvalue: step discarded, instance stops
     return;
@@ -387,13 +396,13 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   const double cfl = (double)cfl32;
   const bool presc = D.dt_seq != nullptr;
   const bool acc = presc || (cfl < D.cfl_max);
-  const int64_t att = ctl->n_att + 1;
+  const int64_t att = n_att + 1;
   ctl->n_att = att;
   ctl->last_cfl = cfl;
   double T = Tnow;
   if (acc) {
     if (D.dt_log) {   // record the executed dt/T sequence (adjoint runs)
-      const int64_t idx = ctl->n_acc - D.log_base;
+      const int64_t idx = n_acc - D.log_base;
       if (idx >= 0 && idx < D.log_len) {
         D.dt_log[idx] = dt;
         D.T_log[idx] = Tnow;
@@ -401,10 +410,10 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     }
     T = Tnow + dt;
     ctl->T = T;
-    ctl->n_acc = ctl->n_acc + 1;
+    ctl->n_acc = n_acc + 1;
     ctl->cur = cur ^ 1;
   } else {
-    ctl->n_rej = ctl->n_rej + 1;
+    ctl->n_rej = n_rej + 1;
   }
   double ndt;
   if (presc) {
@@ -415,13 +424,17 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     if (!acc && ndt > dt * 0.9) ndt = dt * 0.9;
   }
   ctl->dt = ndt;
-  ctl->dtx = (float)(ndt / D.dx);
-  ctl->dty = (float)(ndt / D.dy);
-  ctl->ddx2 = (float)(D.delta * ndt / (D.dx * D.dx));
-  ctl->ddy2 = (float)(D.delta * ndt / (D.dy * D.dy));
-  const double th = fmod(D.omega * T, 6.283185307179586476925286766559);
-  ctl->drive_s = sin(th);
-  ctl->drive_c = cos(th);
+  ctl->dtx = (float)(ndt * D.inv_dx);
+  ctl->dty = (float)(ndt * D.inv_dy);
+  ctl->ddx2 = (float)(ndt * D.dif_x);
+  ctl->ddy2 = (float)(ndt * D.dif_y);
+  // drive phase omega*T in turns, reduced in double to [0, 1); the step
+  // uses sin/cos in FP32
+  const double turns = D.omega_turns * T;
+  float fs, fc;
+  sincospif((float)(2.0 * (turns - floor(turns))), &fs, &fc);
+  ctl->drive_s = fs;
+  ctl->drive_c = fc;
 }
 
 // Closed-form ghost ring of the Taylor-shock instances (apps/taylor-angle-xy/

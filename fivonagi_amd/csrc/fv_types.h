@@ -36,7 +36,8 @@ struct Ctl {
 // Reduction slots of the running attempt, per instance: the waves of an
 // instance are split into kGroups groups (local item index mod kGroups), each
 // with its own 128-byte line, so the same-address atomics of one step are
-// spread over kGroups + 1 lines.  Reset by the finalizer.
+// spread over kGroups + 1 lines (one line taking every wave's atomics slows
+// the whole step).  Reset by the finalizer.
 constexpr int kGroups = 8;
 struct alignas(128) GroupSlot {
   uint32_t cfl_bits;    // max cfl of the group's cells (float bits, >= 0)
@@ -62,6 +63,9 @@ struct InstDesc {
   double* T_log;
   int64_t log_base, log_len;
   double dx, dy, delta, cfl_max, cfl_wish, omega;
+  double inv_dx, inv_dy;     // 1/dx, 1/dy
+  double dif_x, dif_y;       // delta/dx^2, delta/dy^2
+  double omega_turns;        // omega / (2 pi)
   double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff;
   int32_t nx, ny;
   int32_t bc, cfl_mask;  // cfl_mask: host-side record of the fold subset kind
