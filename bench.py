@@ -80,7 +80,7 @@ def traffic_from_profiles(workload_tag):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(prob, seconds=12.0):
+def cpu_baseline(prob, seconds=15.0):
     """The FP32 build of the CPU oracle (a port of the reference step), on this
     host's cores, over a bounded number of steps of the same grid."""
     from oracle import pyoracle as po
@@ -91,9 +91,11 @@ def cpu_baseline(prob, seconds=12.0):
                     form=po.FORM_RHO1, omega=prob.omega, edge_amp=prob.edge_amp,
                     nthreads=threads)
     q = prob.q0.astype(np.float32)
-    # calibrate on a few steps, then time one bounded run of the same grid
+    # warm up (threads, first touch), calibrate on a few more steps, then time
+    # one bounded run of the same grid
+    q, st, rc = po.run(op, q, 3, dtype=np.float32, fast=True)
     t0 = time.perf_counter()
-    q, st, rc = po.run(op, q, 5, dtype=np.float32, fast=True)
+    q, st, rc = po.run(op, q, 5, dtype=np.float32, state=st, fast=True)
     per_step = max((time.perf_counter() - t0) / 5, 1e-4)
     steps = max(5, int(seconds / per_step))
     t0 = time.perf_counter()
@@ -244,7 +246,7 @@ def main():
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--rows-per-wave", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain copy of known size (counter calibration)")
     ap.add_argument("--workload", default="edge_drive",
