@@ -36,6 +36,12 @@ struct AdjDesc {
   const double* T_log;                 // T at the start of accepted step n
 };
 
+// One wave's work in the adjoint march kernels: a strip of one instance,
+// output columns [x0, x0 + width), rows [y0, y1).
+struct AdjItem {
+  int32_t inst, x0, y0, y1;
+};
+
 }  // namespace fv
 
 #endif  // FIVONAGI_FV_ADJOINT_H

@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void adj_flux_bw(const AdjDesc* __restrict__ d
   for (int c = 0; c < 7; ++c) ig[(size_t)c * n + o] = gW[c];
   for (int c = 0; c < 3; ++c) ig[(size_t)(7 + c) * n + o] = gs[c];
   for (int c = 0; c < 3; ++c) ig[(size_t)(10 + c) * n + o] = gd[c];
-  d.inb[o] = (l.nb[0] > 0 ? 1 : 0) | (l.nb[1] > 0 ? 2 : 0) | (l.nb[2] > 0 ? 4 : 0);
+  d.inb[o] = l.nb;
 }
 
 __device__ __forceinline__ void add_wave(float gW[7], const float* iw, int n, int o, int p, float g) {

@@ -22,10 +22,12 @@
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, hipStream_t stream);
 extern "C" hipError_t fv_step_waves_per_cu(int* waves);
-extern "C" hipError_t fv_adj_forward_step(const fv::AdjDesc* ds, int n_inst, int max_cells, int k,
-                                          int slot, int store_next, hipStream_t s);
-extern "C" hipError_t fv_adj_reverse_step(const fv::AdjDesc* ds, int n_inst, int max_cells,
-                                          int max_ny, int k, int slot, hipStream_t s);
+extern "C" hipError_t fv_adj_march_forward(const fv::AdjDesc* ds, const fv::AdjItem* items,
+                                           int n_items, int k, int slot, hipStream_t s);
+extern "C" hipError_t fv_adj_march_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items_x,
+                                           int n_items_x, const fv::AdjItem* items_y,
+                                           int n_items_y, int k, int slot, hipStream_t s);
+extern "C" hipError_t fv_adj_march_occupancy(int* x_fwd, int* x_rev, int* y_rev);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, hipStream_t stream);
 
@@ -681,16 +683,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     float* ea = sc.get<float>(2 * (size_t)p.ny);
     a.seg = sc.get<float>((size_t)(K + 1) * 3 * n);
     a.lam = sc.get<float>(3 * n);
-    a.lam2 = sc.get<float>(3 * n);
     a.lamx = sc.get<float>(3 * n);
-    a.qb = sc.get<float>(3 * n);
-    a.qy = sc.get<float>(3 * n);
-    a.qxy = sc.get<float>(3 * n);
-    a.iw = sc.get<float>(10 * n);
-    a.pm = sc.get<float>(6 * n);
-    a.ig = sc.get<float>(13 * n);
-    a.inb = sc.get<int32_t>(n);
-    a.iq = sc.get<float>(6 * n);
     a.grad_amp = sc.get<double>(p.ny);
     a.grad_phase = sc.get<double>(p.ny);
     a.jstep = sc.get<double>(nsteps);
@@ -711,7 +704,46 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     HIP_TRY(hipMemsetAsync(a.lam, 0, sizeof(float) * 3 * n, c->stream));
     HIP_TRY(hipMemsetAsync(a.grad_amp, 0, sizeof(double) * p.ny, c->stream));
     HIP_TRY(hipMemsetAsync(a.grad_phase, 0, sizeof(double) * p.ny, c->stream));
+    HIP_TRY(hipMemsetAsync(a.jstep, 0, sizeof(double) * nsteps, c->stream));
   }
+  // work items of the march kernels: strips of 56 (x march, 4-column halo)
+  // and 60 (y march, 2-column halo) output columns, rows split so every
+  // kernel's waves fit one resident round of the device
+  std::vector<fv::AdjItem> ix, iy;
+  {
+    int ox = 0, orx = 0, ory = 0, cus = 0;
+    if (fv_adj_march_occupancy(&ox, &orx, &ory) != hipSuccess || orx <= 0 || ory <= 0) orx = ory = 2;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    auto build = [&](int width, int occ, std::vector<fv::AdjItem>& out) {
+      const long target = (long)cus * occ;
+      int H = 8;
+      for (; H < 4096; ++H) {
+        long waves = 0;
+        for (const auto& p : c->params)
+          waves += (long)((p.nx + width - 1) / width) * std::max(1L, std::lround((double)p.ny / H));
+        if (waves <= target) break;
+      }
+      for (int k = 0; k < ni; ++k) {
+        const auto& p = c->params[k];
+        const int chunks = std::max(1, (int)std::lround((double)p.ny / H));
+        for (int x0 = 0; x0 < p.nx; x0 += width)
+          for (int ch = 0; ch < chunks; ++ch)
+            out.push_back({k, x0, (int)((long)p.ny * ch / chunks),
+                           (int)((long)p.ny * (ch + 1) / chunks)});
+      }
+    };
+    build(56, orx, ix);
+    build(60, ory, iy);
+  }
+  fv::AdjItem* d_ix = sc.get<fv::AdjItem>(ix.size());
+  fv::AdjItem* d_iy = sc.get<fv::AdjItem>(iy.size());
+  if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint work items)");
+  HIP_TRY(hipMemcpyAsync(d_ix, ix.data(), sizeof(fv::AdjItem) * ix.size(), hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(d_iy, iy.data(), sizeof(fv::AdjItem) * iy.size(), hipMemcpyHostToDevice,
+                         c->stream));
   fv::AdjDesc* d_ad = sc.get<fv::AdjDesc>(ni);
   if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint table)");
   HIP_TRY(hipMemcpyAsync(d_ad, ad.data(), sizeof(fv::AdjDesc) * ni, hipMemcpyHostToDevice,
@@ -762,13 +794,12 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
       HIP_TRY(hipMemcpyAsync(ad[k].seg, cps[k] + (size_t)seg * 3 * n, sizeof(float) * 3 * n,
                              hipMemcpyDeviceToDevice, c->stream));
     }
-    for (int s = 0; s < len; ++s)
-      HIP_TRY(fv_adj_forward_step(d_ad, ni, max_cells, (int)(s0 + s), s, 1, c->stream));
-    for (int s = len - 1; s >= 0; --s) {
-      if (s != len - 1)
-        HIP_TRY(fv_adj_forward_step(d_ad, ni, max_cells, (int)(s0 + s), s, 0, c->stream));
-      HIP_TRY(fv_adj_reverse_step(d_ad, ni, max_cells, max_ny, (int)(s0 + s), s, c->stream));
-    }
+    // recompute the segment's states (the last one is not needed)
+    for (int s = 0; s + 1 < len; ++s)
+      HIP_TRY(fv_adj_march_forward(d_ad, d_ix, (int)ix.size(), (int)(s0 + s), s, c->stream));
+    for (int s = len - 1; s >= 0; --s)
+      HIP_TRY(fv_adj_march_reverse(d_ad, d_ix, (int)ix.size(), d_iy, (int)iy.size(),
+                                   (int)(s0 + s), s, c->stream));
   }
 
   // ---- results ----
