@@ -1,5 +1,5 @@
 // apps/na/adjoint.h -- reverse-mode pieces of the "na" plug-in (device code),
-// used by the adjoint kernels (csrc/fv_adjoint.hip).
+// used by the adjoint march kernels (csrc/fv_adjoint_march.hip).
 //
 // Forward restated (generic direction; L = cell m-1, R = cell m, components
 // (rho', normal momentum, tangential momentum)), as apps/na/fv.h:88-139 and

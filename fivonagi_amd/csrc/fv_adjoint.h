@@ -1,5 +1,5 @@
-// fv_adjoint.h -- per-instance table of the adjoint passes (device memory),
-// shared by fv_adjoint.hip and the host runtime.
+// fv_adjoint.h -- per-instance table of the adjoint march kernels (device
+// memory), shared by fv_adjoint_march.hip and the host runtime.
 #ifndef FIVONAGI_FV_ADJOINT_H
 #define FIVONAGI_FV_ADJOINT_H
 
@@ -19,16 +19,7 @@ struct AdjDesc {
   const float* edge_phase;             // ny drive phases
   float* seg;                          // (K+1) states of the current segment, 3 grids each
   float* lam;                          // adjoint of the state (3 grids)
-  float* lam2;                         // scratch adjoints (3 grids each)
-  float* lamx;
-  float* qb;                           // recomputed q_b, q_y, q_xy (3 grids each)
-  float* qy;
-  float* qxy;
-  float* iw;                           // per interface: W (7), s (3)
-  float* pm;                           // per interface: P (3), M (3)
-  float* ig;                           // per interface: gW (7), gs (3), gdot (3)
-  int32_t* inb;                        // per interface: upwind side bits of the 3 families
-  float* iq;                           // per interface: gL (3), gR (3)
+  float* lamx;                         // adjoint of q_y between the two reverse kernels
   double* grad_amp;                    // ny, accumulated
   double* grad_phase;                  // ny, accumulated
   double* jstep;                       // per step: dt_n sum_box rho'_{n+1}^2

@@ -42,9 +42,8 @@ struct Cell { float r, u, v; };   // rho', rho u, rho v
 
 // Cell (i, r) of q_b = B(q_n, T) for the lane's column i (rows and columns
 // clamped into the grid; lanes and rows outside it only feed halo work).
-// Ghost rules as adj_bnd_fw / apps/common/boundary.h: left, then top, then
-// bottom, then right; the drive value p and the Taylor closed form are
-// evaluated in double as there.
+// Ghost rules as apps/common/boundary.h: left, then top, then bottom, then
+// right; the drive value p and the Taylor closed form are evaluated in double.
 struct Loader {
   const float* q;
   int nx, ny, n, bc, i, ic;
@@ -173,10 +172,12 @@ __global__ __launch_bounds__(64) void adj_march_x(const AdjDesc* __restrict__ ds
     load_lam(y0 - 1, la[1]);   // becomes row r-3 at r = y0 + 2
     load_lam(y0, la[0]);
   }
+  Cell nx0 = load(y0 - 3);     // q_b rows are fetched one iteration ahead
   for (int r = y0 - 3; r <= r_end; ++r) {
     // ---- q_b row r, y interface r, y limiter at r-1 ----
     b2 = b1; b1 = b0;
-    b0 = load(r);
+    b0 = nx0;
+    nx0 = load(r + 1);
     f2 = f1; f1 = f0;
     if (r >= y0 - 2) y_waves(b1, b0, beta, f0);
     if (r < y0) continue;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(64) void adj_march_x(const AdjDesc* __restrict__ ds
 
     if (!kRev) {
       // ---- source on row m-1 (apps/na/source.h:84-104), store q_{n+1} ----
-      const float irho = 1.0f / (1.0f + xq[0]);
+      const float irho = fvad::rcp(1.0f + xq[0]);
       const float u0 = xq[1] * irho, v0 = xq[2] * irho;
       // cross-lane moves with every lane active (a disabled source lane reads 0)
       const float uw = shr1(um1), ue = shl1(um1), vw = shr1(vm1), ve = shl1(vm1);
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(64) void adj_march_x(const AdjDesc* __restrict__ ds
                        ddy2 * ((ilo ? la[2][1] : 0.0f) + (ihi ? la[0][1] : 0.0f));
       const float gV = cc * mu2 + ddx2 * (shr1(mu2) + shl1(mu2)) +
                        ddy2 * ((ilo ? la[2][2] : 0.0f) + (ihi ? la[0][2] : 0.0f));
-      const float irho = 1.0f / (1.0f + xq[0]);
+      const float irho = fvad::rcp(1.0f + xq[0]);
       const float u = xq[1] * irho, v = xq[2] * irho;
       float lxy[3];
       lxy[0] = g0 - (gU * u + gV * v) * irho;
@@ -372,8 +373,8 @@ __global__ __launch_bounds__(64) void adj_march_y(const AdjDesc* __restrict__ ds
   float gd2[3] = {0.f, 0.f, 0.f}, gd3[3] = {0.f, 0.f, 0.f};
   int nb2 = 0, nb3 = 0;
   float gRp[3] = {0.f, 0.f, 0.f};   // gR of interface r-3 (cell r-3)
-  // lam_y rows r-1 (next), r-2, r-3; prefetch r
-  float ly[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  // lam_y rows r (prefetched), r-1, r-2, r-3
+  float ly[4][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   // held lam_b rows: 0 and 1 (bottom rule, for row 2), ny-3 (waits for the
   // top rule) and the sum of rows ny-2 and ny-1 (top rule, for row ny-3);
   // only the bottom and top items of a strip ever fill them
@@ -430,19 +431,23 @@ __global__ __launch_bounds__(64) void adj_march_y(const AdjDesc* __restrict__ ds
     }
   };
 
+  Cell nx0 = load(y0 - 3);     // q_b rows are fetched one iteration ahead
   for (int r = y0 - 3; r <= y1 + 2; ++r) {
     b3 = b2; b2 = b1; b1 = b0;
-    b0 = load(r);
+    b0 = nx0;
+    nx0 = load(r + 1);
     f3 = f2; f2 = f1; f1 = f0;
     if (r >= y0 - 2) y_waves(b1, b0, beta, f0);
-    // lam_y window: ly[2] = row r-3, ly[1] = r-2, ly[0] = r-1
+    // lam_y window: ly[3] = row r-3, ly[2] = r-2, ly[1] = r-1, ly[0] = r
+    // (row r is first used one iteration later)
     {
       float nl[3];
       load_ly(r, nl);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) { ly[2][c] = ly[1][c]; ly[1][c] = ly[0][c]; ly[0][c] = nl[c]; }
+      for (int c = 0; c < 3; ++c) {
+        ly[3][c] = ly[2][c]; ly[2][c] = ly[1][c]; ly[1][c] = ly[0][c]; ly[0][c] = nl[c];
+      }
     }
-    // the window now holds rows r-2 (ly[2]), r-1 (ly[1]), r (ly[0])
     if (r < y0) continue;
 
     // ---- limiter and its adjoint at interface r-1 ----
@@ -498,11 +503,9 @@ __global__ __launch_bounds__(64) void adj_march_y(const AdjDesc* __restrict__ ds
         // lam_b(r-3) = lam_y(r-3) + gR(interface r-3) + gL(interface r-2)
         const int jj = r - 3;
         float lb[3];
-        float lyj[3];
-        load_ly(jj, lyj);
-        lb[0] = lyj[0] + gRp[0] + gL[0];
-        lb[2] = lyj[2] + gRp[1] + gL[1];
-        lb[1] = lyj[1] + gRp[2] + gL[2];
+        lb[0] = ly[3][0] + gRp[0] + gL[0];
+        lb[2] = ly[3][2] + gRp[1] + gL[1];
+        lb[1] = ly[3][1] + gRp[2] + gL[2];
         if (jj == 0) { lb0[0] = lb[0]; lb0[1] = lb[1]; lb0[2] = lb[2]; }
         if (jj == 1) { lb1[0] = lb[0]; lb1[1] = lb[1]; lb1[2] = lb[2]; }
         if (jj == ny - 3) { lbt[0] = lb[0]; lbt[1] = lb[1]; lbt[2] = lb[2]; }

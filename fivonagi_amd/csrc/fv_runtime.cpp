@@ -666,12 +666,9 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   std::vector<fv::AdjDesc> ad(ni);
   std::vector<float*> cps(ni);
   std::vector<double*> dlog(ni), tlog(ni);
-  int max_cells = 0, max_ny = 0;
   for (int k = 0; k < ni; ++k) {
     const auto& p = c->params[k];
     const size_t n = (size_t)p.nx * p.ny;
-    max_cells = std::max(max_cells, (int)n);
-    max_ny = std::max(max_ny, p.ny);
     auto& a = ad[k];
     std::memset(&a, 0, sizeof(a));
     a.nx = p.nx; a.ny = p.ny; a.bc = p.bc; a.beta = (float)p.beta;
