@@ -99,3 +99,56 @@ def test_adjoint_of_a_linear_functional_of_the_source(oracle):
     J, ga, gp, gq, rc = oracle.adjoint(op, q0, 4, BOX)
     # columns 0, 1 are overwritten by the drive every step: no dependence
     assert not gq[:, :, 0:2].any()
+
+
+def mk_zero_gradient(q0):
+    """The adjoint case on the zero-gradient boundary (every copy rule, corners
+    included, transposed)."""
+    p = wl.gauss_pulse(NX, NY)
+    p.q0 = np.ascontiguousarray(q0)
+    return p
+
+
+def mk_taylor():
+    """Taylor shock (closed-form ghost ring) with a small smooth perturbation:
+    the bare profile is exactly uniform far from the front (tanh saturates),
+    and at a uniform state every wave is 0 and the limited correction is not
+    differentiable, so finite differences there measure a one-sided quantity."""
+    p = wl.taylor(eta_mult=1, theta=0.3, beta=3.5, delta=1e-3)
+    J_, I_ = np.mgrid[0:p.ny, 0:p.nx]
+    p.q0 = np.ascontiguousarray(p.q0 + 2e-5 * np.stack([
+        np.sin(0.31 * I_ + 0.17 * J_), np.cos(0.29 * I_ + 0.13 * J_),
+        np.sin(0.19 * J_ + 0.07 * I_)]))
+    return p
+
+
+@pytest.mark.parametrize("kind", ["zero_gradient", "taylor"])
+def test_state_adjoint_matches_finite_differences(oracle, kind):
+    rng = np.random.default_rng(4)
+    if kind == "zero_gradient":
+        amp, ph, q0 = setup_case(rng)
+        prob, box, dt = mk_zero_gradient(q0), (1, 30, 12, 9), 0.04
+    else:
+        prob, box, dt = mk_taylor(), (20, 40, 16, 12), 0.05
+        q0 = prob.q0.astype(np.float64)
+    n = 12
+    dts = np.full(n, dt)
+    op = to_oracle(prob, dt_seq=dts)
+    J, ga, gp, gq, rc = oracle.adjoint(op, q0, n, box)
+    assert rc == 0 and J > 0 and np.abs(gq).max() > 0
+    J_, I_ = np.mgrid[0:prob.ny, 0:prob.nx]
+    for trial in range(3):
+        D = np.stack([np.sin(rng.uniform(0.05, 0.4) * I_ + rng.uniform(0, 0.4) * J_
+                             + rng.uniform(0, 6)) for _ in range(3)])
+        fds = []
+        for h in (1e-4, 1e-5, 1e-6):
+            s = h * np.abs(q0).max() if kind == "taylor" else h
+            Ja, _ = oracle.objective(op, q0 + s * D, n, box)
+            Jb, _ = oracle.objective(op, q0 - s * D, n, box)
+            fds.append((Ja - Jb) / (2 * s))
+        assert best_rel(fds, (gq * D).sum()) < 1e-3, (kind, fds, (gq * D).sum())
+    if kind == "taylor":
+        # every ghost cell is the closed form: no dependence on q0 there
+        ghost = np.zeros((prob.ny, prob.nx), bool)
+        ghost[:2, :] = ghost[-2:, :] = ghost[:, :2] = ghost[:, -2:] = True
+        assert not gq[:, ghost].any()

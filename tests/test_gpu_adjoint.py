@@ -10,7 +10,8 @@ import pytest
 
 import fivonagi_amd as fa
 from tests.oracle_bridge import to_oracle
-from tests.test_adjoint import BOX, NSTEPS, NX, NY, best_rel, mk, setup_case
+from tests.test_adjoint import (BOX, NSTEPS, NX, NY, best_rel, mk, mk_taylor, mk_zero_gradient,
+                                setup_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -107,3 +108,26 @@ def test_feedback_dt_adjoint_runs_and_plain_forward_agrees():
     assert np.isfinite(ga1[0]).all() and J1[0] > 0
     assert abs(J1[0] - J2[0]) <= 1e-6 * J1[0]
     assert rel_max(ga1[0], ga2[0]) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["zero_gradient", "taylor"])
+def test_hip_state_adjoint_other_boundaries_matches_fp64(oracle, kind):
+    """dJ/dq0 through the zero-gradient copy rules (box on the left and top
+    ghost rows) and through the Taylor closed-form ring."""
+    rng = np.random.default_rng(4)
+    if kind == "zero_gradient":
+        amp, ph, q0 = setup_case(rng)
+        prob, box, dt = mk_zero_gradient(q0), (1, 30, 12, 9), 0.04
+    else:
+        prob, box, dt = mk_taylor(), (20, 40, 16, 12), 0.05
+    n = 12
+    prob.q0 = np.ascontiguousarray(prob.q0, np.float32)
+    dts = np.full(n + 5, dt)
+    J, ga, gp, lam = gpu_adjoint(prob, n, dts, box)
+    op = to_oracle(prob, dt_seq=dts)
+    Jo, gao, gpo, gqo, rc = oracle.adjoint(op, prob.q0.astype(np.float64), n, box)
+    assert rc == 0
+    e = [abs(J - Jo) / Jo] + [rel_max(lam[k], gqo[k]) for k in range(3)]
+    print(f"[adjoint {kind}] J rel err {e[0]:.2e}; dJ/dq0 rel err " +
+          " ".join(f"{v:.2e}" for v in e[1:]), flush=True)
+    assert e[0] < 1e-5 and max(e[1:]) < 1e-3
