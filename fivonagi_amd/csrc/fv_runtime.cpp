@@ -20,7 +20,10 @@
 #include "fv_types.h"
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
-                                     const uint64_t* rowmask, int n_items, hipStream_t stream);
+                                     const uint64_t* rowmask, int n_items, int phase,
+                                     hipStream_t stream);
+extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, int phase,
+                                       hipStream_t stream);
 extern "C" hipError_t fv_step_waves_per_cu(int* waves);
 extern "C" hipError_t fv_adj_march_forward(const fv::AdjDesc* ds, const fv::AdjItem* items,
                                            int n_items, int k, int slot, hipStream_t s);
@@ -29,7 +32,7 @@ extern "C" hipError_t fv_adj_march_reverse(const fv::AdjDesc* ds, const fv::AdjI
                                            int n_items_y, int k, int slot, hipStream_t s);
 extern "C" hipError_t fv_adj_march_occupancy(int* x_fwd, int* x_rev, int* y_rev);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
-                                              int max_ring, hipStream_t stream);
+                                              int max_ring, int phase, hipStream_t stream);
 
 namespace {
 
@@ -90,7 +93,9 @@ struct fivonagi_ctx {
   fv::InstDesc* d_desc = nullptr;
   fv::WorkItem* d_items = nullptr;
   uint64_t* d_rowmask = nullptr;
-  fv::Ctl* d_ctl = nullptr;
+  fv::Ctl* d_ctl = nullptr;                // 2 control blocks per instance
+  fv::SyncSlots* d_sync = nullptr;         // 3 reduction slot sets per instance
+  int64_t attempts = 0;                    // step attempts launched (the phase is this mod 6)
   int graph_len = 0;
   int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
   bool unlimited = false;                  // targets set to "run every attempt"
@@ -231,50 +236,84 @@ void set_ctl_derived(fv::Ctl& t, const fv::InstDesc& d) {
   t.drive_c = std::cos(th);
 }
 
+// The control blocks after the last launched attempt: the settle kernel
+// folds that attempt's reduction into block (a+1)%2 (a = next attempt index),
+// which is where the next launch will store the same result.
 int read_ctls(fivonagi_ctx* c, std::vector<fv::Ctl>& out) {
-  out.resize(c->n_inst);
-  HIP_TRY(hipMemcpyAsync(out.data(), c->d_ctl, sizeof(fv::Ctl) * c->n_inst,
+  const int phase = (int)(c->attempts % 6);
+  HIP_TRY(fv_launch_settle(c->d_desc, c->n_inst, phase, c->stream));
+  std::vector<fv::Ctl> both(2 * (size_t)c->n_inst);
+  HIP_TRY(hipMemcpyAsync(both.data(), c->d_ctl, sizeof(fv::Ctl) * both.size(),
                          hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  out.resize(c->n_inst);
+  const int b = (phase + 1) & 1;
+  for (int k = 0; k < c->n_inst; ++k) out[k] = both[2 * (size_t)k + b];
+  return 0;
+}
+
+// Replace the control blocks the next attempt starts from: block a%2, with
+// the reduction of attempt a-1 marked as already folded in.
+int write_ctls(fivonagi_ctx* c, const std::vector<fv::Ctl>& in) {
+  const int phase = (int)(c->attempts % 6);
+  const uint32_t zero = 0u;
+  for (int k = 0; k < c->n_inst; ++k) {
+    HIP_TRY(hipMemcpyAsync(c->d_ctl + 2 * (size_t)k + (phase & 1), &in[k], sizeof(fv::Ctl),
+                           hipMemcpyHostToDevice, c->stream));
+    fv::SyncSlots* prev = c->d_sync + 3 * (size_t)k + (phase + 2) % 3;
+    HIP_TRY(hipMemcpyAsync(&prev->ran, &zero, sizeof(uint32_t), hipMemcpyHostToDevice,
+                           c->stream));
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 // One step attempt: the Taylor ghost pre-pass (only if a Taylor instance
-// exists) and the fused step.
-hipError_t launch_one(fivonagi_ctx* c) {
+// exists) and the fused step, for attempt index = phase (mod 6).
+hipError_t launch_phase(fivonagi_ctx* c, int phase) {
   if (c->taylor_ring > 0) {
-    hipError_t e = fv_launch_taylor_ghosts(c->d_desc, c->n_inst, c->taylor_ring, c->stream);
+    hipError_t e =
+        fv_launch_taylor_ghosts(c->d_desc, c->n_inst, c->taylor_ring, phase, c->stream);
     if (e != hipSuccess) return e;
   }
-  return fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(), c->stream);
+  return fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(), phase,
+                        c->stream);
 }
 
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
-  const int G = 32;   // attempts per captured graph
-  if (n >= G) {
-    if (!c->graph_exec) {
-      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < G; ++k) {
-        hipError_t e = launch_one(c);
-        if (e != hipSuccess) {
-          hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(c->stream, &g);
-          if (g) (void)hipGraphDestroy(g);
-          return hip_fail(e, "fv_launch_step (capture)");
-        }
-      }
-      HIP_TRY(hipStreamEndCapture(c->stream, &c->graph));
-      HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-      c->graph_len = G;
-    }
-    while (n >= G) {
-      HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
-      n -= G;
-    }
-  }
-  for (int64_t k = 0; k < n; ++k) {
-    hipError_t e = launch_one(c);
+  const int G = 30;   // attempts per captured graph (a multiple of 6: phases 0..5 repeat)
+  // single launches up to a phase-0 boundary, then whole graphs, then singles
+  while (n > 0 && (c->attempts % 6 != 0 || n < G)) {
+    hipError_t e = launch_phase(c, (int)(c->attempts % 6));
     if (e != hipSuccess) return hip_fail(e, "fv_launch_step");
+    ++c->attempts;
+    --n;
+  }
+  if (n >= G && !c->graph_exec) {
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < G; ++k) {
+      hipError_t e = launch_phase(c, k % 6);
+      if (e != hipSuccess) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e, "fv_launch_step (capture)");
+      }
+    }
+    HIP_TRY(hipStreamEndCapture(c->stream, &c->graph));
+    HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    c->graph_len = G;
+  }
+  while (n >= G) {
+    HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    c->attempts += G;
+    n -= G;
+  }
+  while (n > 0) {
+    hipError_t e = launch_phase(c, (int)(c->attempts % 6));
+    if (e != hipSuccess) return hip_fail(e, "fv_launch_step");
+    ++c->attempts;
+    --n;
   }
   return 0;
 }
@@ -289,10 +328,7 @@ int write_targets(fivonagi_ctx* c, const std::vector<int64_t>& tgt) {
   std::vector<fv::Ctl> ctl;
   if (int rc = read_ctls(c, ctl)) return rc;
   for (int k = 0; k < c->n_inst; ++k) ctl[k].target_acc = tgt[k];
-  HIP_TRY(hipMemcpyAsync(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * c->n_inst,
-                         hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  return 0;
+  return write_ctls(c, ctl);
 }
 
 }  // namespace
@@ -329,13 +365,13 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
   c->stream = c->own_stream;
 
-  c->d_ctl = (fv::Ctl*)c->dalloc(sizeof(fv::Ctl) * n_inst, &e);
+  c->d_ctl = (fv::Ctl*)c->dalloc(sizeof(fv::Ctl) * 2 * n_inst, &e);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc ctl"); }
-  auto* d_sync = (fv::SyncSlots*)c->dalloc(sizeof(fv::SyncSlots) * n_inst, &e);
+  c->d_sync = (fv::SyncSlots*)c->dalloc(sizeof(fv::SyncSlots) * 3 * n_inst, &e);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc sync"); }
-  e = hipMemset(d_sync, 0, sizeof(fv::SyncSlots) * n_inst);
+  e = hipMemset(c->d_sync, 0, sizeof(fv::SyncSlots) * 3 * n_inst);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemset sync"); }
-  std::vector<fv::Ctl> ctl(n_inst);
+  std::vector<fv::Ctl> ctl(2 * (size_t)n_inst);
   for (int k = 0; k < n_inst; ++k) {
     const auto& q = p[k];
     auto& d = c->desc[k];
@@ -383,8 +419,8 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
         d.cfl_mask = fv::kCflBitmap;
       }
     }
-    d.ctl = c->d_ctl + k;
-    d.sync = d_sync + k;
+    d.ctl = c->d_ctl + 2 * k;
+    d.sync = c->d_sync + 3 * k;
     d.dt_seq = nullptr;
     d.dt_seq_len = 0;
     d.dx = q.dx; d.dy = q.dy; d.delta = q.delta;
@@ -400,14 +436,16 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
       c->taylor_ring = std::max(c->taylor_ring, 4 * q.nx + 4 * (q.ny - 4));
     d.beta2m2 = (float)(2.0 * (q.beta - 1.0));
 
-    auto& t = ctl[k];
+    // attempt 0 starts from block 0; slot set 2 (attempt -1) never ran
+    auto& t = ctl[2 * (size_t)k];
     std::memset(&t, 0, sizeof(t));
     t.dt = q.dt0;
     t.T = q.T0;
     t.target_acc = 0;
     set_ctl_derived(t, d);
+    ctl[2 * (size_t)k + 1] = t;
   }
-  e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * n_inst, hipMemcpyHostToDevice);
+  e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * 2 * n_inst, hipMemcpyHostToDevice);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
 
   build_items(c, auto_rows(c->params, resident_waves(c->device)));
@@ -448,7 +486,7 @@ int fivonagi_set_dt_sequence(fivonagi_ctx* c, int inst, const double* dt, int64_
     if (t.n_att >= n) return fail(FIVONAGI_ERR_ARG, "dt sequence shorter than attempts made");
     t.dt = dt[t.n_att];
     set_ctl_derived(t, d);
-    HIP_TRY(hipMemcpy(c->d_ctl + inst, &t, sizeof(fv::Ctl), hipMemcpyHostToDevice));
+    if (int rc = write_ctls(c, ctl)) return rc;
   } else {
     d.dt_seq = nullptr;
     d.dt_seq_len = 0;
@@ -512,15 +550,14 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
       (void)hipMemcpyAsync(c->desc[k].q[saved[k].cur][f], keep[k] + (size_t)f * n,
                            sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
   }
-  hipError_t e = hipMemcpyAsync(c->d_ctl, saved.data(), sizeof(fv::Ctl) * c->n_inst,
-                                hipMemcpyHostToDevice, c->stream);
   hipError_t es = hipStreamSynchronize(c->stream);
+  int rc3 = (rc || rc2 || es != hipSuccess) ? 0 : write_ctls(c, saved);
   release();
   c->unlimited = false;
   if (rc) return rc;
   if (rc2) return rc2;
-  if (e != hipSuccess) return hip_fail(e, "restore (autotune)");
   if (es != hipSuccess) return hip_fail(es, "hipStreamSynchronize (autotune)");
+  if (rc3) return rc3;
   if (rows_out) *rows_out = best_h;
   return 0;
 }

@@ -20,14 +20,15 @@
 // by renaming instead of moves.  No LDS and no barriers: the 3-column halo on
 // each side makes the waves independent.
 //
-// CFL and accept/reject: each wave reduces its cells' cfl to one float, then
-// one relaxed agent-scope atomicMax + one ticket atomicAdd on its group's
-// 128-byte line (8 groups per instance); the last wave of a group bumps the
-// instance ticket, and the last group's last wave (the finalizer) decides
-// accept/reject, dt, T, and flips the ping-pong index in device memory.  No
-// memory fences are used: the only data crossing waves inside a launch is
-// read and written with atomics, and each wave waits for its max to complete
-// before it takes a ticket.  Everything else crosses launches.
+// CFL and accept/reject: each wave reduces its cells' cfl to one float and
+// folds it into its group's 128-byte line (8 groups per instance) with one
+// relaxed agent-scope atomicMax that nothing waits for.  Attempt a's maxima
+// are read by every wave of launch a+1, which decides accept/reject, dt, T
+// and the ping-pong index of attempt a+1 itself (advance(), identical in
+// every wave); one wave per instance stores the result for the launch after.
+// The control blocks and reduction slots rotate with the attempt index
+// (2 control blocks, 3 slot sets), so no launch reads what it writes and the
+// step ends when its last wave ends -- no ticket chain, no fences.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -109,6 +110,80 @@ __device__ __forceinline__ uint32_t axchg_relaxed(uint32_t* p, uint32_t v) {
   return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Control block of attempt a+1 from the one attempt a ran with and attempt
+// a's reduction slots: common/main.cu:199-230 with a clean retry on
+// rejection.  log: record dt and T of an accepted step (adjoint runs); every
+// writer records the same values.
+__device__ Ctl advance(const InstDesc& D, const Ctl& p, const SyncSlots* __restrict__ sl,
+                       bool log) {
+  Ctl o = p;
+  if (!uni(sl->ran)) return o;   // attempt a did not run (instance done or stopped)
+  if (uni(sl->err_flag)) {
+    o.err = 2;                   // imaginary eigenvalue: step discarded, instance stops
+    return o;
+  }
+  float cfl32 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) cfl32 = fmaxf(cfl32, unif(__uint_as_float(sl->grp[k].cfl_bits)));
+  const double cfl = (double)cfl32;
+  const bool presc = D.dt_seq != nullptr;
+  const bool acc = presc || (cfl < D.cfl_max);
+  const int64_t att = p.n_att + 1;
+  o.n_att = att;
+  o.last_cfl = cfl;
+  double T = p.T;
+  if (acc) {
+    if (log && D.dt_log) {
+      const int64_t idx = p.n_acc - D.log_base;
+      if (idx >= 0 && idx < D.log_len) {
+        D.dt_log[idx] = p.dt;
+        D.T_log[idx] = p.T;
+      }
+    }
+    T = p.T + p.dt;
+    o.T = T;
+    o.n_acc = p.n_acc + 1;
+    o.cur = p.cur ^ 1;
+  } else {
+    o.n_rej = p.n_rej + 1;
+  }
+  double ndt;
+  if (presc) {
+    ndt = (att < D.dt_seq_len) ? D.dt_seq[att] : p.dt;
+    if (att >= D.dt_seq_len) o.err = 9;   // ran past the prescribed sequence
+  } else {
+    ndt = p.dt * D.cfl_wish / cfl;
+    if (!acc && ndt > p.dt * 0.9) ndt = p.dt * 0.9;
+  }
+  o.dt = ndt;
+  o.dtx = (float)(ndt * D.inv_dx);
+  o.dty = (float)(ndt * D.inv_dy);
+  o.ddx2 = (float)(ndt * D.dif_x);
+  o.ddy2 = (float)(ndt * D.dif_y);
+  // drive phase omega*T in turns, reduced in double to [0, 1); the step uses
+  // sin/cos in FP32
+  const double turns = D.omega_turns * T;
+  float fs, fc;
+  sincospif((float)(2.0 * (turns - floor(turns))), &fs, &fc);
+  o.drive_s = fs;
+  o.drive_c = fc;
+  return o;
+}
+
+// Control block of the attempt a launch runs (phase = a mod 6), read-only.
+__device__ __forceinline__ Ctl current(const InstDesc& D, int phase, bool log) {
+  const Ctl* c = D.ctl + (phase & 1);
+  Ctl p;
+  p.dt = c->dt; p.T = c->T; p.last_cfl = c->last_cfl;
+  p.drive_s = c->drive_s; p.drive_c = c->drive_c;
+  p.n_acc = uni64(c->n_acc); p.n_rej = uni64(c->n_rej); p.n_att = uni64(c->n_att);
+  p.target_acc = uni64(c->target_acc);
+  p.cur = uni(c->cur); p.err = uni(c->err);
+  p.dtx = c->dtx; p.dty = c->dty; p.ddx2 = c->ddx2; p.ddy2 = c->ddy2;
+  p.pad0 = 0; p.pad1 = 0;
+  return advance(D, p, D.sync + (phase + 2) % 3, log);
+}
+
 struct Row {         // one cell of a row as the y sweep sees it
   float q1, q2, q3;  // rho', rho u, rho v
   float un, ut;      // y normal velocity v = q3/rho, tangential u = q2/rho
@@ -127,7 +202,7 @@ struct XY {          // one cell of q_xy, with its velocities for the source
 __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
                                                      const WorkItem* __restrict__ items,
                                                      const uint64_t* __restrict__ rowmask,
-                                                     int n_items) {
+                                                     int n_items, int phase) {
   // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
   // run of work items so vertically adjacent chunks of a strip (which share
   // halo rows) hit the same L2.
@@ -139,22 +214,28 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   }
   const WorkItem it = items[item];
   const InstDesc& D = insts[it.inst];
-  Ctl* ctl = D.ctl;
+  const int local = item - D.first_item;
+  const int lane = threadIdx.x;
 
-  // ---- per-step scalars (written by the previous step's finalizer) ----
-  // The control block is read through vector loads (it is written in this
-  // launch); readfirstlane moves the uniform values to scalar registers so
-  // the state pointers stay scalar and free VGPRs.
-  const int64_t n_acc = uni64(ctl->n_acc);
-  if (uni(ctl->err) != 0 || n_acc >= uni64(ctl->target_acc)) return;
-  // counters the finalizer updates, read now so its tail has no loads
-  const int64_t n_att = uni64(ctl->n_att), n_rej = uni64(ctl->n_rej);
-  const int cur = uni(ctl->cur);
-  const double dt = ctl->dt;
-  const double Tnow = ctl->T;
-  const float dtx = unif(ctl->dtx), dty = unif(ctl->dty);
-  const float ddx2 = unif(ctl->ddx2), ddy2 = unif(ctl->ddy2);
-  const float sin_t = unif((float)ctl->drive_s), cos_t = unif((float)ctl->drive_c);
+  // ---- control block of this attempt: the previous attempt folded in ----
+  const Ctl P = current(D, phase, local == 0);
+  const bool runs = P.err == 0 && P.n_acc < P.target_acc;
+  if (local == 0 && lane == 0) {
+    // one wave per instance: publish the control block for the next launch,
+    // clear the slots the next attempt reduces into, mark this attempt
+    D.ctl[(phase + 1) & 1] = P;
+    SyncSlots* nx_sl = D.sync + (phase + 1) % 3;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) nx_sl->grp[k].cfl_bits = 0u;
+    nx_sl->err_flag = 0u;
+    nx_sl->ran = 0u;
+    if (runs) D.sync[phase % 3].ran = 1u;
+  }
+  if (!runs) return;
+  const int cur = P.cur;
+  const float dtx = unif(P.dtx), dty = unif(P.dty);
+  const float ddx2 = unif(P.ddx2), ddy2 = unif(P.ddy2);
+  const float sin_t = unif((float)P.drive_s), cos_t = unif((float)P.drive_c);
   const float hbeta = 0.5f * D.beta2m2;     // BETA - 1
   const int nx = D.nx, ny = D.ny;
   const int bc = D.bc;
@@ -168,7 +249,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   kfloat* __restrict__ edge_c = (kfloat*)D.edge_c;
   kfloat* __restrict__ edge_s = (kfloat*)D.edge_s;
 
-  const int lane = threadIdx.x;
   const int i = it.x0 - kHalo + lane;
   const bool col_in = (i >= 0) && (i < nx);
   const bool col_x = col_in && (i >= 1);     // lane holds an x interface of the grid
@@ -354,87 +434,24 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     default: source_store(y1 - 1, X2, X0, X1); break;
   }
 
-  // ---- CFL / error reduction, then the instance finalizer ----
-  // per-cell cfl = |s| * dt/d (fv.h:148-180); max over the wave's cells
+  // ---- CFL / error reduction ----
+  // per-cell cfl = |s| * dt/d (fv.h:148-180); max over the wave's cells,
+  // folded into the group's slot of this attempt (read by the next launch)
   const float cflw = wave_max(fmaxf(Sy * dty, Sx * dtx));
-  const bool any_err = err != 0;
   if (lane != 0) return;
-  SyncSlots* sy = D.sync;
-  const int local = item - D.first_item;
-  const int n = D.n_items;
-  const int g = local % kGroups;
-  const int gsize = n / kGroups + (g < n % kGroups ? 1 : 0);
-  const int ngroups = n < kGroups ? n : kGroups;
-  if (any_err) aadd_relaxed(&sy->err_flag, 1u);
-  const uint32_t prev = amax_relaxed(&sy->grp[g].cfl_bits, __float_as_uint(cflw));
-  // the max (and the error flag) must be performed before the ticket is
-  // taken: consume the returned value, then wait for every memory operation
-  asm volatile("" ::"v"(prev));
-  __builtin_amdgcn_s_waitcnt(0);
-  if (aadd_relaxed(&sy->grp[g].ticket, 1u) != (uint32_t)(gsize - 1)) return;
-  if (aadd_relaxed(&sy->top_ticket, 1u) != (uint32_t)(ngroups - 1)) return;
+  SyncSlots* sl = D.sync + phase % 3;
+  if (err != 0) aadd_relaxed(&sl->err_flag, 1u);
+  amax_relaxed(&sl->grp[local % kGroups].cfl_bits, __float_as_uint(cflw));
+}
 
-  // finalizer: every wave of the instance has taken its ticket.  The group
-  // slots are read and cleared by independent exchanges/stores issued back
-  // to back (slots of groups an instance never uses stay 0).
-  uint32_t gm[kGroups];
-#pragma unroll
-  for (int k = 0; k < kGroups; ++k) gm[k] = axchg_relaxed(&sy->grp[k].cfl_bits, 0u);
-  const uint32_t e = axchg_relaxed(&sy->err_flag, 0u);
-#pragma unroll
-  for (int k = 0; k < kGroups; ++k)
-    __hip_atomic_store(&sy->grp[k].ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&sy->top_ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  float cfl32 = 0.0f;
-#pragma unroll
-  for (int k = 0; k < kGroups; ++k) cfl32 = fmaxf(cfl32, __uint_as_float(gm[k]));
-  if (e) {
-    ctl->err = 2;   // imaginary eigenvalue: step discarded, instance stops
-    return;
-  }
-  // common/main.cu:199-230 with a clean retry on rejection
-  const double cfl = (double)cfl32;
-  const bool presc = D.dt_seq != nullptr;
-  const bool acc = presc || (cfl < D.cfl_max);
-  const int64_t att = n_att + 1;
-  ctl->n_att = att;
-  ctl->last_cfl = cfl;
-  double T = Tnow;
-  if (acc) {
-    if (D.dt_log) {   // record the executed dt/T sequence (adjoint runs)
-      const int64_t idx = n_acc - D.log_base;
-      if (idx >= 0 && idx < D.log_len) {
-        D.dt_log[idx] = dt;
-        D.T_log[idx] = Tnow;
-      }
-    }
-    T = Tnow + dt;
-    ctl->T = T;
-    ctl->n_acc = n_acc + 1;
-    ctl->cur = cur ^ 1;
-  } else {
-    ctl->n_rej = n_rej + 1;
-  }
-  double ndt;
-  if (presc) {
-    ndt = (att < D.dt_seq_len) ? D.dt_seq[att] : dt;
-    if (att >= D.dt_seq_len) ctl->err = 9;   // ran past the prescribed sequence
-  } else {
-    ndt = dt * D.cfl_wish / cfl;
-    if (!acc && ndt > dt * 0.9) ndt = dt * 0.9;
-  }
-  ctl->dt = ndt;
-  ctl->dtx = (float)(ndt * D.inv_dx);
-  ctl->dty = (float)(ndt * D.inv_dy);
-  ctl->ddx2 = (float)(ndt * D.dif_x);
-  ctl->ddy2 = (float)(ndt * D.dif_y);
-  // drive phase omega*T in turns, reduced in double to [0, 1); the step
-  // uses sin/cos in FP32
-  const double turns = D.omega_turns * T;
-  float fs, fc;
-  sincospif((float)(2.0 * (turns - floor(turns))), &fs, &fc);
-  ctl->drive_s = fs;
-  ctl->drive_c = fc;
+// Control block after the last launched attempt (phase = index of the next
+// attempt mod 6), stored where that attempt's launch will store it too; the
+// host reads it after a synchronisation.  One block per instance.
+__global__ __launch_bounds__(64) void fv_settle_kernel(const InstDesc* __restrict__ insts,
+                                                       int phase) {
+  const InstDesc& D = insts[blockIdx.x];
+  const Ctl P = current(D, phase, true);
+  if (threadIdx.x == 0) D.ctl[(phase + 1) & 1] = P;
 }
 
 // Closed-form ghost ring of the Taylor-shock instances (apps/taylor-angle-xy/
@@ -442,15 +459,16 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
 // reads it.  The values depend on the cell and on T only, never on the state,
 // so writing them in place is the reference's boundary kernel.  blockIdx.y =
 // instance; the ring (2 cells wide) is enumerated by a flat index.
-__global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __restrict__ insts) {
+__global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __restrict__ insts,
+                                                             int phase) {
   const InstDesc& D = insts[blockIdx.y];
   if (D.bc != kBcTaylor) return;
-  const Ctl* ctl = D.ctl;
-  if (ctl->err != 0 || ctl->n_acc >= ctl->target_acc) return;
+  const Ctl P = current(D, phase, false);
+  if (P.err != 0 || P.n_acc >= P.target_acc) return;
   const int nx = D.nx, ny = D.ny;
   const int ring = 4 * nx + 4 * (ny - 4);
-  const double T = ctl->T;
-  const int cur = ctl->cur;
+  const double T = P.T;
+  const int cur = P.cur;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ring; k += gridDim.x * blockDim.x) {
     int i, j;
     if (k < 4 * nx) {            // rows 0, 1, ny-2, ny-1
@@ -476,11 +494,18 @@ __global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __
 }  // namespace fv
 
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
-                                              int max_ring, hipStream_t stream) {
+                                              int max_ring, int phase, hipStream_t stream) {
   if (n_inst <= 0) return hipSuccess;
   const int blocks = (max_ring + 255) / 256;
   hipLaunchKernelGGL(fv::fv_taylor_ghost_kernel, dim3(blocks, n_inst), dim3(256), 0, stream,
-                     insts);
+                     insts, phase);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, int phase,
+                                       hipStream_t stream) {
+  if (n_inst <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fv::fv_settle_kernel, dim3(n_inst), dim3(64), 0, stream, insts, phase);
   return hipGetLastError();
 }
 
@@ -490,9 +515,10 @@ extern "C" hipError_t fv_step_waves_per_cu(int* waves) {
 }
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
-                                     const uint64_t* rowmask, int n_items, hipStream_t stream) {
+                                     const uint64_t* rowmask, int n_items, int phase,
+                                     hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
   hipLaunchKernelGGL(fv::fv_step_kernel, dim3(n_items), dim3(fv::kLanes), 0, stream, insts,
-                     items, rowmask, n_items);
+                     items, rowmask, n_items, phase);
   return hipGetLastError();
 }

@@ -16,9 +16,11 @@ constexpr int kStripCols = kLanes - 2 * kHalo;   // 58
 enum BoundaryKind : int32_t { kBcZeroGrad = 0, kBcEdgeDrive = 1, kBcTaylor = 2 };
 enum CflMask : int32_t { kCflAll = 0, kCflEven = 1, kCflBitmap = 2 };
 
-// Mutable per-instance control block (device memory).  Written only by the
-// last wave of an instance in each step ("finalizer"), read by every wave at
-// the start of the next step.
+// Per-instance control block (device memory): the parameters of one step
+// attempt.  Two per instance, alternating with the attempt index: launch a
+// reads block a%2 (attempt a-1's parameters) and the reduction slots of
+// attempt a-1, forms attempt a's parameters in every wave, and one wave
+// stores them into block (a+1)%2.
 struct Ctl {
   double dt;            // dt of the next attempt
   double T;             // time at the start of the next attempt
@@ -33,28 +35,28 @@ struct Ctl {
   uint32_t pad0, pad1;
 };
 
-// Reduction slots of the running attempt, per instance: the waves of an
-// instance are split into kGroups groups (local item index mod kGroups), each
-// with its own 128-byte line, so the same-address atomics of one step are
-// spread over kGroups + 1 lines (one line taking every wave's atomics slows
-// the whole step).  Reset by the finalizer.
+// Reduction slots of one attempt, per instance: the waves of an instance
+// are split into kGroups groups (local item index mod kGroups), each with its
+// own 128-byte line, so the same-address atomics of one step are spread over
+// kGroups lines (one line taking every wave's atomics slows the whole step).
+// Three sets per instance, by attempt index mod 3: launch a reduces into set
+// a%3, reads set (a-1)%3 and clears set (a+1)%3.
 constexpr int kGroups = 8;
 struct alignas(128) GroupSlot {
   uint32_t cfl_bits;    // max cfl of the group's cells (float bits, >= 0)
-  uint32_t ticket;      // waves of the group done
-  uint32_t pad[30];
+  uint32_t pad[31];
 };
 struct alignas(128) SyncSlots {
   GroupSlot grp[kGroups];
-  uint32_t top_ticket;  // groups done
   uint32_t err_flag;    // any imaginary eigenvalue in the attempt
+  uint32_t ran;         // the attempt ran (the instance was not done or stopped)
   uint32_t pad[30];
 };
 
 // Read-only per-instance description (device memory).
 struct InstDesc {
   float* q[2][3];       // ping-pong SoA state buffers, offset i + j*nx
-  Ctl* ctl;
+  Ctl* ctl;             // 2 control blocks (attempt index mod 2)
   const float* edge_c;  // amp_j * cos(phase_j), length ny (kBcEdgeDrive)
   const float* edge_s;  // amp_j * sin(phase_j), length ny
   const double* dt_seq; // prescribed dt per attempt, or null
@@ -69,7 +71,7 @@ struct InstDesc {
   double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff;
   int32_t nx, ny;
   int32_t bc, cfl_mask;  // cfl_mask: host-side record of the fold subset kind
-  SyncSlots* sync;
+  SyncSlots* sync;      // 3 reduction slot sets (attempt index mod 3)
   int32_t n_items;      // work items (waves) of this instance per step
   int32_t first_item;   // index of the instance's first work item
   int32_t tay_jc;
