@@ -177,3 +177,27 @@ def test_autotune_leaves_state_and_results_unchanged():
         plain = [ctx.collect(k)[0] for k in range(2)]
     for a, b in zip(tuned, plain):
         assert np.array_equal(a, b)
+
+
+def test_call_pattern_does_not_change_results():
+    """Each launch folds the previous attempt's reduction into its own control
+    block (two control blocks and three slot sets rotate with the attempt
+    index); a host read settles the last attempt first.  One long call, single
+    steps with reads in between, and a mix (graph replays of 30 attempts with
+    single launches around them) must give the same state, time and counters,
+    rejections included."""
+    prob = small_pulse(64, 64, dt0=0.5)   # the first attempts are rejected
+
+    def run(pattern):
+        with fa.Context(prob) as ctx:
+            for n in pattern:
+                ctx.step(n)
+                ctx.info(0)
+            return ctx.collect(0)
+
+    ref_q, ref_s = run([67])
+    for pattern in ([1] * 67, [7, 30, 1, 29], [2, 31, 34]):
+        q, s = run(pattern)
+        assert np.array_equal(q, ref_q), pattern
+        assert s == ref_s, (pattern, s, ref_s)
+    assert ref_s["n_rejected"] > 0 and ref_s["n_accepted"] == 67
