@@ -19,8 +19,8 @@ OK, ERR_EIGEN, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 8, 9, 10
 EXPORTS = (
     "fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence", "fivonagi_step",
     "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect", "fivonagi_info",
-    "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_destroy",
-    "fivonagi_last_error",
+    "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_set_skew",
+    "fivonagi_destroy", "fivonagi_last_error",
 )
 
 
@@ -48,6 +48,7 @@ class Info(C.Structure):
         ("n_accepted", C.c_int64), ("n_rejected", C.c_int64), ("n_attempts", C.c_int64),
         ("error", C.c_int32), ("cur", C.c_int32),
         ("n_items", C.c_int32), ("rows_per_wave", C.c_int32),
+        ("skew", C.c_double),
     ]
 
 
@@ -92,6 +93,7 @@ def load():
                                           C.POINTER(AdjointOpts), dp, dp, dp]
     lib.fivonagi_adjoint_state.argtypes = [vp, C.c_int, fp, fp, fp]
     lib.fivonagi_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int32)]
+    lib.fivonagi_set_skew.argtypes = [vp, C.c_double]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
     lib.fivonagi_last_error.argtypes = []
@@ -99,7 +101,7 @@ def load():
     for name in ("fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
                  "fivonagi_step", "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect",
                  "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state",
-                 "fivonagi_autotune"):
+                 "fivonagi_autotune", "fivonagi_set_skew"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -264,6 +266,10 @@ class Context:
         _check(load().fivonagi_adjoint_state(
             self._h, inst, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in ls]))
         return np.stack(ls)
+
+    def set_skew(self, skew):
+        """Height skew of the work items (< 0: automatic); results do not depend on it."""
+        _check(load().fivonagi_set_skew(self._h, C.c_double(skew)))
 
     def autotune(self, trial_steps=20):
         """Pick the rows per wave with the fastest step; state is unchanged."""

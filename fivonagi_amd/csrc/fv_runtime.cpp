@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -21,6 +22,7 @@
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0,
                                      hipStream_t stream);
 extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, int phase,
                                        hipStream_t stream);
@@ -87,6 +89,10 @@ struct fivonagi_ctx {
   std::vector<fv::InstDesc> desc;          // host copy of the instance table
   std::vector<fv::WorkItem> items;
   std::vector<uint64_t> rowmask;           // CFL lane mask per item row (fv::WorkItem::mrow)
+  float skew = -1.0f;                      // height skew of the work items (< 0: automatic)
+  float skew_used = 0.0f;                  // the skew of the current work list
+  long cus = 256;                          // CUs of the device
+  int per_cu = 16;                         // resident step waves per CU
   std::vector<std::vector<uint32_t>> fold_bits;  // kCflBitmap: bit i + j*nx = cell counted
   std::vector<std::vector<double>> dt_seq;
   std::vector<void*> allocs;               // every device allocation
@@ -145,12 +151,16 @@ int auto_rows(const std::vector<fivonagi_params>& ps, long target_waves) {
   return 4096;
 }
 
-long resident_waves(int device) {
-  int cus = 0, per_cu = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      fv_step_waves_per_cu(&per_cu) != hipSuccess || cus <= 0 || per_cu <= 0)
-    return 3072;   // 256 CUs x 12 waves
-  return (long)cus * per_cu;
+// CUs of the device and resident step waves per CU (4 SIMDs per CU)
+void device_shape(int device, long* cus, int* per_cu) {
+  int n = 0, w = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      fv_step_waves_per_cu(&w) != hipSuccess || n <= 0 || w <= 0) {
+    n = 256;   // MI355X: 256 CUs, 4 waves of the step kernel per SIMD
+    w = 16;
+  }
+  *cus = n;
+  *per_cu = w;
 }
 
 // Is cell (i, j) of instance k one the CFL reduction counts?
@@ -162,24 +172,87 @@ bool cfl_counted(const fivonagi_ctx* c, int k, int i, int j) {
   return true;
 }
 
-void build_items(fivonagi_ctx* c, int auto_h) {
+// Row boundaries of one strip cut into `chunks` work items.  With skew > 0
+// the items get `levels` height levels (item j: level j % levels) with
+// weights falling linearly from 1 + skew to 1 - skew (mean 1), so the strip
+// keeps its number of items, some taller and some shorter.
+std::vector<int> chunk_bounds(int ny, int chunks, float skew, int levels) {
+  const int L = std::max(1, std::min(levels, chunks));
+  std::vector<double> cum(chunks + 1, 0.0);
+  for (int j = 0; j < chunks; ++j) {
+    const int level = j % L;
+    const double w = (L > 1) ? 1.0 + (double)skew * (L - 1 - 2 * level) / (L - 1) : 1.0;
+    cum[j + 1] = cum[j] + std::max(w, 0.05);
+  }
+  std::vector<int> b(chunks + 1);
+  for (int j = 0; j <= chunks; ++j) b[j] = (int)std::lround((double)ny * cum[j] / cum[chunks]);
+  b[0] = 0;
+  b[chunks] = ny;
+  for (int j = 1; j <= chunks; ++j) b[j] = std::max(b[j], b[j - 1] + 1);   // no empty item
+  for (int j = chunks - 1; j >= 0; --j) b[j] = std::min(b[j], b[j + 1] - 1);
+  return b;
+}
+
+// Launch order of the work items.  Workgroup b of a launch runs on XCD b % 8
+// and takes item x*q + min(x, rem) + (b >> 3) (fv_step_kernel), so XCD x
+// works through the run of items [x*q + min(x, rem), ...) in order, and within
+// a run the hardware starts items in that order, one per SIMD, then the next
+// round of SIMD slots.  Each run is sorted by height, tallest first: the
+// waves that start first (and, as the oldest on their SIMD, issue first) get
+// the most rows, so the SIMDs' waves finish together instead of leaving one
+// wave alone at the end; on launches of several rounds the short items come
+// last.  Stable, so equal heights keep the strip order (neighbouring items,
+// which share halo rows, stay on the same XCD).
+void order_for_launch(std::vector<fv::WorkItem>& items) {
+  const int n = (int)items.size();
+  const int q = n >> 3, rem = n & 7;
+  for (int x = 0; x < 8; ++x) {
+    const int b = x * q + std::min(x, rem), e = b + q + (x < rem ? 1 : 0);
+    std::stable_sort(items.begin() + b, items.begin() + e,
+                     [](const fv::WorkItem& u, const fv::WorkItem& v) {
+                       return (u.y1 - u.y0) > (v.y1 - v.y0);
+                     });
+  }
+}
+
+constexpr float kAutoSkew = 0.5f;
+
+int item_chunks(const fivonagi_params& p, int auto_h) {
+  const int H = p.rows_per_wave > 0 ? p.rows_per_wave : auto_h;
+  return std::max(1, (int)std::lround((double)p.ny / H));
+}
+
+void build_items(fivonagi_ctx* c, int auto_h, float skew) {
   c->items.clear();
   c->rowmask.clear();
+  const long simds = 4 * c->cus;
+  const int per_simd = std::max(1, c->per_cu / 4);
+  // height levels = waves each SIMD starts in the launch's first round
+  long total = 0;
+  for (const auto& p : c->params)
+    total += (long)((p.nx + fv::kStripCols - 1) / fv::kStripCols) * item_chunks(p, auto_h);
+  const int levels = (int)std::max(1L, std::min<long>(per_simd, (total + simds - 1) / simds));
+  // automatic: skewed only when every SIMD starts its full complement of
+  // waves (there the oldest-first issue order staggers their ends; with
+  // fewer waves per SIMD they progress together)
+  if (skew < 0.0f) skew = (levels >= 4 && levels == per_simd) ? kAutoSkew : 0.0f;
+  c->skew_used = skew;
   for (int k = 0; k < c->n_inst; ++k) {
     const auto& p = c->params[k];
-    const int H = p.rows_per_wave > 0 ? p.rows_per_wave : auto_h;
-    const int chunks = std::max(1, (int)std::lround((double)p.ny / H));
+    const int chunks = item_chunks(p, auto_h);
     const int strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
+    const std::vector<int> bounds = chunk_bounds(p.ny, chunks, skew, levels);
     int n = 0;
     for (int s = 0; s < strips; ++s) {
       for (int ch = 0; ch < chunks; ++ch) {
         fv::WorkItem w;
         w.inst = k;
         w.x0 = s * fv::kStripCols;
-        w.y0 = (int)((long)p.ny * ch / chunks);
-        w.y1 = (int)((long)p.ny * (ch + 1) / chunks);
+        w.y0 = bounds[ch];
+        w.y1 = bounds[ch + 1];
         w.mrow = (int)c->rowmask.size();
-        w.pad[0] = w.pad[1] = w.pad[2] = 0;
+        w.local = n;
+        w.pad[0] = w.pad[1] = 0;
         for (int j = w.y0; j < w.y1; ++j) {
           uint64_t m = 0;
           for (int l = fv::kHalo; l < fv::kLanes - fv::kHalo; ++l) {
@@ -193,8 +266,8 @@ void build_items(fivonagi_ctx* c, int auto_h) {
       }
     }
     c->desc[k].n_items = n;
-    c->desc[k].first_item = (int)c->items.size() - n;
   }
+  order_for_launch(c->items);
 }
 
 // (re)upload the work list after build_items; the captured graph holds the
@@ -277,7 +350,7 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
     if (e != hipSuccess) return e;
   }
   return fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(), phase,
-                        c->stream);
+                        c->n_inst, c->d_ctl, c->d_sync, c->stream);
 }
 
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
@@ -448,7 +521,8 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * 2 * n_inst, hipMemcpyHostToDevice);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
 
-  build_items(c, auto_rows(c->params, resident_waves(c->device)));
+  device_shape(c->device, &c->cus, &c->per_cu);
+  build_items(c, auto_rows(c->params, c->cus * c->per_cu), c->skew);
   c->d_desc = (fv::InstDesc*)c->dalloc(sizeof(fv::InstDesc) * n_inst, &e);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc desc"); }
   int rc = upload_items(c);
@@ -513,36 +587,48 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
       (void)hipMemcpyAsync(keep[k] + (size_t)f * n, c->desc[k].q[saved[k].cur][f],
                            sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
   }
-  const int auto_h = auto_rows(c->params, resident_waves(c->device));
+  const int auto_h = auto_rows(c->params, c->cus * c->per_cu);
   std::vector<int> cands = {auto_h, 8, 12, 16, 20, 24, 32, 40, 48, 64, 96, 128};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   float best_ms = 1e30f;
   int best_h = auto_h;
+  float best_skew = c->skew;
   int rc = 0;
   std::vector<fivonagi_params> orig = c->params;
-  for (int h : cands) {
+  // one candidate split: trial_steps attempts after 2 warm-up attempts
+  auto trial = [&](int h, float skew) -> int {
     for (auto& p : c->params) p.rows_per_wave = h;
-    build_items(c, h);
-    if ((rc = upload_items(c))) break;
+    build_items(c, h, skew);
+    if (int r = upload_items(c)) return r;
     std::vector<int64_t> tgt(c->n_inst, INT64_MAX);
-    if ((rc = write_targets(c, tgt))) break;
-    if ((rc = launch_attempts(c, 2))) break;   // warm-up
+    if (int r = write_targets(c, tgt)) return r;
+    if (int r = launch_attempts(c, 2)) return r;
     (void)hipEventRecord(e0, c->stream);
-    if ((rc = launch_attempts(c, trial_steps))) break;
+    if (int r = launch_attempts(c, trial_steps)) return r;
     (void)hipEventRecord(e1, c->stream);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms < best_ms) { best_ms = ms; best_h = h; }
+    if (ms < best_ms) { best_ms = ms; best_h = h; best_skew = skew; }
+    return 0;
+  };
+  // heights with the automatic skew, then the skew at the best height
+  for (int h : cands)
+    if ((rc = trial(h, c->skew))) break;
+  if (!rc) {
+    const int h = best_h;
+    for (float sk : {0.0f, 0.25f, 0.5f, 0.75f})
+      if ((rc = trial(h, sk))) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   // restore: chosen work split, the saved state in the saved buffer, counters
   c->params = orig;
   for (auto& p : c->params) p.rows_per_wave = best_h;
-  build_items(c, best_h);
+  c->skew = best_skew;
+  build_items(c, best_h, c->skew);
   int rc2 = upload_items(c);
   for (int k = 0; k < c->n_inst; ++k) {
     const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
@@ -560,6 +646,15 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
   if (rc3) return rc3;
   if (rows_out) *rows_out = best_h;
   return 0;
+}
+
+int fivonagi_set_skew(fivonagi_ctx* c, double skew) {
+  if (!c) return fail(FIVONAGI_ERR_ARG, "null context");
+  if (!(skew < 0.0 || (skew >= 0.0 && skew <= 0.95)))
+    return fail(FIVONAGI_ERR_ARG, "skew must be < 0 (automatic) or in [0, 0.95]");
+  c->skew = skew < 0.0 ? -1.0f : (float)skew;
+  build_items(c, auto_rows(c->params, c->cus * c->per_cu), c->skew);
+  return upload_items(c);
 }
 
 int fivonagi_enqueue(fivonagi_ctx* c, int64_t n) {
@@ -643,10 +738,12 @@ int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
   info->error = t.err;
   info->cur = t.cur;
   info->n_items = c->desc[inst].n_items;
-  int h = 0;
-  for (const auto& w : c->items)
-    if (w.inst == inst) { h = w.y1 - w.y0; break; }
-  info->rows_per_wave = h;
+  info->skew = c->skew_used;
+  // mean rows per work item (heights differ when the split is skewed)
+  const auto& p = c->params[inst];
+  const int strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
+  const int chunks = std::max(1, c->desc[inst].n_items / strips);
+  info->rows_per_wave = (int)std::lround((double)p.ny / chunks);
   return 0;
 }
 

@@ -110,24 +110,76 @@ __device__ __forceinline__ uint32_t axchg_relaxed(uint32_t* p, uint32_t v) {
   return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Control block of attempt a+1 from the one attempt a ran with and attempt
-// a's reduction slots: common/main.cu:199-230 with a clean retry on
-// rejection.  log: record dt and T of an accepted step (adjoint runs); every
-// writer records the same values.
-__device__ Ctl advance(const InstDesc& D, const Ctl& p, const SyncSlots* __restrict__ sl,
-                       bool log) {
+// What attempt a's reduction slots say about it, with the control block it
+// ran with.  fold_load issues every load before anything is decided, so the
+// fold costs one memory round trip.
+struct FoldRaw {
+  Ctl p;          // control block attempt a ran with
+  uint32_t ran, errf, gbits;
+};
+struct Fold {
+  Ctl p;
+  float cfl;      // max cfl over the counted cells of attempt a
+  int state;      // kFoldIdle, kFoldError, kFoldAccept, kFoldReject
+};
+constexpr int kFoldIdle = 0, kFoldError = 1, kFoldAccept = 2, kFoldReject = 3;
+
+__device__ __forceinline__ FoldRaw fold_load(const Ctl* __restrict__ c,
+                                             const SyncSlots* __restrict__ sl) {
+  FoldRaw r;
+  Ctl& p = r.p;
+  p.dt = c->dt; p.T = c->T; p.last_cfl = c->last_cfl;
+  p.drive_s = c->drive_s; p.drive_c = c->drive_c;
+  p.n_acc = c->n_acc; p.n_rej = c->n_rej; p.n_att = c->n_att;
+  p.target_acc = c->target_acc;
+  p.cur = c->cur; p.err = c->err;
+  p.dtx = c->dtx; p.dty = c->dty; p.ddx2 = c->ddx2; p.ddy2 = c->ddy2;
+  p.pad0 = 0; p.pad1 = 0;
+  r.ran = sl->ran;
+  r.errf = sl->err_flag;
+  // the group maxima with one vector load (lane l reads group l % kGroups)
+  // rather than kGroups scalar loads
+  r.gbits = sl->grp[__lane_id() & (kGroups - 1)].cfl_bits;
+  return r;
+}
+
+__device__ __forceinline__ Fold fold_decide(const FoldRaw& r, double cfl_max, bool presc) {
+  Fold f;
+  f.p = r.p;
+  f.p.n_acc = uni64(r.p.n_acc); f.p.n_rej = uni64(r.p.n_rej); f.p.n_att = uni64(r.p.n_att);
+  f.p.target_acc = uni64(r.p.target_acc);
+  f.p.cur = uni(r.p.cur); f.p.err = uni(r.p.err);
+  // non-negative floats order like their bit patterns
+  uint32_t mbits = 0u;
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) mbits = max(mbits, (uint32_t)__builtin_amdgcn_readlane(r.gbits, k));
+  const float cfl32 = __uint_as_float(mbits);
+  f.cfl = cfl32;
+  const bool acc = presc || ((double)cfl32 < cfl_max);
+  f.state = !uni(r.ran) ? kFoldIdle
+                        : (uni(r.errf) ? kFoldError : (acc ? kFoldAccept : kFoldReject));
+  return f;
+}
+
+// Ping-pong index attempt a+1 reads (known before the rest of advance()).
+__device__ __forceinline__ int next_cur(const Fold& f) {
+  return f.state == kFoldAccept ? (f.p.cur ^ 1) : f.p.cur;
+}
+
+// Control block of attempt a+1 from attempt a's fold: common/main.cu:199-230
+// with a clean retry on rejection.  log: record dt and T of an accepted step
+// (adjoint runs); every writer records the same values.
+__device__ Ctl advance(const InstDesc& D, const Fold& f, bool log) {
+  const Ctl& p = f.p;
   Ctl o = p;
-  if (!uni(sl->ran)) return o;   // attempt a did not run (instance done or stopped)
-  if (uni(sl->err_flag)) {
-    o.err = 2;                   // imaginary eigenvalue: step discarded, instance stops
+  if (f.state == kFoldIdle) return o;   // attempt a did not run (instance done or stopped)
+  if (f.state == kFoldError) {
+    o.err = 2;                         // imaginary eigenvalue: step discarded, instance stops
     return o;
   }
-  float cfl32 = 0.0f;
-#pragma unroll
-  for (int k = 0; k < kGroups; ++k) cfl32 = fmaxf(cfl32, unif(__uint_as_float(sl->grp[k].cfl_bits)));
-  const double cfl = (double)cfl32;
+  const double cfl = (double)f.cfl;
   const bool presc = D.dt_seq != nullptr;
-  const bool acc = presc || (cfl < D.cfl_max);
+  const bool acc = f.state == kFoldAccept;
   const int64_t att = p.n_att + 1;
   o.n_att = att;
   o.last_cfl = cfl;
@@ -172,16 +224,10 @@ __device__ Ctl advance(const InstDesc& D, const Ctl& p, const SyncSlots* __restr
 
 // Control block of the attempt a launch runs (phase = a mod 6), read-only.
 __device__ __forceinline__ Ctl current(const InstDesc& D, int phase, bool log) {
-  const Ctl* c = D.ctl + (phase & 1);
-  Ctl p;
-  p.dt = c->dt; p.T = c->T; p.last_cfl = c->last_cfl;
-  p.drive_s = c->drive_s; p.drive_c = c->drive_c;
-  p.n_acc = uni64(c->n_acc); p.n_rej = uni64(c->n_rej); p.n_att = uni64(c->n_att);
-  p.target_acc = uni64(c->target_acc);
-  p.cur = uni(c->cur); p.err = uni(c->err);
-  p.dtx = c->dtx; p.dty = c->dty; p.ddx2 = c->ddx2; p.ddy2 = c->ddy2;
-  p.pad0 = 0; p.pad1 = 0;
-  return advance(D, p, D.sync + (phase + 2) % 3, log);
+  return advance(D,
+                 fold_decide(fold_load(D.ctl + (phase & 1), D.sync + (phase + 2) % 3),
+                             D.cfl_max, D.dt_seq != nullptr),
+                 log);
 }
 
 struct Row {         // one cell of a row as the y sweep sees it
@@ -199,10 +245,18 @@ struct XY {          // one cell of q_xy, with its velocities for the source
   float q1, q2, q3, u, v;
 };
 
+// kOne: the launch covers a single instance, whose control block and
+// reduction slots of this attempt are passed directly (ctl_in = its control
+// block of attempt a = ctl + a % 2, sync_in = the slots of attempt a-1 =
+// sync + (a + 2) % 3), so the fold's loads go out together with the work
+// item's instead of after the instance description's.
+template <bool kOne>
 __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
                                                      const WorkItem* __restrict__ items,
                                                      const uint64_t* __restrict__ rowmask,
-                                                     int n_items, int phase) {
+                                                     int n_items, int phase,
+                                                     const Ctl* __restrict__ ctl_in,
+                                                     const SyncSlots* __restrict__ sync_in) {
   // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
   // run of work items so vertically adjacent chunks of a strip (which share
   // halo rows) hit the same L2.
@@ -213,41 +267,45 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     item = x * q + min(x, rem) + k;
   }
   const WorkItem it = items[item];
-  const InstDesc& D = insts[it.inst];
-  const int local = item - D.first_item;
+  const InstDesc& D = insts[kOne ? 0 : it.inst];
+  const int local = it.local;
   const int lane = threadIdx.x;
 
-  // ---- control block of this attempt: the previous attempt folded in ----
-  const Ctl P = current(D, phase, local == 0);
-  const bool runs = P.err == 0 && P.n_acc < P.target_acc;
-  if (local == 0 && lane == 0) {
-    // one wave per instance: publish the control block for the next launch,
-    // clear the slots the next attempt reduces into, mark this attempt
-    D.ctl[(phase + 1) & 1] = P;
-    SyncSlots* nx_sl = D.sync + (phase + 1) % 3;
-#pragma unroll
-    for (int k = 0; k < kGroups; ++k) nx_sl->grp[k].cfl_bits = 0u;
-    nx_sl->err_flag = 0u;
-    nx_sl->ran = 0u;
-    if (runs) D.sync[phase % 3].ran = 1u;
-  }
-  if (!runs) return;
-  const int cur = P.cur;
-  const float dtx = unif(P.dtx), dty = unif(P.dty);
-  const float ddx2 = unif(P.ddx2), ddy2 = unif(P.ddy2);
-  const float sin_t = unif((float)P.drive_s), cos_t = unif((float)P.drive_c);
-  const float hbeta = 0.5f * D.beta2m2;     // BETA - 1
+  // ---- the previous attempt's reduction, folded: accept/reject decides
+  // which buffer this attempt reads, so its first row can be requested
+  // before the rest of the control block (dt, T, drive phase) is formed ----
+  // the instance's geometry and both buffers' pointers are requested with
+  // the fold's loads; the asm barrier keeps the compiler from re-loading a
+  // pointer by the buffer index once accept/reject is known
   const int nx = D.nx, ny = D.ny;
   const int bc = D.bc;
+  float* qb[2][3];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) qb[h][f] = D.q[h][f];
+  double cfl_max = D.cfl_max;
+  const double* dt_seq = D.dt_seq;
+  const float* ec_p = D.edge_c;
+  const float* es_p = D.edge_s;
+  const FoldRaw FR = fold_load(kOne ? ctl_in : D.ctl + (phase & 1),
+                               kOne ? sync_in : D.sync + (phase + 2) % 3);
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) asm volatile("" : "+s"(qb[h][f]));
+  asm volatile("" : "+s"(cfl_max), "+s"(dt_seq), "+s"(ec_p), "+s"(es_p));
+  const Fold F = fold_decide(FR, cfl_max, dt_seq != nullptr);
+  const int cur = next_cur(F);
 
-  cgfloat* __restrict__ in1 = (cgfloat*)D.q[cur][0];
-  cgfloat* __restrict__ in2 = (cgfloat*)D.q[cur][1];
-  cgfloat* __restrict__ in3 = (cgfloat*)D.q[cur][2];
-  gfloat* __restrict__ o1 = (gfloat*)D.q[cur ^ 1][0];
-  gfloat* __restrict__ o2 = (gfloat*)D.q[cur ^ 1][1];
-  gfloat* __restrict__ o3 = (gfloat*)D.q[cur ^ 1][2];
-  kfloat* __restrict__ edge_c = (kfloat*)D.edge_c;
-  kfloat* __restrict__ edge_s = (kfloat*)D.edge_s;
+  cgfloat* __restrict__ in1 = (cgfloat*)(cur ? qb[1][0] : qb[0][0]);
+  cgfloat* __restrict__ in2 = (cgfloat*)(cur ? qb[1][1] : qb[0][1]);
+  cgfloat* __restrict__ in3 = (cgfloat*)(cur ? qb[1][2] : qb[0][2]);
+  gfloat* __restrict__ o1 = (gfloat*)(cur ? qb[0][0] : qb[1][0]);
+  gfloat* __restrict__ o2 = (gfloat*)(cur ? qb[0][1] : qb[1][1]);
+  gfloat* __restrict__ o3 = (gfloat*)(cur ? qb[0][2] : qb[1][2]);
+  kfloat* __restrict__ edge_c = (kfloat*)ec_p;
+  kfloat* __restrict__ edge_s = (kfloat*)es_p;
 
   const int i = it.x0 - kHalo + lane;
   const bool col_in = (i >= 0) && (i < nx);
@@ -290,6 +348,28 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
       es = edge_s[rc];
     }
   };
+  float n1, n2, n3, nec = 0.0f, nes = 0.0f;
+  load_raw(it.y0 - kHalo, n1, n2, n3, nec, nes);   // in flight during advance()
+
+  // ---- control block of this attempt ----
+  const Ctl P = advance(D, F, local == 0);
+  const bool runs = P.err == 0 && P.n_acc < P.target_acc;
+  if (local == 0 && lane == 0) {
+    // one wave per instance: publish the control block for the next launch,
+    // clear the slots the next attempt reduces into, mark this attempt
+    D.ctl[(phase + 1) & 1] = P;
+    SyncSlots* nx_sl = D.sync + (phase + 1) % 3;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) nx_sl->grp[k].cfl_bits = 0u;
+    nx_sl->err_flag = 0u;
+    nx_sl->ran = 0u;
+    if (runs) D.sync[phase % 3].ran = 1u;
+  }
+  if (!runs) return;
+  const float dtx = unif(P.dtx), dty = unif(P.dty);
+  const float ddx2 = unif(P.ddx2), ddy2 = unif(P.ddy2);
+  const float sin_t = unif((float)P.drive_s), cos_t = unif((float)P.drive_c);
+  const float hbeta = 0.5f * D.beta2m2;     // BETA - 1
   auto patch = [&](Row& q, float ec, float es) {
     if (bc == kBcEdgeDrive) {
       float a, b, c;
@@ -337,7 +417,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // prefetch, store row r-4, issue the next prefetch -- so each store has a
   // whole step to drain before the next step waits for its prefetch (stores
   // and loads share one in-order counter).
-  float n1, n2, n3, nec = 0.0f, nes = 0.0f;
   auto body = [&](int r, Row& bn, Row& bm, Row& bo, YI& fn, YI& fc, YI& fl, XY& xn, XY& xm,
                   XY& xo) {
     // every load of the prefetch (and every store before it) is done: an
@@ -419,7 +498,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   Row B0{}, B1{}, B2{};
   YI F0{}, F1{}, F2{};
   XY X0{}, X1{}, X2{};
-  load_raw(y0 - kHalo, n1, n2, n3, nec, nes);
   for (int r = y0 - kHalo; r <= y1 + 2; r += 3) {
     body(r, B0, B2, B1, F0, F2, F1, X0, X2, X1);
     if (r + 1 > y1 + 2) break;
@@ -511,14 +589,25 @@ extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, in
 
 // Resident step waves per CU (occupancy of the step kernel on this device).
 extern "C" hipError_t fv_step_waves_per_cu(int* waves) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(waves, fv::fv_step_kernel, fv::kLanes, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(waves, fv::fv_step_kernel<false>, fv::kLanes,
+                                                      0);
 }
 
+// One step attempt of every instance.  ctl0/sync0: the control blocks and
+// slot sets of the only instance when n_inst == 1 (null otherwise).
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0,
                                      hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
-  hipLaunchKernelGGL(fv::fv_step_kernel, dim3(n_items), dim3(fv::kLanes), 0, stream, insts,
-                     items, rowmask, n_items, phase);
+  if (n_inst == 1 && ctl0 && sync0)
+    hipLaunchKernelGGL(fv::fv_step_kernel<true>, dim3(n_items), dim3(fv::kLanes), 0, stream,
+                       insts, items, rowmask, n_items, phase,
+                       (const fv::Ctl*)(ctl0 + (phase & 1)),
+                       (const fv::SyncSlots*)(sync0 + (phase + 2) % 3));
+  else
+    hipLaunchKernelGGL(fv::fv_step_kernel<false>, dim3(n_items), dim3(fv::kLanes), 0, stream,
+                       insts, items, rowmask, n_items, phase, (const fv::Ctl*)nullptr,
+                       (const fv::SyncSlots*)nullptr);
   return hipGetLastError();
 }

@@ -73,7 +73,7 @@ struct InstDesc {
   int32_t bc, cfl_mask;  // cfl_mask: host-side record of the fold subset kind
   SyncSlots* sync;      // 3 reduction slot sets (attempt index mod 3)
   int32_t n_items;      // work items (waves) of this instance per step
-  int32_t first_item;   // index of the instance's first work item
+  int32_t pad2;
   int32_t tay_jc;
   int32_t pad1;
   float beta2m2;        // 2*(BETA - 1)
@@ -83,13 +83,16 @@ struct InstDesc {
 // One wave's work: a 58-column strip of one instance, rows [y0, y1).
 // mrow: index of the item's first row in the CFL lane-mask table; entry
 // mrow + (j - y0) has bit l set when lane l's cell (x0 - 3 + l, j) is an
-// output cell with i >= 1 that the CFL reduction counts.
+// output cell with i >= 1 that the CFL reduction counts.  local: the item's
+// index among its instance's items (0 publishes the instance's control block;
+// local % kGroups picks the reduction slot), independent of the launch order.
 struct WorkItem {
   int32_t inst;
   int32_t x0;           // first output column of the strip
   int32_t y0, y1;
   int32_t mrow;
-  int32_t pad[3];
+  int32_t local;
+  int32_t pad[2];
 };
 
 }  // namespace fv

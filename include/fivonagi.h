@@ -67,7 +67,8 @@ typedef struct {
   double T, dt, last_cfl;
   int64_t n_accepted, n_rejected, n_attempts;
   int32_t error, cur;
-  int32_t n_items, rows_per_wave;
+  int32_t n_items, rows_per_wave; /* work items per step, mean rows per item */
+  double skew;             /* height skew of the work items in use            */
 } fivonagi_info_t;
 
 /* gpu_init (common/data_definitions.h:523-871) + init() upload
@@ -93,10 +94,18 @@ int fivonagi_enqueue(fivonagi_ctx *ctx, int64_t nattempts);
 int fivonagi_sync(fivonagi_ctx *ctx);
 
 /* Autotuner (no counterpart in the reference): time trial_steps step
-   attempts for each candidate height of the work items (rows per wave),
-   keep the fastest, and restore the state and counters as they were.
-   rows_per_wave_out (may be NULL) receives the choice. */
+   attempts for each candidate height of the work items (rows per wave), then
+   for each height skew at the best height, keep the fastest, and restore the
+   state and counters as they were.  rows_per_wave_out (may be NULL) receives
+   the height; fivonagi_info reports the skew. */
 int fivonagi_autotune(fivonagi_ctx *ctx, int trial_steps, int32_t *rows_per_wave_out);
+
+/* Work split (no counterpart in the reference): skew of the work-item
+   heights.  0 = equal heights; s in (0, 0.95]: up to four height levels from
+   (1+s)H to (1-s)H, the tallest launched first, which evens out when the
+   waves sharing a SIMD finish; < 0 = automatic (0.5 when every SIMD starts
+   its full complement of step waves, else 0).  Results do not depend on it. */
+int fivonagi_set_skew(fivonagi_ctx *ctx, double skew);
 
 /* dataCollect/dataExport stand-in (apps/hifu-beam/data_export.h:29-75):
    copy instance inst's state and counters to the host.  Any pointer may be NULL. */

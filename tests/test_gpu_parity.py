@@ -122,6 +122,33 @@ def test_decomposition_invariance():
             assert np.array_equal(q, base), f"rows_per_wave={h} changed the result"
 
 
+def test_skew_invariance():
+    """Results must not depend on the height skew of the work items, on the
+    launch order it brings, or on a change of split between steps.  1200x600
+    at 4 rows per item is 3150 items: four height levels."""
+    probs = [wl.gauss_pulse(1200, 600, sigma_cells=9.0, rows_per_wave=4),
+             wl.edge_drive(300, 90, halfwidth_cells=20.0, rows_per_wave=3)]
+    base = None
+    for sk in (0.0, 0.5, 0.95, -1.0):
+        with fa.Context(probs) as ctx:
+            ctx.set_skew(sk)
+            assert ctx.info(0)["skew"] == pytest.approx(0.5 if sk < 0 else sk)
+            ctx.step(12)
+            ctx.set_skew(0.3)
+            ctx.step(18)
+            out = [ctx.collect(k) for k in range(2)]
+        if base is None:
+            base = out
+            continue
+        for (qa, sa), (qb, sb) in zip(base, out):
+            assert np.array_equal(qa, qb), f"skew {sk} changed the result"
+            assert sa == sb
+    with fa.Context(probs[1]) as ctx:
+        assert ctx.info(0)["skew"] == 0.0     # one item per SIMD: no skew
+        with pytest.raises(fa.FivonagiError):
+            ctx.set_skew(1.5)
+
+
 def test_determinism():
     prob = wl.edge_drive(300, 90, halfwidth_cells=20.0)
     a, _ = run_gpu(prob, 40)
