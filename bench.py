@@ -138,7 +138,8 @@ def bench_batched(args, hip):
     stream = hip.stream()
     ctx = fa.Context(probs)
     ctx.set_stream(stream.value)
-    rows = ctx.autotune(10)
+    rows = ctx.autotune(30)
+    split = ctx.info(0)
     ctx.enqueue(args.warmup)
     if ctx.sync() != 0:
         raise RuntimeError("warmup failed")
@@ -163,7 +164,8 @@ def bench_batched(args, hip):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"batched forward: 64 instances, {total} cells, per-instance dt",
-                   "rows_per_wave": rows, "accepted_cell_updates": cells},
+                   "rows_per_wave": rows, "skew": round(split["skew"], 3),
+                   "rotate": split["rotate"], "accepted_cell_updates": cells},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": KERNEL},
@@ -327,7 +329,8 @@ def main():
             "workload": f"edge_drive {prob.nx}x{prob.ny} fp32 forward (A=0.02, aperture "
                         f"half-width 200 cells, 20 cells/wavelength, beta=3.5, delta=5e-5, CFL 0.9)",
             "accepted_steps": accepted, "attempts": attempts,
-            "rows_per_wave": after["rows_per_wave"], "waves_per_step": after["n_items"],
+            "rows_per_wave": after["rows_per_wave"], "skew": round(after["skew"], 3),
+            "rotate": after["rotate"], "waves_per_step": after["n_items"],
             "T_start": before["T"], "T_end": after["T"],
         },
         "hbm_peak_frac": round(achieved / HBM_PEAK_GBS, 4),

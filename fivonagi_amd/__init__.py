@@ -48,7 +48,7 @@ class Info(C.Structure):
         ("n_accepted", C.c_int64), ("n_rejected", C.c_int64), ("n_attempts", C.c_int64),
         ("error", C.c_int32), ("cur", C.c_int32),
         ("n_items", C.c_int32), ("rows_per_wave", C.c_int32),
-        ("skew", C.c_double),
+        ("skew", C.c_double), ("rotate", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -93,7 +93,7 @@ def load():
                                           C.POINTER(AdjointOpts), dp, dp, dp]
     lib.fivonagi_adjoint_state.argtypes = [vp, C.c_int, fp, fp, fp]
     lib.fivonagi_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int32)]
-    lib.fivonagi_set_skew.argtypes = [vp, C.c_double]
+    lib.fivonagi_set_skew.argtypes = [vp, C.c_double, C.c_int]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
     lib.fivonagi_last_error.argtypes = []
@@ -267,9 +267,10 @@ class Context:
             self._h, inst, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in ls]))
         return np.stack(ls)
 
-    def set_skew(self, skew):
-        """Height skew of the work items (< 0: automatic); results do not depend on it."""
-        _check(load().fivonagi_set_skew(self._h, C.c_double(skew)))
+    def set_skew(self, skew, rotate=False):
+        """Height skew of the work items (< 0: automatic) and the per-strip
+        rotation of its levels; results do not depend on either."""
+        _check(load().fivonagi_set_skew(self._h, C.c_double(skew), int(bool(rotate))))
 
     def autotune(self, trial_steps=20):
         """Pick the rows per wave with the fastest step; state is unchanged."""

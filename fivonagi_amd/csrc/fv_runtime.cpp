@@ -91,6 +91,8 @@ struct fivonagi_ctx {
   std::vector<uint64_t> rowmask;           // CFL lane mask per item row (fv::WorkItem::mrow)
   float skew = -1.0f;                      // height skew of the work items (< 0: automatic)
   float skew_used = 0.0f;                  // the skew of the current work list
+  bool rotate = false;                     // level pattern rotated per strip (chunk_bounds)
+  bool rotate_used = false;
   long cus = 256;                          // CUs of the device
   int per_cu = 16;                         // resident step waves per CU
   std::vector<std::vector<uint32_t>> fold_bits;  // kCflBitmap: bit i + j*nx = cell counted
@@ -173,14 +175,16 @@ bool cfl_counted(const fivonagi_ctx* c, int k, int i, int j) {
 }
 
 // Row boundaries of one strip cut into `chunks` work items.  With skew > 0
-// the items get `levels` height levels (item j: level j % levels) with
-// weights falling linearly from 1 + skew to 1 - skew (mean 1), so the strip
-// keeps its number of items, some taller and some shorter.
-std::vector<int> chunk_bounds(int ny, int chunks, float skew, int levels) {
+// the items get `levels` height levels (item j: level (j + rot) % levels)
+// with weights falling linearly from 1 + skew to 1 - skew (mean 1), so the
+// strip keeps its number of items, some taller and some shorter.  rot (the
+// strip index) spreads the levels a strip has one more item of evenly over
+// the strips, so every level has the same number of items.
+std::vector<int> chunk_bounds(int ny, int chunks, float skew, int levels, int rot) {
   const int L = std::max(1, std::min(levels, chunks));
   std::vector<double> cum(chunks + 1, 0.0);
   for (int j = 0; j < chunks; ++j) {
-    const int level = j % L;
+    const int level = (j + rot) % L;
     const double w = (L > 1) ? 1.0 + (double)skew * (L - 1 - 2 * level) / (L - 1) : 1.0;
     cum[j + 1] = cum[j] + std::max(w, 0.05);
   }
@@ -222,7 +226,7 @@ int item_chunks(const fivonagi_params& p, int auto_h) {
   return std::max(1, (int)std::lround((double)p.ny / H));
 }
 
-void build_items(fivonagi_ctx* c, int auto_h, float skew) {
+void build_items(fivonagi_ctx* c, int auto_h, float skew, bool rotate) {
   c->items.clear();
   c->rowmask.clear();
   const long simds = 4 * c->cus;
@@ -237,13 +241,19 @@ void build_items(fivonagi_ctx* c, int auto_h, float skew) {
   // fewer waves per SIMD they progress together)
   if (skew < 0.0f) skew = (levels >= 4 && levels == per_simd) ? kAutoSkew : 0.0f;
   c->skew_used = skew;
+  c->rotate_used = rotate && skew > 0.0f;
   for (int k = 0; k < c->n_inst; ++k) {
     const auto& p = c->params[k];
     const int chunks = item_chunks(p, auto_h);
     const int strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
-    const std::vector<int> bounds = chunk_bounds(p.ny, chunks, skew, levels);
+    // rotate: the level pattern shifts by one per strip (every level gets the
+    // same number of items; strips no longer share their cut rows)
+    std::vector<std::vector<int>> bounds_rot;
+    for (int r = 0; r < (rotate ? levels : 1); ++r)
+      bounds_rot.push_back(chunk_bounds(p.ny, chunks, skew, levels, r));
     int n = 0;
     for (int s = 0; s < strips; ++s) {
+      const std::vector<int>& bounds = bounds_rot[rotate ? s % levels : 0];
       for (int ch = 0; ch < chunks; ++ch) {
         fv::WorkItem w;
         w.inst = k;
@@ -522,7 +532,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
 
   device_shape(c->device, &c->cus, &c->per_cu);
-  build_items(c, auto_rows(c->params, c->cus * c->per_cu), c->skew);
+  build_items(c, auto_rows(c->params, c->cus * c->per_cu), c->skew, c->rotate);
   c->d_desc = (fv::InstDesc*)c->dalloc(sizeof(fv::InstDesc) * n_inst, &e);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc desc"); }
   int rc = upload_items(c);
@@ -595,12 +605,13 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
   float best_ms = 1e30f;
   int best_h = auto_h;
   float best_skew = c->skew;
+  bool best_rot = c->rotate;
   int rc = 0;
   std::vector<fivonagi_params> orig = c->params;
   // one candidate split: trial_steps attempts after 2 warm-up attempts
-  auto trial = [&](int h, float skew) -> int {
+  auto trial = [&](int h, float skew, bool rot) -> int {
     for (auto& p : c->params) p.rows_per_wave = h;
-    build_items(c, h, skew);
+    build_items(c, h, skew, rot);
     if (int r = upload_items(c)) return r;
     std::vector<int64_t> tgt(c->n_inst, INT64_MAX);
     if (int r = write_targets(c, tgt)) return r;
@@ -611,16 +622,20 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms < best_ms) { best_ms = ms; best_h = h; best_skew = skew; }
+    if (ms < best_ms) { best_ms = ms; best_h = h; best_skew = skew; best_rot = rot; }
     return 0;
   };
-  // heights with the automatic skew, then the skew at the best height
+  // heights with the automatic skew, then skews and level layouts at the
+  // best height
   for (int h : cands)
-    if ((rc = trial(h, c->skew))) break;
+    if ((rc = trial(h, -1.0f, false))) break;
   if (!rc) {
     const int h = best_h;
     for (float sk : {0.0f, 0.25f, 0.5f, 0.75f})
-      if ((rc = trial(h, sk))) break;
+      for (bool rot : {false, true}) {
+        if (sk == 0.0f && rot) continue;
+        if (!rc) rc = trial(h, sk, rot);
+      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -628,7 +643,8 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
   c->params = orig;
   for (auto& p : c->params) p.rows_per_wave = best_h;
   c->skew = best_skew;
-  build_items(c, best_h, c->skew);
+  c->rotate = best_rot;
+  build_items(c, best_h, c->skew, c->rotate);
   int rc2 = upload_items(c);
   for (int k = 0; k < c->n_inst; ++k) {
     const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
@@ -648,12 +664,13 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
   return 0;
 }
 
-int fivonagi_set_skew(fivonagi_ctx* c, double skew) {
+int fivonagi_set_skew(fivonagi_ctx* c, double skew, int rotate) {
   if (!c) return fail(FIVONAGI_ERR_ARG, "null context");
   if (!(skew < 0.0 || (skew >= 0.0 && skew <= 0.95)))
     return fail(FIVONAGI_ERR_ARG, "skew must be < 0 (automatic) or in [0, 0.95]");
   c->skew = skew < 0.0 ? -1.0f : (float)skew;
-  build_items(c, auto_rows(c->params, c->cus * c->per_cu), c->skew);
+  c->rotate = rotate != 0;
+  build_items(c, auto_rows(c->params, c->cus * c->per_cu), c->skew, c->rotate);
   return upload_items(c);
 }
 
@@ -739,6 +756,7 @@ int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
   info->cur = t.cur;
   info->n_items = c->desc[inst].n_items;
   info->skew = c->skew_used;
+  info->rotate = c->rotate_used ? 1 : 0;
   // mean rows per work item (heights differ when the split is skewed)
   const auto& p = c->params[inst];
   const int strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;

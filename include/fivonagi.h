@@ -69,6 +69,7 @@ typedef struct {
   int32_t error, cur;
   int32_t n_items, rows_per_wave; /* work items per step, mean rows per item */
   double skew;             /* height skew of the work items in use            */
+  int32_t rotate, pad;     /* 1: skew levels rotated per strip                */
 } fivonagi_info_t;
 
 /* gpu_init (common/data_definitions.h:523-871) + init() upload
@@ -101,11 +102,15 @@ int fivonagi_sync(fivonagi_ctx *ctx);
 int fivonagi_autotune(fivonagi_ctx *ctx, int trial_steps, int32_t *rows_per_wave_out);
 
 /* Work split (no counterpart in the reference): skew of the work-item
-   heights.  0 = equal heights; s in (0, 0.95]: up to four height levels from
-   (1+s)H to (1-s)H, the tallest launched first, which evens out when the
-   waves sharing a SIMD finish; < 0 = automatic (0.5 when every SIMD starts
-   its full complement of step waves, else 0).  Results do not depend on it. */
-int fivonagi_set_skew(fivonagi_ctx *ctx, double skew);
+   heights.  0 = equal heights; s in (0, 0.95]: as many height levels as
+   waves each SIMD starts (at most its resident count), from (1+s)H to
+   (1-s)H, the tallest launched first, which evens out when the waves sharing
+   a SIMD finish; < 0 = automatic (0.5 when every SIMD starts its full
+   complement of step waves, else 0).  rotate != 0 shifts the level pattern by
+   one per strip (same count per level; strips no longer share cut rows).
+   Which is fastest depends on the grid: fivonagi_autotune tries them.
+   Results do not depend on it. */
+int fivonagi_set_skew(fivonagi_ctx *ctx, double skew, int rotate);
 
 /* dataCollect/dataExport stand-in (apps/hifu-beam/data_export.h:29-75):
    copy instance inst's state and counters to the host.  Any pointer may be NULL. */

@@ -129,19 +129,21 @@ def test_skew_invariance():
     probs = [wl.gauss_pulse(1200, 600, sigma_cells=9.0, rows_per_wave=4),
              wl.edge_drive(300, 90, halfwidth_cells=20.0, rows_per_wave=3)]
     base = None
-    for sk in (0.0, 0.5, 0.95, -1.0):
+    for sk, rot in ((0.0, False), (0.5, False), (0.5, True), (0.95, True), (-1.0, False)):
         with fa.Context(probs) as ctx:
-            ctx.set_skew(sk)
-            assert ctx.info(0)["skew"] == pytest.approx(0.5 if sk < 0 else sk)
+            ctx.set_skew(sk, rot)
+            info = ctx.info(0)
+            assert info["skew"] == pytest.approx(0.5 if sk < 0 else sk)
+            assert info["rotate"] == int(rot and sk > 0)
             ctx.step(12)
-            ctx.set_skew(0.3)
+            ctx.set_skew(0.3, not rot)
             ctx.step(18)
             out = [ctx.collect(k) for k in range(2)]
         if base is None:
             base = out
             continue
         for (qa, sa), (qb, sb) in zip(base, out):
-            assert np.array_equal(qa, qb), f"skew {sk} changed the result"
+            assert np.array_equal(qa, qb), f"skew {sk} rotate {rot} changed the result"
             assert sa == sb
     with fa.Context(probs[1]) as ctx:
         assert ctx.info(0)["skew"] == 0.0     # one item per SIMD: no skew
