@@ -18,6 +18,7 @@ OK, ERR_EIGEN, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 8, 9, 10
 
 EXPORTS = (
     "fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence", "fivonagi_step",
+    "fivonagi_step_until",
     "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect", "fivonagi_info",
     "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_set_skew",
     "fivonagi_destroy", "fivonagi_last_error",
@@ -38,6 +39,8 @@ class Params(C.Structure):
         ("tay_rhoa", C.c_double), ("tay_c", C.c_double),
         ("tay_theta", C.c_double), ("tay_yoff", C.c_double),
         ("tay_jc", C.c_int32), ("rows_per_wave", C.c_int32),
+        ("md_on", C.c_int32),
+        ("md_vx", C.c_double), ("md_vy", C.c_double), ("md_t", C.c_double),
     ]
 
 
@@ -49,6 +52,7 @@ class Info(C.Structure):
         ("error", C.c_int32), ("cur", C.c_int32),
         ("n_items", C.c_int32), ("rows_per_wave", C.c_int32),
         ("skew", C.c_double), ("rotate", C.c_int32), ("pad", C.c_int32),
+        ("md_x", C.c_int32), ("md_y", C.c_int32),
     ]
 
 
@@ -82,6 +86,7 @@ def load():
     lib.fivonagi_set_stream.argtypes = [vp, vp]
     lib.fivonagi_set_dt_sequence.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_int64]
     lib.fivonagi_step.argtypes = [vp, C.c_int64]
+    lib.fivonagi_step_until.argtypes = [vp, C.c_double]
     lib.fivonagi_enqueue.argtypes = [vp, C.c_int64]
     lib.fivonagi_sync.argtypes = [vp]
     fp = C.POINTER(C.c_float)
@@ -99,7 +104,8 @@ def load():
     lib.fivonagi_last_error.argtypes = []
     lib.fivonagi_last_error.restype = C.c_char_p
     for name in ("fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
-                 "fivonagi_step", "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect",
+                 "fivonagi_step", "fivonagi_step_until", "fivonagi_enqueue", "fivonagi_sync",
+                 "fivonagi_collect",
                  "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state",
                  "fivonagi_autotune", "fivonagi_set_skew"):
         getattr(lib, name).restype = C.c_int
@@ -126,7 +132,8 @@ class Problem:
 
     def __init__(self, nx, ny, dx, dy, beta, delta, q0, cfl_max=1.0, cfl_wish=0.9,
                  dt0=0.005, T0=0.0, bc=BC_ZEROGRAD, cfl_mode=CFL_REFERENCE_FOLD,
-                 omega=0.0, edge_amp=None, edge_phase=None, taylor=None, rows_per_wave=0):
+                 omega=0.0, edge_amp=None, edge_phase=None, taylor=None, rows_per_wave=0,
+                 moving=None):
         self.nx, self.ny = int(nx), int(ny)
         self.dx, self.dy, self.beta, self.delta = float(dx), float(dy), float(beta), float(delta)
         self.cfl_max, self.cfl_wish, self.dt0, self.T0 = map(float, (cfl_max, cfl_wish, dt0, T0))
@@ -135,6 +142,8 @@ class Problem:
         self.edge_phase = None if edge_phase is None else np.ascontiguousarray(edge_phase, np.float64)
         self.taylor = taylor
         self.rows_per_wave = int(rows_per_wave)
+        # moving window: (md_vx, md_vy, md_t) or None (fixed domain)
+        self.moving = None if moving is None else tuple(float(v) for v in moving)
         self.q0 = np.ascontiguousarray(q0, np.float32)
         assert self.q0.shape == (3, self.ny, self.nx), self.q0.shape
 
@@ -152,6 +161,9 @@ class Problem:
             p.tay_rhoa, p.tay_c, p.tay_theta = t["rhoa"], t["c"], t["theta"]
             p.tay_yoff, p.tay_jc = t["yoff"], int(t["jc"])
         p.rows_per_wave = self.rows_per_wave
+        if self.moving is not None:
+            p.md_on = 1
+            p.md_vx, p.md_vy, p.md_t = self.moving
         return p
 
 
@@ -207,6 +219,10 @@ class Context:
 
     def step(self, nsteps):
         return _check(load().fivonagi_step(self._h, int(nsteps)))
+
+    def step_until(self, T_stop):
+        """Attempts until every instance has T >= T_stop (the frame loop)."""
+        return _check(load().fivonagi_step_until(self._h, C.c_double(T_stop)))
 
     def step_rc(self, nsteps):
         """fivonagi_step returning the raw code instead of raising."""

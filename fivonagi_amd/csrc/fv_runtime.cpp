@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -22,7 +23,7 @@
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
-                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0,
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int md,
                                      hipStream_t stream);
 extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, int phase,
                                        hipStream_t stream);
@@ -89,6 +90,7 @@ struct fivonagi_ctx {
   std::vector<fv::InstDesc> desc;          // host copy of the instance table
   std::vector<fv::WorkItem> items;
   std::vector<uint64_t> rowmask;           // CFL lane mask per item row (fv::WorkItem::mrow)
+  bool md_any = false;                     // some instance has a moving window
   float skew = -1.0f;                      // height skew of the work items (< 0: automatic)
   float skew_used = 0.0f;                  // the skew of the current work list
   bool rotate = false;                     // level pattern rotated per strip (chunk_bounds)
@@ -360,7 +362,7 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
     if (e != hipSuccess) return e;
   }
   return fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(), phase,
-                        c->n_inst, c->d_ctl, c->d_sync, c->stream);
+                        c->n_inst, c->d_ctl, c->d_sync, c->md_any ? 1 : 0, c->stream);
 }
 
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
@@ -407,10 +409,16 @@ int check_ctx(fivonagi_ctx* c, int inst) {
   return 0;
 }
 
-int write_targets(fivonagi_ctx* c, const std::vector<int64_t>& tgt) {
+// Stop conditions of the attempts launched from now on: an instance runs
+// while n_acc < tgt[k] and T < T_stop.
+int write_targets(fivonagi_ctx* c, const std::vector<int64_t>& tgt,
+                  double T_stop = std::numeric_limits<double>::infinity()) {
   std::vector<fv::Ctl> ctl;
   if (int rc = read_ctls(c, ctl)) return rc;
-  for (int k = 0; k < c->n_inst; ++k) ctl[k].target_acc = tgt[k];
+  for (int k = 0; k < c->n_inst; ++k) {
+    ctl[k].target_acc = tgt[k];
+    ctl[k].T_stop = T_stop;
+  }
   return write_ctls(c, ctl);
 }
 
@@ -434,6 +442,10 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     if (q.cfl_mode < 0 || q.cfl_mode > 1) return fail(FIVONAGI_ERR_ARG, "unknown cfl mode");
     if (!q_host[3 * k] || !q_host[3 * k + 1] || !q_host[3 * k + 2])
       return fail(FIVONAGI_ERR_ARG, "null state pointer");
+    if (q.md_on != 0 && q.md_on != 1) return fail(FIVONAGI_ERR_ARG, "md_on must be 0 or 1");
+    if (q.md_on && !(q.md_vx >= 0 && q.md_vy >= 0 && std::isfinite(q.md_vx) &&
+                     std::isfinite(q.md_vy) && std::isfinite(q.md_t)))
+      return fail(FIVONAGI_ERR_ARG, "moving window: md_vx, md_vy must be finite and >= 0");
   }
   auto* c = new fivonagi_ctx();
   c->n_inst = n_inst;
@@ -518,6 +530,9 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     if (q.bc == FIVONAGI_BC_TAYLOR)
       c->taylor_ring = std::max(c->taylor_ring, 4 * q.nx + 4 * (q.ny - 4));
     d.beta2m2 = (float)(2.0 * (q.beta - 1.0));
+    d.md_on = q.md_on;
+    d.md_vx = q.md_vx; d.md_vy = q.md_vy; d.md_t = q.md_t;
+    if (q.md_on) c->md_any = true;
 
     // attempt 0 starts from block 0; slot set 2 (attempt -1) never ran
     auto& t = ctl[2 * (size_t)k];
@@ -525,6 +540,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     t.dt = q.dt0;
     t.T = q.T0;
     t.target_acc = 0;
+    t.T_stop = std::numeric_limits<double>::infinity();
     set_ctl_derived(t, d);
     ctl[2 * (size_t)k + 1] = t;
   }
@@ -721,6 +737,32 @@ int fivonagi_step(fivonagi_ctx* c, int64_t nsteps) {
   return 0;
 }
 
+int fivonagi_step_until(fivonagi_ctx* c, double T_stop) {
+  if (!c || std::isnan(T_stop)) return fail(FIVONAGI_ERR_ARG, "bad argument");
+  std::vector<fv::Ctl> ctl;
+  if (int rc = read_ctls(c, ctl)) return rc;
+  for (int k = 0; k < c->n_inst; ++k)
+    if (ctl[k].err) return fivonagi_sync(c);
+  if (int rc = write_targets(c, std::vector<int64_t>(c->n_inst, INT64_MAX), T_stop)) return rc;
+  c->unlimited = false;
+  // rounds of attempts sized by the slowest instance's remaining time over
+  // its dt (+10%); an instance past T_stop skips the extra attempts
+  for (int round = 0;; ++round) {
+    int64_t need = 0;
+    for (int k = 0; k < c->n_inst; ++k) {
+      if (ctl[k].err) return fivonagi_sync(c);
+      if (!(ctl[k].T < T_stop)) continue;
+      const double steps = (T_stop - ctl[k].T) / ctl[k].dt;
+      need = std::max<int64_t>(need, (int64_t)std::min(1e9, std::ceil(1.1 * steps)) + 1);
+    }
+    if (need == 0) break;
+    if (round > 10000) return fail(FIVONAGI_ERR_STALL, "T does not reach T_stop");
+    if (int rc = launch_attempts(c, std::min<int64_t>(need, 1 << 20))) return rc;
+    if (int rc = read_ctls(c, ctl)) return rc;
+  }
+  return 0;
+}
+
 int fivonagi_collect(fivonagi_ctx* c, int inst, float* q1, float* q2, float* q3, double* T,
                      double* dt, int64_t* n_acc, int64_t* n_rej) {
   if (int rc = check_ctx(c, inst)) return rc;
@@ -757,6 +799,8 @@ int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
   info->n_items = c->desc[inst].n_items;
   info->skew = c->skew_used;
   info->rotate = c->rotate_used ? 1 : 0;
+  info->md_x = t.mdx;
+  info->md_y = t.mdy;
   // mean rows per work item (heights differ when the split is skewed)
   const auto& p = c->params[inst];
   const int strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
@@ -795,6 +839,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
                           double* grad_phase) {
   if (!c || !obj || !J || nsteps <= 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
   if (nsteps > (1 << 30)) return fail(FIVONAGI_ERR_ARG, "too many steps");
+  if (c->md_any) return fail(FIVONAGI_ERR_ARG, "the adjoint does not support a moving window");
   const int ni = c->n_inst;
   for (int k = 0; k < ni; ++k) {
     const auto& p = c->params[k];

@@ -134,7 +134,7 @@ __device__ __forceinline__ FoldRaw fold_load(const Ctl* __restrict__ c,
   p.target_acc = c->target_acc;
   p.cur = c->cur; p.err = c->err;
   p.dtx = c->dtx; p.dty = c->dty; p.ddx2 = c->ddx2; p.ddy2 = c->ddy2;
-  p.pad0 = 0; p.pad1 = 0;
+  p.mdx = c->mdx; p.mdy = c->mdy; p.T_stop = c->T_stop;
   r.ran = sl->ran;
   r.errf = sl->err_flag;
   // the group maxima with one vector load (lane l reads group l % kGroups)
@@ -149,6 +149,7 @@ __device__ __forceinline__ Fold fold_decide(const FoldRaw& r, double cfl_max, bo
   f.p.n_acc = uni64(r.p.n_acc); f.p.n_rej = uni64(r.p.n_rej); f.p.n_att = uni64(r.p.n_att);
   f.p.target_acc = uni64(r.p.target_acc);
   f.p.cur = uni(r.p.cur); f.p.err = uni(r.p.err);
+  f.p.mdx = uni(r.p.mdx); f.p.mdy = uni(r.p.mdy);
   // non-negative floats order like their bit patterns
   uint32_t mbits = 0u;
 #pragma unroll
@@ -164,6 +165,13 @@ __device__ __forceinline__ Fold fold_decide(const FoldRaw& r, double cfl_max, bo
 // Ping-pong index attempt a+1 reads (known before the rest of advance()).
 __device__ __forceinline__ int next_cur(const Fold& f) {
   return f.state == kFoldAccept ? (f.p.cur ^ 1) : f.p.cur;
+}
+
+// Moving window (common/main.cu:109-135, :167-176): once T >= MDT the window
+// sits int((T - MDT) * MDV / d) cells from its start; after an accepted step
+// ending at T it moves by the part not applied yet (md = the current offset).
+__device__ __forceinline__ int md_shift(double T, double md_t, double v, double d, int md) {
+  return T >= md_t ? (int)((T - md_t) * v / d) - md : 0;
 }
 
 // Control block of attempt a+1 from attempt a's fold: common/main.cu:199-230
@@ -196,6 +204,10 @@ __device__ Ctl advance(const InstDesc& D, const Fold& f, bool log) {
     o.T = T;
     o.n_acc = p.n_acc + 1;
     o.cur = p.cur ^ 1;
+    if (D.md_on) {
+      o.mdx = p.mdx + md_shift(T, D.md_t, D.md_vx, D.dx, p.mdx);
+      o.mdy = p.mdy + md_shift(T, D.md_t, D.md_vy, D.dy, p.mdy);
+    }
   } else {
     o.n_rej = p.n_rej + 1;
   }
@@ -250,7 +262,13 @@ struct XY {          // one cell of q_xy, with its velocities for the source
 // block of attempt a = ctl + a % 2, sync_in = the slots of attempt a-1 =
 // sync + (a + 2) % 3), so the fold's loads go out together with the work
 // item's instead of after the instance description's.
-template <bool kOne>
+// kMd: some instance has a moving window.  The shift that follows an accepted
+// step (moveDomain, common/main.cu:109-164) is folded into the stores: cell
+// (i, j) of the new state lands at (i - sx, j - sy), and also stays at (i, j)
+// when (i + sx, j + sy) is off the grid (moveDomain's else branch); every
+// cell of the output is written exactly once.  A rejected attempt's output
+// is discarded, as without the window.
+template <bool kOne, bool kMd>
 __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
                                                      const WorkItem* __restrict__ items,
                                                      const uint64_t* __restrict__ rowmask,
@@ -326,7 +344,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   const int colA = taylor ? ic : (left ? 2 : (ic >= nx - 2 ? nx - 3 : ic));
   const int colB = taylor ? ic : (left ? 2 : ic);
   const uint32_t bA = 4u * (uint32_t)colA, bB = 4u * (uint32_t)colB;
-  const bool drive_lane = left && (bc == kBcEdgeDrive);
   const uint32_t row_bytes = 4u * (uint32_t)nx;
 
   // raw (unpatched) load of row r, with the row's drive coefficients; the
@@ -353,7 +370,7 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
 
   // ---- control block of this attempt ----
   const Ctl P = advance(D, F, local == 0);
-  const bool runs = P.err == 0 && P.n_acc < P.target_acc;
+  const bool runs = P.err == 0 && P.n_acc < P.target_acc && P.T < P.T_stop;
   if (local == 0 && lane == 0) {
     // one wave per instance: publish the control block for the next launch,
     // clear the slots the next attempt reduces into, mark this attempt
@@ -370,8 +387,20 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   const float ddx2 = unif(P.ddx2), ddy2 = unif(P.ddy2);
   const float sin_t = unif((float)P.drive_s), cos_t = unif((float)P.drive_c);
   const float hbeta = 0.5f * D.beta2m2;     // BETA - 1
+  // the edge source runs while the window has not moved in x
+  // (apps/hifu-beam/boundary.h:82); after that the left ghost columns are
+  // plain zero-gradient copies
+  const bool drive = (bc == kBcEdgeDrive) && (!kMd || P.mdx == 0);
+  const bool drive_lane = left && drive;
+  // shift of the window after this attempt, if it is accepted
+  int sx = 0, sy = 0;
+  if (kMd && D.md_on) {
+    const double Tn = P.T + P.dt;
+    sx = uni(md_shift(Tn, D.md_t, D.md_vx, D.dx, P.mdx));
+    sy = uni(md_shift(Tn, D.md_t, D.md_vy, D.dy, P.mdy));
+  }
   auto patch = [&](Row& q, float ec, float es) {
-    if (bc == kBcEdgeDrive) {
+    if (drive) {
       float a, b, c;
       fvbc::drive(ec, es, sin_t, cos_t, a, b, c);
       q.q1 = drive_lane ? a : q.q1;
@@ -397,11 +426,26 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     r3 = s_int ? r3 : mid.q3;
     pin(r2);
     pin(r3);
-    if (out_lane) {
-      const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
-      stg(o1, o, mid.q1);            // q1 passes the source unchanged
-      stg(o2, o, r2);
-      stg(o3, o, r3);
+    if (!kMd) {
+      if (out_lane) {
+        const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
+        stg(o1, o, mid.q1);            // q1 passes the source unchanged
+        stg(o2, o, r2);
+        stg(o3, o, r3);
+      }
+    } else {
+      if (out_lane && i >= sx && s >= sy) {
+        const uint32_t o = (uint32_t)(s - sy) * row_bytes + 4u * (uint32_t)(i - sx);
+        stg(o1, o, mid.q1);
+        stg(o2, o, r2);
+        stg(o3, o, r3);
+      }
+      if (out_lane && (i + sx > nx - 1 || s + sy > ny - 1)) {
+        const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
+        stg(o1, o, mid.q1);
+        stg(o2, o, r2);
+        stg(o3, o, r3);
+      }
     }
   };
 
@@ -542,7 +586,7 @@ __global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __
   const InstDesc& D = insts[blockIdx.y];
   if (D.bc != kBcTaylor) return;
   const Ctl P = current(D, phase, false);
-  if (P.err != 0 || P.n_acc >= P.target_acc) return;
+  if (P.err != 0 || P.n_acc >= P.target_acc || !(P.T < P.T_stop)) return;
   const int nx = D.nx, ny = D.ny;
   const int ring = 4 * nx + 4 * (ny - 4);
   const double T = P.T;
@@ -560,8 +604,10 @@ __global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __
       i = cc < 2 ? cc : nx - 4 + cc;
     }
     float a, b, c;
+    // the closed form is taken at the cell's position in the unmoved frame
+    // (apps/taylor-angle-xy/boundary.h:45-49: i + MDX, j + MDY)
     fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx, D.dy,
-                 T, i, j, a, b, c);
+                 T, i + P.mdx, j + P.mdy, a, b, c);
     const size_t o = (size_t)j * (size_t)nx + (size_t)i;
     D.q[cur][0][o] = a;
     D.q[cur][1][o] = b;
@@ -589,25 +635,32 @@ extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, in
 
 // Resident step waves per CU (occupancy of the step kernel on this device).
 extern "C" hipError_t fv_step_waves_per_cu(int* waves) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(waves, fv::fv_step_kernel<false>, fv::kLanes,
-                                                      0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      waves, fv::fv_step_kernel<false, false>, fv::kLanes, 0);
 }
 
 // One step attempt of every instance.  ctl0/sync0: the control blocks and
-// slot sets of the only instance when n_inst == 1 (null otherwise).
+// slot sets of the only instance when n_inst == 1 (null otherwise); md: some
+// instance has a moving window.
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
-                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0,
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int md,
                                      hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
-  if (n_inst == 1 && ctl0 && sync0)
-    hipLaunchKernelGGL(fv::fv_step_kernel<true>, dim3(n_items), dim3(fv::kLanes), 0, stream,
-                       insts, items, rowmask, n_items, phase,
-                       (const fv::Ctl*)(ctl0 + (phase & 1)),
-                       (const fv::SyncSlots*)(sync0 + (phase + 2) % 3));
+  const bool one = n_inst == 1 && ctl0 && sync0;
+  const fv::Ctl* c_in = one ? ctl0 + (phase & 1) : nullptr;
+  const fv::SyncSlots* s_in = one ? sync0 + (phase + 2) % 3 : nullptr;
+  if (one && !md)
+    hipLaunchKernelGGL((fv::fv_step_kernel<true, false>), dim3(n_items), dim3(fv::kLanes), 0,
+                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
+  else if (one)
+    hipLaunchKernelGGL((fv::fv_step_kernel<true, true>), dim3(n_items), dim3(fv::kLanes), 0,
+                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
+  else if (!md)
+    hipLaunchKernelGGL((fv::fv_step_kernel<false, false>), dim3(n_items), dim3(fv::kLanes), 0,
+                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
   else
-    hipLaunchKernelGGL(fv::fv_step_kernel<false>, dim3(n_items), dim3(fv::kLanes), 0, stream,
-                       insts, items, rowmask, n_items, phase, (const fv::Ctl*)nullptr,
-                       (const fv::SyncSlots*)nullptr);
+    hipLaunchKernelGGL((fv::fv_step_kernel<false, true>), dim3(n_items), dim3(fv::kLanes), 0,
+                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
   return hipGetLastError();
 }

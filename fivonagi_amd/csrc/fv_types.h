@@ -32,7 +32,8 @@ struct Ctl {
   int32_t err;          // 0 ok, 2 imaginary eigenvalue
   float dtx, dty;       // dt/dx, dt/dy of the next attempt
   float ddx2, ddy2;     // delta*dt/dx^2, delta*dt/dy^2
-  uint32_t pad0, pad1;
+  int32_t mdx, mdy;     // moving-window offset of the current state (MDX, MDY)
+  double T_stop;        // ... or once T reaches this (the frame loop's Tprint)
 };
 
 // Reduction slots of one attempt, per instance: the waves of an instance
@@ -75,9 +76,10 @@ struct InstDesc {
   int32_t n_items;      // work items (waves) of this instance per step
   int32_t pad2;
   int32_t tay_jc;
-  int32_t pad1;
+  int32_t md_on;        // moving window (common/main.cu:109-177)
   float beta2m2;        // 2*(BETA - 1)
   float pad;
+  double md_vx, md_vy, md_t;   // MDVX, MDVY, MDT
 };
 
 // One wave's work: a 58-column strip of one instance, rows [y0, y1).

@@ -23,8 +23,9 @@ CELLS_PER_UNIT = 20.0
 
 
 def gauss_pulse(nx=400, ny=400, amp=0.01, sigma_cells=10.0, beta=3.5, delta=1e-4,
-                cfl_mode=CFL_REFERENCE_FOLD, dt0=0.005, rows_per_wave=0):
-    """Gaussian rho' pulse centred at ((nx-1)/2, (ny-1)/2), zero velocity."""
+                cfl_mode=CFL_REFERENCE_FOLD, dt0=0.005, rows_per_wave=0, moving=None):
+    """Gaussian rho' pulse centred at ((nx-1)/2, (ny-1)/2), zero velocity.
+    moving: (MDVX, MDVY, MDT) of a moving window, or None."""
     x = np.arange(nx) - (nx - 1) / 2.0
     y = np.arange(ny) - (ny - 1) / 2.0
     X, Y = np.meshgrid(x, y)
@@ -32,7 +33,8 @@ def gauss_pulse(nx=400, ny=400, amp=0.01, sigma_cells=10.0, beta=3.5, delta=1e-4
     q[0] = amp * np.exp(-(X ** 2 + Y ** 2) / (2.0 * sigma_cells ** 2))
     d = 1.0 / CELLS_PER_UNIT
     return Problem(nx, ny, d, d, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=dt0,
-                   T0=0.0, bc=BC_ZEROGRAD, cfl_mode=cfl_mode, rows_per_wave=rows_per_wave)
+                   T0=0.0, bc=BC_ZEROGRAD, cfl_mode=cfl_mode, rows_per_wave=rows_per_wave,
+                   moving=moving)
 
 
 def edge_profile(ny, amp=0.02, halfwidth_cells=200.0, centre=None):
@@ -43,9 +45,11 @@ def edge_profile(ny, amp=0.02, halfwidth_cells=200.0, centre=None):
 
 def edge_drive(nx=4096, ny=1024, amp=0.02, halfwidth_cells=200.0, cells_per_wavelength=20.0,
                beta=3.5, delta=5e-5, cfl_mode=CFL_REFERENCE_FOLD, dt0=0.005,
-               edge_amp=None, edge_phase=None, rows_per_wave=0):
+               edge_amp=None, edge_phase=None, rows_per_wave=0, moving=None):
     """Quiescent medium driven from the left ghost columns:
-    rho' = amp_j sin(omega T + phase_j), omega = 2*pi/(wavelength in time units)."""
+    rho' = amp_j sin(omega T + phase_j), omega = 2*pi/(wavelength in time units).
+    moving: (MDVX, MDVY, MDT) of a moving window (the drive stops once the
+    window has moved), or None."""
     d = 1.0 / CELLS_PER_UNIT
     omega = 2.0 * math.pi * CELLS_PER_UNIT / cells_per_wavelength
     q = np.zeros((3, ny, nx))
@@ -53,7 +57,7 @@ def edge_drive(nx=4096, ny=1024, amp=0.02, halfwidth_cells=200.0, cells_per_wave
         edge_amp = edge_profile(ny, amp, halfwidth_cells)
     return Problem(nx, ny, d, d, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=dt0, T0=0.0,
                    bc=BC_EDGE_DRIVE, cfl_mode=cfl_mode, omega=omega, edge_amp=edge_amp,
-                   edge_phase=edge_phase, rows_per_wave=rows_per_wave)
+                   edge_phase=edge_phase, rows_per_wave=rows_per_wave, moving=moving)
 
 
 def plane_packet(nx, ny, amp, theta, wavelength_cells=20.0, width_cells=60.0):
@@ -100,9 +104,12 @@ def batched(n=64, seed=1234, lo=256, hi=2048, step=64, cfl_mode=CFL_REFERENCE_FO
 TAYLOR_C = 2.1972245773362196   # apps/taylor-angle-xy/parameters.h:105
 
 
-def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_REFERENCE_FOLD):
+def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_REFERENCE_FOLD,
+           moving=False):
     """apps/taylor-angle-xy: NX = NY = 64*ETAC, XMAX = YMAX = 12.5; the exact
-    Taylor profile is both the initial state and the Dirichlet ghost data."""
+    Taylor profile is both the initial state and the Dirichlet ghost data.
+    moving: the app's moving window, MDVX = cos(theta), MDVY = sin(theta),
+    MDT = XMAX/2/cos(theta) (apps/taylor-angle-xy/parameters.h:92-94)."""
     nx = ny = 64 * eta_mult
     dx = dy = xmax / (nx - 1)
     rhoa = -delta / beta
@@ -117,8 +124,9 @@ def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_
                                    * math.sin(theta) - T0))
     q = np.stack([r, r * math.cos(theta), r * math.sin(theta)])
     t = dict(rhoa=rhoa, c=TAYLOR_C, theta=theta, yoff=yoff, jc=jc)
+    md = (math.cos(theta), math.sin(theta), mdt) if moving else None
     return Problem(nx, ny, dx, dy, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=0.005,
-                   T0=T0, bc=BC_TAYLOR, cfl_mode=cfl_mode, taylor=t)
+                   T0=T0, bc=BC_TAYLOR, cfl_mode=cfl_mode, taylor=t, moving=md)
 
 
 def aperture_params(ny, n_rows, rng, amp=0.02, halfwidth_rows=None, noise=0.05):

@@ -61,6 +61,14 @@ typedef struct {
   double tay_rhoa, tay_c, tay_theta, tay_yoff; /* Taylor closed form          */
   int32_t tay_jc;
   int32_t rows_per_wave;   /* 0 = automatic; else rows per work item          */
+  /* Moving window (common/main.cu:109-177): after every accepted step with
+     T >= md_t the grid content shifts so that the window sits
+     int((T - md_t) * md_v / d) cells from its start (MDX, MDY); cells whose
+     source falls off the far edge keep their value.  md_v >= 0.  The edge
+     drive runs only while MDX == 0 (apps/hifu-beam/boundary.h:82); the Taylor
+     closed form is taken at (i + MDX, j + MDY). */
+  int32_t md_on;           /* 0 = fixed domain                                */
+  double md_vx, md_vy, md_t; /* MDVX, MDVY, MDT                               */
 } fivonagi_params;
 
 typedef struct {
@@ -70,6 +78,7 @@ typedef struct {
   int32_t n_items, rows_per_wave; /* work items per step, mean rows per item */
   double skew;             /* height skew of the work items in use            */
   int32_t rotate, pad;     /* 1: skew levels rotated per strip                */
+  int32_t md_x, md_y;      /* moving-window offset MDX, MDY                   */
 } fivonagi_info_t;
 
 /* gpu_init (common/data_definitions.h:523-871) + init() upload
@@ -87,6 +96,11 @@ int fivonagi_set_dt_sequence(fivonagi_ctx *ctx, int inst, const double *dt, int6
 /* The while (T < Tprint) body of common/main.cu:310-635, nsteps accepted
    steps for every instance (rejected attempts retried).  Blocking. */
 int fivonagi_step(fivonagi_ctx *ctx, int64_t nsteps);
+
+/* The frame loop `while (T < Tprint) { step }` of calcFrame
+   (common/main.cu:310): attempts until every instance has T >= T_stop
+   (rejected attempts retried).  Blocking. */
+int fivonagi_step_until(fivonagi_ctx *ctx, double T_stop);
 
 /* Asynchronous: enqueue nattempts step attempts (graph replays) on the
    context stream; no host synchronisation.  For timing loops. */

@@ -85,7 +85,7 @@ int orc_boundary_f64(const orc_params *p, const double *in1, const double *in2,
   if (check_grid(p)) return ORC_ERR_ARG;
   double *src[3] = {(double *)in1, (double *)in2, (double *)in3};
   double *dst[3] = {o1, o2, o3};
-  boundary_f64(p, src, dst, T);
+  boundary_f64(p, src, dst, T, 0, 0);
   return 0;
 }
 

@@ -60,12 +60,24 @@ typedef struct {
   /* prescribed dt: attempt k uses dt_seq[k]; NULL = CFL feedback */
   const double *dt_seq;
   int64_t dt_seq_len;
+  /* moving window (common/main.cu:109-177): after each accepted step with
+     T >= md_t the grid shifts by int((T - md_t)*md_v/d) - MD cells (moveDomain,
+     cells whose source is off the far edge keep their value) and MD grows by
+     that much (updateMDXY).  The edge drive runs only while MDX == 0
+     (apps/hifu-beam/boundary.h:82); Taylor ghosts sit at (i+MDX, j+MDY). */
+  int32_t md_on;
+  /* stop_at_T: orc_run also stops once T >= T_stop (the frame loop's
+     while (T < Tprint), common/main.cu:310) */
+  int32_t stop_at_T;
+  double md_vx, md_vy, md_t;
+  double T_stop;
 } orc_params;
 
 typedef struct {
   double T, dt, last_cfl;
   int64_t n_accepted, n_rejected, n_attempts;
   int32_t error, pad;
+  int32_t mdx, mdy;             /* moving-window offset MDX, MDY            */
 } orc_state;
 
 /* Per-interface Roe-type solve (apps/na/fv.h:88-180), generic in the normal

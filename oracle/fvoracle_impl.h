@@ -143,7 +143,7 @@ static REAL SFX(philim)(REAL wnorm2, REAL dotl, REAL dotr, REAL s) {
  * the other texture, so corner ghosts lag one step).
  * ------------------------------------------------------------------------- */
 static void SFX(boundary)(const orc_params *p, REAL *const src[3],
-                          REAL *const dst[3], double T) {
+                          REAL *const dst[3], double T, int mdx, int mdy) {
   const int NX = p->nx, NY = p->ny;
 #pragma omp parallel for schedule(static)
   for (int j = 0; j < NY; ++j) {
@@ -151,13 +151,14 @@ static void SFX(boundary)(const orc_params *p, REAL *const src[3],
       size_t o = (size_t)i + (size_t)j * NX;
       int ir = i, jr = j;
       if (p->bc == ORC_BC_TAYLOR) {
-        /* apps/taylor-angle-xy/boundary.h:38-55 (MDX = MDY = 0) */
+        /* apps/taylor-angle-xy/boundary.h:38-55, at (i + MDX, j + MDY) */
         if (i == NX - 1 || i == NX - 2 || j == 0 || j == 1 || j == NY - 1 ||
             j == NY - 2 || i == 0 || i == 1) {
           double th = p->tay_theta;
           double rhop = p->tay_rhoa *
-              tanh(p->tay_c * ((double)i * p->dx * cos(th) +
-                               ((double)(j - p->tay_jc) * p->dy + p->tay_yoff) * sin(th) - T));
+              tanh(p->tay_c * ((double)(i + mdx) * p->dx * cos(th) +
+                               ((double)(j + mdy - p->tay_jc) * p->dy + p->tay_yoff) * sin(th) -
+                               T));
           dst[0][o] = (REAL)rhop;
           dst[1][o] = (REAL)(rhop * cos(th));
           dst[2][o] = (REAL)(rhop * sin(th));
@@ -171,7 +172,7 @@ static void SFX(boundary)(const orc_params *p, REAL *const src[3],
       if (j == 0 || j == 1) { ir = i; jr = 2; }
       if (j == NY - 1 || j == NY - 2) { ir = i; jr = NY - 3; }
       if (i == 0 || i == 1) {
-        if (p->bc == ORC_BC_EDGE_DRIVE) {
+        if (p->bc == ORC_BC_EDGE_DRIVE && mdx == 0) {
           /* apps/hifu-beam/boundary.h:77-93 with nu = (1, 0):
              u1 = pack, u2 = nux*pack*(pack+1), u3 = nuy*pack*(pack+1) */
           double amp = p->edge_amp ? p->edge_amp[j] : 0.0;
@@ -386,13 +387,36 @@ static void SFX(sweep)(SFX(ctx_t) *c, const orc_params *p, int axis, REAL dt) {
   SFX(write_limiters)(c, p, axis, c->B, c->A, dt);
 }
 
+/* moveDomain + updateMDXY, common/main.cu:109-176, on the accepted state in
+   c->C at the updated T (main.cu:575-586): C(i,j) <- C(i+MDi, j+MDj) when
+   that cell is on the grid (the reference checks only the upper bounds;
+   md_v >= 0 keeps MDi, MDj >= 0), else C(i,j) unchanged. */
+static void SFX(move_domain)(SFX(ctx_t) *c, const orc_params *p, orc_state *st) {
+  if (!(st->T >= p->md_t)) return;
+  const int NX = c->nx, NY = c->ny;
+  const int mdi = (int)((st->T - p->md_t) * p->md_vx / p->dx) - st->mdx;
+  const int mdj = (int)((st->T - p->md_t) * p->md_vy / p->dy) - st->mdy;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < NY; ++j) {
+    for (int i = 0; i < NX; ++i) {
+      const int ir = i + mdi, jr = j + mdj;
+      const size_t o = (size_t)i + (size_t)j * NX;
+      const size_t os = (ir <= NX - 1 && jr <= NY - 1) ? (size_t)ir + (size_t)jr * NX : o;
+      for (int f = 0; f < 3; ++f) c->A[f][o] = c->C[f][os];
+    }
+  }
+  for (int f = 0; f < 3; ++f) memcpy(c->C[f], c->A[f], c->n * sizeof(REAL));
+  st->mdx += mdi;
+  st->mdy += mdj;
+}
+
 /* One step attempt on the state in c->C (the backup buffer holds the input,
    main.cu:334).  Returns 0 accepted, 1 rejected, 2 eigenvalue error. */
 static int SFX(attempt)(SFX(ctx_t) *c, const orc_params *p, orc_state *st,
                         double dt_used, int prescribed) {
   REAL dt = (REAL)dt_used;
   c->simError = 0;
-  SFX(boundary)(p, c->C, c->A, st->T);                    /* main.cu:338 */
+  SFX(boundary)(p, c->C, c->A, st->T, st->mdx, st->mdy);  /* main.cu:338 */
   SFX(sweep)(c, p, 1, dt);                                /* main.cu:353-418 */
   if (c->simError == 2) return 2;                         /* main.cu:361-369 */
   SFX(sweep)(c, p, 0, dt);                                /* main.cu:430-495 */
@@ -406,6 +430,7 @@ static int SFX(attempt)(SFX(ctx_t) *c, const orc_params *p, orc_state *st,
     for (int f = 0; f < 3; ++f) memcpy(c->C[f], c->B[f], c->n * sizeof(REAL));
     st->T += (double)dt;                                  /* main.cu:232-238 */
     st->n_accepted++;
+    if (p->md_on) SFX(move_domain)(c, p, st);             /* main.cu:575-586 */
   } else {
     st->n_rejected++;                                     /* clean retry */
   }
@@ -430,7 +455,7 @@ static int SFX(run_impl)(const orc_params *p, REAL *q1, REAL *q2, REAL *q3,
   memcpy(c.C[2], q3, c.n * sizeof(REAL));
   int rc = 0, consecutive_rej = 0;
   int64_t target = st->n_accepted + n_accept;
-  while (st->n_accepted < target) {
+  while (st->n_accepted < target && !(p->stop_at_T && !(st->T < p->T_stop))) {
     int prescribed = p->dt_seq != NULL;
     double dt_used = st->dt;
     if (prescribed) {

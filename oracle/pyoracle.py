@@ -31,6 +31,9 @@ class Params(C.Structure):
         ("tay_theta", C.c_double), ("tay_yoff", C.c_double),
         ("tay_jc", C.c_int32), ("pad0", C.c_int32),
         ("dt_seq", C.POINTER(C.c_double)), ("dt_seq_len", C.c_int64),
+        ("md_on", C.c_int32), ("stop_at_T", C.c_int32),
+        ("md_vx", C.c_double), ("md_vy", C.c_double), ("md_t", C.c_double),
+        ("T_stop", C.c_double),
     ]
 
 
@@ -39,6 +42,7 @@ class State(C.Structure):
         ("T", C.c_double), ("dt", C.c_double), ("last_cfl", C.c_double),
         ("n_accepted", C.c_int64), ("n_rejected", C.c_int64),
         ("n_attempts", C.c_int64), ("error", C.c_int32), ("pad", C.c_int32),
+        ("mdx", C.c_int32), ("mdy", C.c_int32),
     ]
 
 
@@ -102,7 +106,7 @@ class Problem:
     def __init__(self, nx, ny, dx, dy, beta, delta, cfl_max=1.0, cfl_wish=0.9,
                  dt0=0.005, T0=0.0, bc=BC_ZEROGRAD, cfl_mode=CFL_REFERENCE_FOLD,
                  form=FORM_LITERAL, omega=0.0, edge_amp=None, edge_phase=None,
-                 taylor=None, dt_seq=None, nthreads=0):
+                 taylor=None, dt_seq=None, nthreads=0, moving=None, T_stop=None):
         self.nx, self.ny = int(nx), int(ny)
         self.dx, self.dy = float(dx), float(dy)
         self.beta, self.delta = float(beta), float(delta)
@@ -115,6 +119,8 @@ class Problem:
         self.taylor = taylor  # dict(rhoa, c, theta, yoff, jc)
         self.dt_seq = None if dt_seq is None else np.ascontiguousarray(dt_seq, np.float64)
         self.nthreads = int(nthreads)
+        self.moving = None if moving is None else tuple(float(v) for v in moving)
+        self.T_stop = None if T_stop is None else float(T_stop)
 
     def cparams(self):
         p = Params()
@@ -134,11 +140,18 @@ class Problem:
         if self.dt_seq is not None:
             p.dt_seq = _ptr(self.dt_seq, C.c_double)
             p.dt_seq_len = len(self.dt_seq)
+        if self.moving is not None:
+            p.md_on = 1
+            p.md_vx, p.md_vy, p.md_t = self.moving
+        if self.T_stop is not None:
+            p.stop_at_T = 1
+            p.T_stop = self.T_stop
         return p
 
 
 def run(prob, q, nsteps, dtype=np.float64, state=None, fast=False):
-    """Advance q (3, ny, nx) by nsteps accepted steps.  Returns (q, state, rc)."""
+    """Advance q (3, ny, nx) by nsteps accepted steps (or until T >= prob.T_stop
+    when set).  Returns (q, state, rc)."""
     L = lib(fast)
     p = prob.cparams()
     q = np.ascontiguousarray(q, dtype=dtype).copy()

@@ -6,6 +6,8 @@ FP32 rounding alone (the oracle's own FP32 build, rho'-direct form) is
 GPU path is held to 1e-5 on runs of up to ~300 steps; longer runs are checked
 against the FP32 oracle's own error (the GPU may be at most 5x worse).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -149,6 +151,71 @@ def test_skew_invariance():
         assert ctx.info(0)["skew"] == 0.0     # one item per SIMD: no skew
         with pytest.raises(fa.FivonagiError):
             ctx.set_skew(1.5)
+
+
+@pytest.mark.parametrize("case", ["pulse", "edge_drive", "taylor"])
+def test_moving_window_matches_oracle(oracle, case):
+    """moveDomain/updateMDXY (common/main.cu:109-177), folded into the step's
+    stores, against the oracle's separate shift; MDX, MDY and the counts
+    must match exactly."""
+    if case == "pulse":       # both directions, window starting mid-run
+        prob = wl.gauss_pulse(200, 130, sigma_cells=8.0, moving=(1.0, 0.5, 0.3))
+        n = 60
+    elif case == "edge_drive":  # the drive stops once the window has moved
+        prob = wl.edge_drive(300, 90, halfwidth_cells=20.0, moving=(1.0, 0.0, 1.0))
+        n = 80
+    else:                     # the app's own window (parameters.h:92-95)
+        prob = wl.taylor(1, theta=math.pi / 6, moving=True)
+        n = 110
+    q, st = run_gpu(prob, n)
+    qo, sto = run_oracle(oracle, prob, n)
+    assert (st["md_x"], st["md_y"]) == (sto.mdx, sto.mdy)
+    assert st["md_x"] > 3
+    assert st["n_accepted"] == sto.n_accepted and st["n_rejected"] == sto.n_rejected
+    e = rel_linf(q, qo)
+    report(f"moving window {case} (MDX={sto.mdx}, MDY={sto.mdy})", e)
+    if case == "taylor":      # q3 as in test_taylor
+        q32, _ = run_oracle(oracle, prob, n, dtype=np.float32)
+        assert max(e[0], e[1]) < TOL and e[2] < 5 * rel_linf(q32, qo)[2]
+    else:
+        assert max(e) < TOL
+
+
+def test_moving_window_in_a_batch(oracle):
+    """A batch where one instance moves and one does not (the multi-instance
+    form of the moving-window kernel)."""
+    probs = [wl.gauss_pulse(150, 100, sigma_cells=7.0, moving=(1.0, 0.0, 0.2)),
+             small_pulse(64, 64)]
+    with fa.Context(probs) as ctx:
+        ctx.step(50)
+        outs = [ctx.collect(k) for k in range(2)]
+        md = [(ctx.info(k)["md_x"], ctx.info(k)["md_y"]) for k in range(2)]
+    for k, p in enumerate(probs):
+        qo, sto = run_oracle(oracle, p, 50)
+        assert md[k] == (sto.mdx, sto.mdy)
+        assert max(rel_linf(outs[k][0], qo)) < TOL
+    assert md[0][0] > 3 and md[1] == (0, 0)
+
+
+def test_step_until_matches_oracle(oracle):
+    """The frame loop while (T < Tprint) (common/main.cu:310): frames of
+    DTPRINT until finalTime, against the oracle stopping at the same T."""
+    prob = wl.edge_drive(200, 96, halfwidth_cells=20.0, moving=(1.0, 0.0, 2.0))
+    frames = []
+    with fa.Context(prob) as ctx:
+        for k in range(1, 5):
+            ctx.step_until(0.75 * k)
+            q, st = ctx.collect(0)
+            frames.append((q, st))
+    state = None
+    q_or = prob.q0.astype(np.float64)
+    for k, (q, st) in enumerate(frames, start=1):
+        op = to_oracle(prob, T_stop=0.75 * k)
+        q_or, state, rc = oracle.run(op, q_or, 10 ** 6, state=state)
+        assert rc == 0
+        assert st["T"] >= 0.75 * k and st["n_accepted"] == state.n_accepted
+        assert st["T"] == pytest.approx(state.T, rel=1e-5)
+        assert max(rel_linf(q, q_or)) < TOL
 
 
 def test_determinism():

@@ -355,3 +355,76 @@ def test_rejection_is_a_clean_retry(oracle):
     op = to_oracle(p2)
     _, st1, _ = oracle.run(op, p2.q0.astype(np.float64), 0)   # no attempts
     assert st1.n_attempts == 0
+
+
+# ---------------------------------------------------- moving window, frames --
+
+def test_moving_window_is_a_translation(oracle):
+    # moveDomain (common/main.cu:109-176) only relabels cells: away from the
+    # edges the moving run equals the fixed run read at (i + MDX, j + MDY),
+    # bit for bit, and MDX = int((T - MDT) * MDVX / dx) after every step
+    nst, r = 12, 48                       # r: reach of the edges in 12 steps
+    md = (1.0, 0.5, 0.0)
+    fixed = wl.gauss_pulse(200, 160, sigma_cells=8.0)
+    moving = wl.gauss_pulse(200, 160, sigma_cells=8.0, moving=md)
+    dts = [0.8 * fixed.dx] * nst           # prescribed: 0.8 and 0.4 cells per step
+    outs = []
+    for p in (fixed, moving):
+        q, st, rc = oracle.run(to_oracle(p, dt_seq=dts), p.q0.astype(np.float64), nst)
+        assert rc == 0 and st.n_accepted == nst
+        outs.append((q, st))
+    (qf, stf), (qm, stm) = outs
+    assert (stf.mdx, stf.mdy) == (0, 0)
+    mx, my = stm.mdx, stm.mdy
+    assert mx == int((stm.T - md[2]) * md[0] / fixed.dx) == 9
+    assert my == int((stm.T - md[2]) * md[1] / fixed.dy) == 4
+    sub_m = qm[:, r:160 - r - my, r:200 - r - mx]
+    sub_f = qf[:, r + my:160 - r, r + mx:200 - r]
+    assert np.abs(sub_f[0]).max() > 1e-3          # the pulse is inside the window
+    assert np.array_equal(sub_m, sub_f)
+    # a wrong direction is not a translation
+    assert not np.array_equal(qm[:, r:160 - r - my, r:200 - r - mx],
+                              qf[:, r:160 - r - my, r:200 - r - mx])
+
+
+def test_moving_window_keeps_the_taylor_shock_in_view(oracle):
+    # apps/taylor-angle-xy with its moving window (parameters.h:92-95): the
+    # window follows the shock at unit speed along theta, so long after the
+    # window starts the state still matches the closed form taken at
+    # (i + MDX, j + MDY) as closely as the fixed run does before it starts
+    theta = math.pi / 6
+    p = wl.taylor(2, theta=theta, moving=True)
+    t = p.taylor
+    I = np.arange(p.nx)[None, :]
+    J = np.arange(p.ny)[:, None]
+
+    def err(q, st):
+        exact = t["rhoa"] * np.tanh(t["c"] * ((I + st.mdx) * p.dx * math.cos(theta)
+                                              + ((J + st.mdy - t["jc"]) * p.dy + t["yoff"])
+                                              * math.sin(theta) - st.T))
+        return np.abs(q[0][8:-8, 8:-8] - exact[8:-8, 8:-8]).mean() / abs(t["rhoa"])
+
+    md_t = p.moving[2]
+    op = to_oracle(p, T_stop=md_t)           # up to the start of the window
+    q1, st1, rc = oracle.run(op, p.q0.astype(np.float64), 10 ** 6)
+    assert rc == 0 and st1.T >= md_t and (st1.mdx, st1.mdy) == (0, 0)
+    e_before = err(q1, st1)
+    op = to_oracle(p, T_stop=md_t + 5.0)     # five more time units
+    q2, st2, rc = oracle.run(op, q1, 10 ** 6, state=st1)
+    assert rc == 0 and st2.T >= md_t + 5.0
+    assert st2.mdx == int((st2.T - md_t) * math.cos(theta) / p.dx) > 20
+    assert st2.mdy == int((st2.T - md_t) * math.sin(theta) / p.dy) > 10
+    assert err(q2, st2) < 2.0 * e_before + 1e-3, (err(q2, st2), e_before)
+
+
+def test_run_stops_at_T_stop(oracle):
+    # the frame loop while (T < Tprint) (common/main.cu:310): the run stops
+    # at the first accepted step that reaches T_stop, as a step count does
+    p = wl.gauss_pulse(64, 48, sigma_cells=5.0)
+    q, st, rc = oracle.run(to_oracle(p, T_stop=0.3), p.q0.astype(np.float64), 10 ** 6)
+    assert rc == 0 and st.T >= 0.3
+    n = st.n_accepted
+    q2, st2, rc = oracle.run(to_oracle(p), p.q0.astype(np.float64), n - 1)
+    assert st2.T < 0.3
+    q3, st3, rc = oracle.run(to_oracle(p), q2, 1, state=st2)
+    assert st3.T == st.T and np.array_equal(q3, q)
