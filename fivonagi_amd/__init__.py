@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfivonagi.so")
 
-BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR = 0, 1, 2
+BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR, BC_HIFU = 0, 1, 2, 3
 CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
 OK, ERR_EIGEN, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 8, 9, 10
 
@@ -41,6 +41,8 @@ class Params(C.Structure):
         ("tay_jc", C.c_int32), ("rows_per_wave", C.c_int32),
         ("md_on", C.c_int32),
         ("md_vx", C.c_double), ("md_vy", C.c_double), ("md_t", C.c_double),
+        ("hifu_ampl", C.c_double), ("hifu_frec", C.c_double),
+        ("hifu_a", C.c_double), ("hifu_f", C.c_double),
     ]
 
 
@@ -133,7 +135,7 @@ class Problem:
     def __init__(self, nx, ny, dx, dy, beta, delta, q0, cfl_max=1.0, cfl_wish=0.9,
                  dt0=0.005, T0=0.0, bc=BC_ZEROGRAD, cfl_mode=CFL_REFERENCE_FOLD,
                  omega=0.0, edge_amp=None, edge_phase=None, taylor=None, rows_per_wave=0,
-                 moving=None):
+                 moving=None, hifu=None):
         self.nx, self.ny = int(nx), int(ny)
         self.dx, self.dy, self.beta, self.delta = float(dx), float(dy), float(beta), float(delta)
         self.cfl_max, self.cfl_wish, self.dt0, self.T0 = map(float, (cfl_max, cfl_wish, dt0, T0))
@@ -144,6 +146,8 @@ class Problem:
         self.rows_per_wave = int(rows_per_wave)
         # moving window: (md_vx, md_vy, md_t) or None (fixed domain)
         self.moving = None if moving is None else tuple(float(v) for v in moving)
+        # BC_HIFU: dict(ampl=AMPL, frec=FREC, a=ISPa, f=ISPF)
+        self.hifu = hifu
         self.q0 = np.ascontiguousarray(q0, np.float32)
         assert self.q0.shape == (3, self.ny, self.nx), self.q0.shape
 
@@ -164,6 +168,9 @@ class Problem:
         if self.moving is not None:
             p.md_on = 1
             p.md_vx, p.md_vy, p.md_t = self.moving
+        if self.hifu:
+            h = self.hifu
+            p.hifu_ampl, p.hifu_frec, p.hifu_a, p.hifu_f = h["ampl"], h["frec"], h["a"], h["f"]
         return p
 
 

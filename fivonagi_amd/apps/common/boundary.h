@@ -12,6 +12,9 @@
 //                       p = amp_j * sin(omega*T + phase_j); other edges zero-gradient
 //   Taylor shock ...... apps/taylor-angle-xy/boundary.h:38-55: every ghost cell
 //                       holds the closed form rho' = rhoa*tanh(c*(x cos + y sin - T))
+//   HIFU app .......... apps/hifu-beam/boundary.h:20-133: focused packet on the
+//                       aperture rows of the left ghost columns, mirrored beam
+//                       axis (rows 0, 1 <- 4, 3 with rho v negated), zero corners
 #ifndef FIVONAGI_APPS_COMMON_BOUNDARY_H
 #define FIVONAGI_APPS_COMMON_BOUNDARY_H
 
@@ -55,6 +58,22 @@ __device__ __forceinline__ void drive(float ampc, float amps, float sin_t, float
   q1 = p;
   q2 = p * (p + 1.0f);
   q3 = 0.0f;
+}
+
+// HIFU focused packet (apps/hifu-beam/boundary.h:10-17, 84-93):
+// pack = AMPL sin(ISPW (T + phi)) exp(-opT^10), opT = FREC (T + phi)/4, with
+// AMPL sin(ISPW (T + phi)) = ampc sin(ISPW T) + amps cos(ISPW T) from the
+// per-row ampc = AMPL cos(ISPW phi), amps = AMPL sin(ISPW phi);
+// u1 = pack, (u2, u3) = nu pack (pack + 1).
+__device__ __forceinline__ void hifu_packet(float ampc, float amps, float sin_t, float cos_t,
+                                            float opT, float nux, float nuy, float& q1,
+                                            float& q2, float& q3) {
+  const float o2 = opT * opT, o4 = o2 * o2, o8 = o4 * o4;
+  const float p = fmaf(ampc, sin_t, amps * cos_t) * expf(-(o8 * o2));
+  const float pp = p * (p + 1.0f);
+  q1 = p;
+  q2 = nux * pp;
+  q3 = nuy * pp;
 }
 
 __device__ __forceinline__ void taylor(double rhoa, double c, double cs, double sn,

@@ -91,6 +91,7 @@ struct fivonagi_ctx {
   std::vector<fv::WorkItem> items;
   std::vector<uint64_t> rowmask;           // CFL lane mask per item row (fv::WorkItem::mrow)
   bool md_any = false;                     // some instance has a moving window
+  bool ext_any = false;                    // ... or the HIFU boundary (extended kernel)
   float skew = -1.0f;                      // height skew of the work items (< 0: automatic)
   float skew_used = 0.0f;                  // the skew of the current work list
   bool rotate = false;                     // level pattern rotated per strip (chunk_bounds)
@@ -362,7 +363,7 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
     if (e != hipSuccess) return e;
   }
   return fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(), phase,
-                        c->n_inst, c->d_ctl, c->d_sync, c->md_any ? 1 : 0, c->stream);
+                        c->n_inst, c->d_ctl, c->d_sync, c->ext_any ? 1 : 0, c->stream);
 }
 
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
@@ -438,7 +439,10 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     if ((long)q.nx * q.ny > (1L << 30)) return fail(FIVONAGI_ERR_ARG, "grid too large");
     if (!(q.dx > 0) || !(q.dy > 0) || !(q.dt0 > 0) || !(q.cfl_wish > 0))
       return fail(FIVONAGI_ERR_ARG, "dx, dy, dt0, cfl_wish must be > 0");
-    if (q.bc < 0 || q.bc > 2) return fail(FIVONAGI_ERR_ARG, "unknown boundary kind");
+    if (q.bc < 0 || q.bc > 3) return fail(FIVONAGI_ERR_ARG, "unknown boundary kind");
+    if (q.bc == FIVONAGI_BC_HIFU &&
+        !(q.hifu_f > 0 && q.hifu_frec > 0 && q.hifu_a >= 0 && std::isfinite(q.hifu_ampl)))
+      return fail(FIVONAGI_ERR_ARG, "hifu: hifu_f, hifu_frec must be > 0, hifu_a >= 0");
     if (q.cfl_mode < 0 || q.cfl_mode > 1) return fail(FIVONAGI_ERR_ARG, "unknown cfl mode");
     if (!q_host[3 * k] || !q_host[3 * k + 1] || !q_host[3 * k + 2])
       return fail(FIVONAGI_ERR_ARG, "null state pointer");
@@ -482,13 +486,44 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
       e = hipMemcpy(d.q[1][f], q_host[3 * k + f], sizeof(float) * n, hipMemcpyHostToDevice);
       if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy state"); }
     }
-    // edge drive amplitude/phase folded into amp*cos(phase), amp*sin(phase)
-    std::vector<float> ec(q.ny, 0.f), es(q.ny, 0.f);
+    // edge drive amplitude/phase folded into amp*cos(phase), amp*sin(phase);
+    // the HIFU aperture's AMPL and phase delay ISPW*phi_j the same way
+    std::vector<float> ec(q.ny, 0.f), es(q.ny, 0.f), tab;
+    const bool hifu = q.bc == FIVONAGI_BC_HIFU;
+    const double ispw = 2.0 * 3.14159265358979323846 * q.hifu_frec;   // ISPW
+    if (hifu) tab.assign(4 * (size_t)q.ny, 0.f);
     for (int j = 0; j < q.ny; ++j) {
+      if (hifu) {
+        // apps/hifu-beam/boundary.h:79-90
+        const int width = (int)(q.hifu_a / q.dy);
+        const int jp = j - 2;
+        if (jp >= 0 && jp < width) {
+          const double F = q.hifu_f, jp2 = (double)jp * jp;
+          const double nd = std::sqrt(jp2 + F * F);
+          const double phi = (q.dx * jp2) / (2.0 * F);
+          ec[j] = (float)(q.hifu_ampl * std::cos(ispw * phi));
+          es[j] = (float)(q.hifu_ampl * std::sin(ispw * phi));
+          tab[4 * (size_t)j + 0] = (float)phi;
+          tab[4 * (size_t)j + 1] = (float)(F / nd);
+          tab[4 * (size_t)j + 2] = (float)(-jp / nd);
+          tab[4 * (size_t)j + 3] = 1.f;
+        }
+        continue;
+      }
       const double a = q.edge_amp ? q.edge_amp[j] : 0.0;
       const double ph = q.edge_phase ? q.edge_phase[j] : 0.0;
       ec[j] = (float)(a * std::cos(ph));
       es[j] = (float)(a * std::sin(ph));
+    }
+    d.hifu_tab = nullptr;
+    d.hifu_k = (float)(q.hifu_frec / 4.0);
+    if (hifu) {
+      float* t = (float*)c->dalloc(sizeof(float) * tab.size(), &e);
+      if (e == hipSuccess)
+        e = hipMemcpy(t, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) { delete c; return hip_fail(e, "hifu table"); }
+      d.hifu_tab = t;
+      c->ext_any = true;
     }
     float* edge = (float*)c->dalloc(sizeof(float) * 2 * q.ny, &e);
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc edge"); }
@@ -522,7 +557,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     d.cfl_max = q.cfl_max; d.cfl_wish = q.cfl_wish; d.omega = q.omega;
     d.inv_dx = 1.0 / q.dx; d.inv_dy = 1.0 / q.dy;
     d.dif_x = q.delta / (q.dx * q.dx); d.dif_y = q.delta / (q.dy * q.dy);
-    d.omega_turns = q.omega / 6.283185307179586476925286766559;
+    d.omega_turns = (q.bc == FIVONAGI_BC_HIFU ? ispw : q.omega) / 6.283185307179586476925286766559;
     d.tay_rhoa = q.tay_rhoa; d.tay_c = q.tay_c;
     d.tay_cos = std::cos(q.tay_theta); d.tay_sin = std::sin(q.tay_theta);
     d.tay_yoff = q.tay_yoff; d.tay_jc = q.tay_jc;
@@ -532,7 +567,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     d.beta2m2 = (float)(2.0 * (q.beta - 1.0));
     d.md_on = q.md_on;
     d.md_vx = q.md_vx; d.md_vy = q.md_vy; d.md_t = q.md_t;
-    if (q.md_on) c->md_any = true;
+    if (q.md_on) c->md_any = c->ext_any = true;
 
     // attempt 0 starts from block 0; slot set 2 (attempt -1) never ran
     auto& t = ctl[2 * (size_t)k];
@@ -840,6 +875,9 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   if (!c || !obj || !J || nsteps <= 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
   if (nsteps > (1 << 30)) return fail(FIVONAGI_ERR_ARG, "too many steps");
   if (c->md_any) return fail(FIVONAGI_ERR_ARG, "the adjoint does not support a moving window");
+  for (const auto& p : c->params)
+    if (p.bc == FIVONAGI_BC_HIFU)
+      return fail(FIVONAGI_ERR_ARG, "the adjoint does not support the HIFU boundary");
   const int ni = c->n_inst;
   for (int k = 0; k < ni; ++k) {
     const auto& p = c->params[k];

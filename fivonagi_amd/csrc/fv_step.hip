@@ -262,13 +262,17 @@ struct XY {          // one cell of q_xy, with its velocities for the source
 // block of attempt a = ctl + a % 2, sync_in = the slots of attempt a-1 =
 // sync + (a + 2) % 3), so the fold's loads go out together with the work
 // item's instead of after the instance description's.
-// kMd: some instance has a moving window.  The shift that follows an accepted
-// step (moveDomain, common/main.cu:109-164) is folded into the stores: cell
-// (i, j) of the new state lands at (i - sx, j - sy), and also stays at (i, j)
-// when (i + sx, j + sy) is off the grid (moveDomain's else branch); every
-// cell of the output is written exactly once.  A rejected attempt's output
-// is discarded, as without the window.
-template <bool kOne, bool kMd>
+// kExt: some instance has a moving window or the HIFU boundary.
+//  - Moving window: the shift that follows an accepted step (moveDomain,
+//    common/main.cu:109-164) is folded into the stores: cell (i, j) of the new
+//    state lands at (i - sx, j - sy), and also stays at (i, j) when
+//    (i + sx, j + sy) is off the grid (moveDomain's else branch); every cell
+//    of the output is written exactly once.  A rejected attempt's output is
+//    discarded, as without the window.
+//  - HIFU boundary (apps/hifu-beam/boundary.h:20-133): rows 0, 1 read rows
+//    4, 3 with rho v negated, the 2x2 corners are 0, and the aperture rows of
+//    the left ghost columns take the focused packet while MDX == 0.
+template <bool kOne, bool kExt>
 __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
                                                      const WorkItem* __restrict__ items,
                                                      const uint64_t* __restrict__ rowmask,
@@ -345,28 +349,44 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   const int colB = taylor ? ic : (left ? 2 : ic);
   const uint32_t bA = 4u * (uint32_t)colA, bB = 4u * (uint32_t)colB;
   const uint32_t row_bytes = 4u * (uint32_t)nx;
+  const bool hifu = kExt && bc == kBcHifu;
+  kfloat* __restrict__ htab = (kfloat*)D.hifu_tab;
+  struct HRow { float ph, nux, nuy, on; };   // aperture row data (hifu)
 
   // raw (unpatched) load of row r, with the row's drive coefficients; the
   // boundary patch is applied when the row is consumed one iteration later,
   // so the load latency overlaps a full step
-  auto load_raw = [&](int r, float& a, float& b, float& c, float& ec, float& es) {
+  auto load_raw = [&](int r, float& a, float& b, float& c, float& ec, float& es, HRow& h) {
     const int rc = min(max(r, 0), ny - 1);
     const bool grow = (rc <= 1) || (rc >= ny - 2);
-    const int jm = taylor ? rc : (rc >= ny - 2 ? ny - 3 : (rc <= 1 ? 2 : rc));
-    const uint32_t rowL = (uint32_t)rc * row_bytes, rowM = (uint32_t)jm * row_bytes;
     uint32_t o;
-    if (grow) o = (left ? rowL : rowM) + bB;   // ghost rows
-    else o = rowM + bA;                        // rows 2..ny-3: rc == jm
+    if (hifu) {
+      // bottom rows mirror rows 4, 3; top rows copy ny-3; columns as colA
+      const int jm = rc <= 1 ? 4 - rc : (rc >= ny - 2 ? ny - 3 : rc);
+      o = (uint32_t)jm * row_bytes + bA;
+    } else {
+      const int jm = taylor ? rc : (rc >= ny - 2 ? ny - 3 : (rc <= 1 ? 2 : rc));
+      const uint32_t rowL = (uint32_t)rc * row_bytes, rowM = (uint32_t)jm * row_bytes;
+      if (grow) o = (left ? rowL : rowM) + bB;   // ghost rows
+      else o = rowM + bA;                        // rows 2..ny-3: rc == jm
+    }
     a = ldg(in1, o);
     b = ldg(in2, o);
     c = ldg(in3, o);
-    if (bc == kBcEdgeDrive) {
+    if (bc == kBcEdgeDrive || hifu) {
       ec = edge_c[rc];
       es = edge_s[rc];
     }
+    if (hifu) {
+      h.ph = htab[4 * rc];
+      h.nux = htab[4 * rc + 1];
+      h.nuy = htab[4 * rc + 2];
+      h.on = htab[4 * rc + 3];
+    }
   };
   float n1, n2, n3, nec = 0.0f, nes = 0.0f;
-  load_raw(it.y0 - kHalo, n1, n2, n3, nec, nes);   // in flight during advance()
+  HRow nh{0.0f, 0.0f, 0.0f, 0.0f};
+  load_raw(it.y0 - kHalo, n1, n2, n3, nec, nes, nh);   // in flight during advance()
 
   // ---- control block of this attempt ----
   const Ctl P = advance(D, F, local == 0);
@@ -390,22 +410,43 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // the edge source runs while the window has not moved in x
   // (apps/hifu-beam/boundary.h:82); after that the left ghost columns are
   // plain zero-gradient copies
-  const bool drive = (bc == kBcEdgeDrive) && (!kMd || P.mdx == 0);
+  const bool drive = (bc == kBcEdgeDrive) && (!kExt || P.mdx == 0);
   const bool drive_lane = left && drive;
   // shift of the window after this attempt, if it is accepted
   int sx = 0, sy = 0;
-  if (kMd && D.md_on) {
+  if (kExt && D.md_on) {
     const double Tn = P.T + P.dt;
     sx = uni(md_shift(Tn, D.md_t, D.md_vx, D.dx, P.mdx));
     sy = uni(md_shift(Tn, D.md_t, D.md_vy, D.dy, P.mdy));
   }
-  auto patch = [&](Row& q, float ec, float es) {
+  // HIFU: the packet runs while the window has not moved (boundary.h:82)
+  const bool hdrive = hifu && P.mdx == 0 && it.x0 == 0;
+  const double Td = P.T;
+  const float hk = D.hifu_k;
+  auto patch = [&](Row& q, float ec, float es, const HRow& h, int r) {
     if (drive) {
       float a, b, c;
       fvbc::drive(ec, es, sin_t, cos_t, a, b, c);
       q.q1 = drive_lane ? a : q.q1;
       q.q2 = drive_lane ? b : q.q2;
       q.q3 = drive_lane ? c : q.q3;
+    }
+    if (hifu) {
+      const int rc = min(max(r, 0), ny - 1);
+      q.q3 = rc <= 1 ? -q.q3 : q.q3;          // reflective beam axis
+      if (hdrive && h.on != 0.0f && rc >= 2 && rc <= ny - 3) {
+        float a, b, c;
+        // T + phi in FP64: the envelope's 10th power magnifies its rounding
+        fvbc::hifu_packet(ec, es, sin_t, cos_t, hk * (float)(Td + (double)h.ph), h.nux, h.nuy,
+                          a, b, c);
+        q.q1 = left ? a : q.q1;
+        q.q2 = left ? b : q.q2;
+        q.q3 = left ? c : q.q3;
+      }
+      const bool corner = (rc <= 1 || rc >= ny - 2) && (ic <= 1 || ic >= nx - 2);
+      q.q1 = corner ? 0.0f : q.q1;
+      q.q2 = corner ? 0.0f : q.q2;
+      q.q3 = corner ? 0.0f : q.q3;
     }
     const float ir = fvna::rcp(1.0f + q.q1);
     q.un = q.q3 * ir;
@@ -426,7 +467,7 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     r3 = s_int ? r3 : mid.q3;
     pin(r2);
     pin(r3);
-    if (!kMd) {
+    if (!kExt) {
       if (out_lane) {
         const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
         stg(o1, o, mid.q1);            // q1 passes the source unchanged
@@ -468,9 +509,9 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     // registers from landing after this step's stores
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     bn.q1 = n1; bn.q2 = n2; bn.q3 = n3;
-    patch(bn, nec, nes);
+    patch(bn, nec, nes, nh, r);
     if (r >= y0 + 4) source_store(r - 4, xn, xo, xm);
-    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3, nec, nes);
+    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3, nec, nes, nh);
     if (r < y0 - 2) return;
     const int m = r - 2;
     // CFL lane masks of rows r (y interfaces) and m (x interfaces); the index

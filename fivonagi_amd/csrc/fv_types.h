@@ -13,7 +13,7 @@ constexpr int kLanes = 64;
 constexpr int kHalo = 3;
 constexpr int kStripCols = kLanes - 2 * kHalo;   // 58
 
-enum BoundaryKind : int32_t { kBcZeroGrad = 0, kBcEdgeDrive = 1, kBcTaylor = 2 };
+enum BoundaryKind : int32_t { kBcZeroGrad = 0, kBcEdgeDrive = 1, kBcTaylor = 2, kBcHifu = 3 };
 enum CflMask : int32_t { kCflAll = 0, kCflEven = 1, kCflBitmap = 2 };
 
 // Per-instance control block (device memory): the parameters of one step
@@ -80,6 +80,11 @@ struct InstDesc {
   float beta2m2;        // 2*(BETA - 1)
   float pad;
   double md_vx, md_vy, md_t;   // MDVX, MDVY, MDT
+  // kBcHifu: edge_c/edge_s hold AMPL cos/sin(ISPW phi_j); per row j,
+  // hifu_tab[4j..4j+3] = phi_j, nu_x, nu_y, 1 on aperture rows (0 elsewhere)
+  const float* hifu_tab;
+  float hifu_k;         // FREC/4: envelope argument per unit of T + phi
+  float pad4;
 };
 
 // One wave's work: a 58-column strip of one instance, rows [y0, y1).

@@ -17,7 +17,7 @@ import math
 
 import numpy as np
 
-from . import BC_EDGE_DRIVE, BC_TAYLOR, BC_ZEROGRAD, CFL_REFERENCE_FOLD, Problem
+from . import BC_EDGE_DRIVE, BC_HIFU, BC_TAYLOR, BC_ZEROGRAD, CFL_REFERENCE_FOLD, Problem
 
 CELLS_PER_UNIT = 20.0
 
@@ -127,6 +127,30 @@ def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_
     md = (math.cos(theta), math.sin(theta), mdt) if moving else None
     return Problem(nx, ny, dx, dy, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=0.005,
                    T0=T0, bc=BC_TAYLOR, cfl_mode=cfl_mode, taylor=t, moving=md)
+
+
+def hifu(etac=1, ispam=30.0, cflw=0.9, nx=None, ny=None, moving=True,
+         cfl_mode=CFL_REFERENCE_FOLD, rows_per_wave=0):
+    """apps/hifu-beam as configured by its parameters.h: NX = 512*ETAC,
+    NY = 1024*ETAC, XMAX = 25, YMAX = 50 (wavelengths), BETA = 4.8,
+    DIFFDELTA = 2.974e-4, AMPL = 4.217e-4, FREC = 1, aperture ISPa = ISPAM
+    wavelengths, focal distance ISPF = 50 wavelengths (in points), quiescent
+    start at T = -TIDE, moving window MDVX = 1, MDVY = 0, MDT = XMAX/2.
+    nx, ny override the grid (same physical size)."""
+    nx = 512 * etac if nx is None else nx
+    ny = 1024 * etac if ny is None else ny
+    xmax, ymax, frec = 25.0, 50.0, 1.0
+    dx, dy = xmax / (nx - 1), ymax / (ny - 1)
+    lam = 2.0 * math.pi / (2.0 * math.pi * frec)           # ISPlambda = 1
+    a = ispam * lam
+    f = 50.0 * lam / dx
+    tide = math.floor(2 + 4 / frec + (a * a * dx) / (2.0 * f * dy * dy))
+    q = np.zeros((3, ny, nx))
+    h = dict(ampl=4.217e-4, frec=frec, a=a, f=f)
+    md = (1.0, 0.0, xmax * 0.5) if moving else None
+    return Problem(nx, ny, dx, dy, 4.8, 2.974e-4, q, cfl_max=1.0, cfl_wish=cflw, dt0=0.005,
+                   T0=-float(tide), bc=BC_HIFU, cfl_mode=cfl_mode, rows_per_wave=rows_per_wave,
+                   moving=md, hifu=h)
 
 
 def aperture_params(ny, n_rows, rng, amp=0.02, halfwidth_rows=None, noise=0.05):

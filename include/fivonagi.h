@@ -35,7 +35,8 @@ typedef struct fivonagi_ctx fivonagi_ctx;
 enum {
   FIVONAGI_BC_ZEROGRAD = 0,   /* common/boundary.h:11-82                        */
   FIVONAGI_BC_EDGE_DRIVE = 1, /* apps/hifu-beam/boundary.h:77-93, nu = (1,0)    */
-  FIVONAGI_BC_TAYLOR = 2      /* apps/taylor-angle-xy/boundary.h:38-55          */
+  FIVONAGI_BC_TAYLOR = 2,     /* apps/taylor-angle-xy/boundary.h:38-55          */
+  FIVONAGI_BC_HIFU = 3        /* apps/hifu-beam/boundary.h:9-133 (full app)     */
 };
 enum {
   FIVONAGI_CFL_REFERENCE_FOLD = 0, /* reduceCFL schedule, common/main.cu:503-524 */
@@ -69,6 +70,15 @@ typedef struct {
      closed form is taken at (i + MDX, j + MDY). */
   int32_t md_on;           /* 0 = fixed domain                                */
   double md_vx, md_vy, md_t; /* MDVX, MDVY, MDT                               */
+  /* FIVONAGI_BC_HIFU (apps/hifu-beam/boundary.h:9-133): focused aperture on
+     the left ghost columns, rows 2 <= j < 2 + int(hifu_a/dy) while MDX == 0:
+     phi_j = dx (j-2)^2 / (2 hifu_f), pack = hifu_ampl sin(2 pi hifu_frec (T+phi_j))
+     exp(-(hifu_frec (T+phi_j)/4)^10), direction nu_j = (F, -(j-2))/|.|;
+     rows 0,1 mirror rows 4,3 with rho v negated (beam axis); corners 0;
+     right/top edges and the other left ghosts zero-gradient.  omega and the
+     edge_* arrays are not used. */
+  double hifu_ampl, hifu_frec; /* AMPL, FREC                                  */
+  double hifu_a, hifu_f;   /* ISPa (length units), ISPF (points)              */
 } fivonagi_params;
 
 typedef struct {

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-enum { ORC_BC_ZEROGRAD = 0, ORC_BC_EDGE_DRIVE = 1, ORC_BC_TAYLOR = 2 };
+enum { ORC_BC_ZEROGRAD = 0, ORC_BC_EDGE_DRIVE = 1, ORC_BC_TAYLOR = 2, ORC_BC_HIFU = 3 };
 enum { ORC_CFL_REFERENCE_FOLD = 0, ORC_CFL_ALL = 1 };
 enum { ORC_FORM_LITERAL = 0, ORC_FORM_RHO1 = 1 };
 enum { ORC_OK = 0, ORC_ERR_DT = 1, ORC_ERR_EIGEN = 2, ORC_ERR_ARG = 8 };
@@ -71,6 +71,15 @@ typedef struct {
   int32_t stop_at_T;
   double md_vx, md_vy, md_t;
   double T_stop;
+  /* ORC_BC_HIFU: apps/hifu-beam/boundary.h:9-133 -- focused aperture on the
+     left ghost columns (rows 2 <= j < 2 + int(hifu_a/dy), while MDX == 0):
+     phi_j = dx*(j-2)^2/(2F), pack = AMPL sin(2 pi FREC (T+phi_j))
+     exp(-(FREC (T+phi_j)/4)^10), nu_j = (F, -(j-2))/sqrt((j-2)^2 + F^2),
+     u1 = pack, (u2, u3) = nu_j pack (pack+1); other left ghosts copy column 2;
+     right and top ghosts zero-gradient; rows 0,1 mirror rows 4,3 with u3
+     negated (the beam axis); the 2x2 corners are 0.
+     hifu_ampl = AMPL, hifu_frec = FREC, hifu_a = ISPa, hifu_f = ISPF (points). */
+  double hifu_ampl, hifu_frec, hifu_a, hifu_f;
 } orc_params;
 
 typedef struct {

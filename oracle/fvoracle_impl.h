@@ -167,6 +167,44 @@ static void SFX(boundary)(const orc_params *p, REAL *const src[3],
         }
         continue;
       }
+      if (p->bc == ORC_BC_HIFU) {
+        /* apps/hifu-beam/boundary.h:20-133, rules in the reference's order
+           (later ones override earlier ones) */
+        int refl = 0, drive = 0;
+        if (i == NX - 1 || i == NX - 2) { ir = NX - 3; jr = j; }
+        if (j == NY - 1 || j == NY - 2) { ir = i; jr = NY - 3; }
+        if (i == 0 || i == 1) {
+          const int width = (int)(p->hifu_a / p->dy);
+          const int jprime = j - 2;
+          if (abs(jprime) < width && jprime >= 0 && mdx == 0) drive = 1;
+          else { ir = 2; jr = j; }
+        }
+        if (j == 0 || j == 1) { ir = i; jr = 4 - j; refl = 1; drive = 0; }
+        if ((i <= 1 && j <= 1) || (i <= 1 && j >= NY - 2) || (i >= NX - 2 && j <= 1) ||
+            (i >= NX - 2 && j >= NY - 2)) {
+          for (int f = 0; f < 3; ++f) dst[f][o] = (REAL)0;
+          continue;
+        }
+        if (drive) {
+          const double F = p->hifu_f, jp = (double)(j - 2), jp2 = jp * jp;
+          const double nudenom = sqrt(jp2 + F * F);
+          const double nux = F / nudenom, nuy = -jp / nudenom;
+          const double phi = (p->dx * jp2) / (2.0 * F);
+          const double opT = 2.0 * p->hifu_frec * (T + phi) / 8.0;
+          const double o2 = opT * opT, o4 = o2 * o2, o8 = o4 * o4;
+          const double ispw = 2.0 * 3.14159265358979323846 * p->hifu_frec;   /* ISPW */
+          const double pack = p->hifu_ampl * sin(ispw * (T + phi)) * exp(-(o8 * o2));
+          dst[0][o] = (REAL)pack;
+          dst[1][o] = (REAL)(nux * pack * (pack + 1.0));
+          dst[2][o] = (REAL)(nuy * pack * (pack + 1.0));
+          continue;
+        }
+        size_t orr = (size_t)ir + (size_t)jr * NX;
+        dst[0][o] = src[0][orr];
+        dst[1][o] = src[1][orr];
+        dst[2][o] = refl ? -src[2][orr] : src[2][orr];
+        continue;
+      }
       /* common/boundary.h:27-81: later rules override earlier ones */
       if (i == NX - 1 || i == NX - 2) { ir = NX - 3; jr = j; }
       if (j == 0 || j == 1) { ir = i; jr = 2; }

@@ -11,7 +11,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
-BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR = 0, 1, 2
+BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR, BC_HIFU = 0, 1, 2, 3
 CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
 FORM_LITERAL, FORM_RHO1 = 0, 1
 
@@ -34,6 +34,8 @@ class Params(C.Structure):
         ("md_on", C.c_int32), ("stop_at_T", C.c_int32),
         ("md_vx", C.c_double), ("md_vy", C.c_double), ("md_t", C.c_double),
         ("T_stop", C.c_double),
+        ("hifu_ampl", C.c_double), ("hifu_frec", C.c_double),
+        ("hifu_a", C.c_double), ("hifu_f", C.c_double),
     ]
 
 
@@ -106,7 +108,7 @@ class Problem:
     def __init__(self, nx, ny, dx, dy, beta, delta, cfl_max=1.0, cfl_wish=0.9,
                  dt0=0.005, T0=0.0, bc=BC_ZEROGRAD, cfl_mode=CFL_REFERENCE_FOLD,
                  form=FORM_LITERAL, omega=0.0, edge_amp=None, edge_phase=None,
-                 taylor=None, dt_seq=None, nthreads=0, moving=None, T_stop=None):
+                 taylor=None, dt_seq=None, nthreads=0, moving=None, T_stop=None, hifu=None):
         self.nx, self.ny = int(nx), int(ny)
         self.dx, self.dy = float(dx), float(dy)
         self.beta, self.delta = float(beta), float(delta)
@@ -121,6 +123,7 @@ class Problem:
         self.nthreads = int(nthreads)
         self.moving = None if moving is None else tuple(float(v) for v in moving)
         self.T_stop = None if T_stop is None else float(T_stop)
+        self.hifu = hifu  # dict(ampl, frec, a, f)
 
     def cparams(self):
         p = Params()
@@ -137,6 +140,9 @@ class Problem:
             t = self.taylor
             p.tay_rhoa, p.tay_c = t["rhoa"], t["c"]
             p.tay_theta, p.tay_yoff, p.tay_jc = t["theta"], t["yoff"], int(t["jc"])
+        if self.hifu:
+            h = self.hifu
+            p.hifu_ampl, p.hifu_frec, p.hifu_a, p.hifu_f = h["ampl"], h["frec"], h["a"], h["f"]
         if self.dt_seq is not None:
             p.dt_seq = _ptr(self.dt_seq, C.c_double)
             p.dt_seq_len = len(self.dt_seq)

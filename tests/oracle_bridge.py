@@ -8,4 +8,4 @@ def to_oracle(p, form=po.FORM_LITERAL, dt_seq=None, nthreads=0, T_stop=None):
                       p.T0, bc=p.bc, cfl_mode=p.cfl_mode, form=form, omega=p.omega,
                       edge_amp=p.edge_amp, edge_phase=p.edge_phase, taylor=p.taylor,
                       dt_seq=dt_seq, nthreads=nthreads, moving=getattr(p, "moving", None),
-                      T_stop=T_stop)
+                      T_stop=T_stop, hifu=getattr(p, "hifu", None))

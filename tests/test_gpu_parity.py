@@ -181,6 +181,25 @@ def test_moving_window_matches_oracle(oracle, case):
         assert max(e) < TOL
 
 
+@pytest.mark.parametrize("moving", [False, True])
+def test_hifu_app_matches_oracle(oracle, moving):
+    """The hifu-beam app's boundary (apps/hifu-beam/boundary.h:20-133: focused
+    packet, mirrored beam axis, zero corners) on a 128x256 grid of the app's
+    physical size, ISPa = 10 wavelengths; 150 steps take T from -TIDE past
+    MDT = 12.5, so with the window the source is switched off mid-run."""
+    prob = wl.hifu(nx=128, ny=256, ispam=10.0, moving=moving)
+    n = 150
+    q, st = run_gpu(prob, n)
+    qo, sto = run_oracle(oracle, prob, n)
+    assert (st["md_x"], st["md_y"]) == (sto.mdx, sto.mdy)
+    assert (sto.mdx > 3) == moving
+    assert st["n_accepted"] == sto.n_accepted and st["n_rejected"] == sto.n_rejected
+    assert np.abs(qo[0]).max() > 1e-5          # the beam is in the grid
+    e = rel_linf(q, qo)
+    report(f"hifu 128x256 moving={moving} (MDX={sto.mdx})", e)
+    assert max(e) < TOL
+
+
 def test_moving_window_in_a_batch(oracle):
     """A batch where one instance moves and one does not (the multi-instance
     form of the moving-window kernel)."""

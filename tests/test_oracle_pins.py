@@ -196,6 +196,66 @@ def test_edge_drive_values(oracle):
     assert out[0][5, nx - 1] == (nx - 3) + 1000.0 * 5
 
 
+def test_hifu_index_map(oracle):
+    # apps/hifu-beam/boundary.h:20-133: right/top zero-gradient, left ghost
+    # copies column 2 outside the aperture, rows 0,1 mirror rows 4,3 with rho v
+    # negated (beam axis), the 2x2 corners are 0
+    nx, ny = 14, 12
+    p = wl.hifu(nx=nx, ny=ny, ispam=0.0)          # no aperture rows
+    out, rc = oracle.boundary(to_oracle(p), ids(nx, ny), 0.0)
+    assert rc == 0
+    table = {
+        (5, 5): (5, 5),                            # interior untouched
+        (0, 5): (2, 5), (1, 7): (2, 7),            # left, outside the aperture
+        (nx - 1, 5): (nx - 3, 5), (nx - 2, 6): (nx - 3, 6),   # right
+        (5, ny - 1): (5, ny - 3), (6, ny - 2): (6, ny - 3),   # top
+        (5, 0): (5, 4), (6, 1): (6, 3),            # mirrored beam axis
+        (2, 0): (2, 4), (nx - 3, 1): (nx - 3, 3),
+    }
+    for (i, j), (si, sj) in table.items():
+        assert out[0][j, i] == si + 1000.0 * sj, (i, j)
+        assert out[1][j, i] == -(si + 1000.0 * sj), (i, j)
+        sign = -1.0 if j <= 1 else 1.0
+        assert out[2][j, i] == sign * 0.5 * (si + 1000.0 * sj), (i, j)
+    for (i, j) in [(0, 0), (1, 1), (0, ny - 1), (1, ny - 2), (nx - 1, 0), (nx - 2, 1),
+                   (nx - 1, ny - 1), (nx - 2, ny - 2)]:
+        assert out[0][j, i] == out[1][j, i] == out[2][j, i] == 0.0, (i, j)
+
+
+def test_hifu_packet_special_values(oracle):
+    # packet(T, phi) = AMPL sin(ISPW (T + phi)) exp(-(FREC (T+phi)/4)^10) on
+    # rows 2 <= j < 2 + int(ISPa/dy); nu = (F, -(j-2))/sqrt((j-2)^2 + F^2),
+    # phi = dx (j-2)^2 / (2F) (apps/hifu-beam/boundary.h:10-17, 79-93)
+    nx, ny = 16, 64
+    p = wl.hifu(nx=nx, ny=ny, ispam=10.0)
+    h = p.hifu
+    width = int(h["a"] / p.dy)
+    assert 2 + width < ny - 2
+    # T = 1/4 (FREC = 1): row 2 (phi = 0) sits at the crest, sin = 1,
+    # envelope exp(-(1/16)^10), direction (1, 0)
+    out, rc = oracle.boundary(to_oracle(p), np.zeros((3, ny, nx)), 0.25)
+    pk = h["ampl"] * math.exp(-(1.0 / 16.0) ** 10)
+    for i in (0, 1):
+        assert out[0][2, i] == pytest.approx(pk, rel=1e-15)
+        assert out[1][2, i] == pytest.approx(pk * (pk + 1), rel=1e-15)
+        assert out[2][2, i] == 0.0
+    # row j: the packet is 0 when T + phi_j = 0 (sin 0), whatever the row
+    for j in (3, 7, 2 + width - 1):
+        phi = p.dx * (j - 2) ** 2 / (2 * h["f"])
+        out, rc = oracle.boundary(to_oracle(p), np.zeros((3, ny, nx)), -phi)
+        assert abs(out[0][j, 0]) < 1e-18
+        assert abs(out[0][j - 1, 0]) > 1e-9       # the neighbour row is off its zero
+    # direction: a unit vector tilted towards the axis, rho v / rho u = -(j-2)/F
+    out, rc = oracle.boundary(to_oracle(p), np.zeros((3, ny, nx)), 0.3)
+    for j in (3, 4):                     # rows whose packet is inside its envelope
+        assert out[2][j, 1] / out[1][j, 1] == pytest.approx(-(j - 2) / h["f"], rel=1e-12)
+    # first row past the aperture: plain copy of column 2 (zero here)
+    assert out[0][2 + width, 0] == 0.0 and out[0][2 + width - 1, 0] != 0.0
+    # once the window has moved, no packet (boundary.h:82): with the window
+    # moving from the start, a run that has shifted equals one whose aperture
+    # is empty from the shift on (compared on the step after the first shift)
+
+
 def test_taylor_ghosts_closed_form(oracle):
     p = wl.taylor(1, theta=math.pi / 8)
     t = p.taylor
