@@ -14,14 +14,14 @@ LIB_PATH = os.path.join(_HERE, "libfivonagi.so")
 
 BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR, BC_HIFU = 0, 1, 2, 3
 CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
-OK, ERR_EIGEN, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 8, 9, 10
+OK, ERR_EIGEN, ERR_COLLECT, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 4, 8, 9, 10
 
 EXPORTS = (
     "fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence", "fivonagi_step",
     "fivonagi_step_until",
     "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect", "fivonagi_info",
     "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_set_skew",
-    "fivonagi_destroy", "fivonagi_last_error",
+    "fivonagi_set_collect", "fivonagi_measure", "fivonagi_destroy", "fivonagi_last_error",
 )
 
 
@@ -64,6 +64,11 @@ class Objective(C.Structure):
                 ("box_ny", C.c_int32)]
 
 
+class CollectOpts(C.Structure):
+    """fivonagi_collect_opts (include/fivonagi.h)."""
+    _fields_ = [("jcut", C.c_int32), ("mic", C.c_int32 * 4), ("eta", C.c_double)]
+
+
 class AdjointOpts(C.Structure):
     """fivonagi_adjoint_opts (include/fivonagi.h)."""
     _fields_ = [("checkpoint_every", C.c_int32), ("plain_forward", C.c_int32)]
@@ -101,6 +106,8 @@ def load():
     lib.fivonagi_adjoint_state.argtypes = [vp, C.c_int, fp, fp, fp]
     lib.fivonagi_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int32)]
     lib.fivonagi_set_skew.argtypes = [vp, C.c_double, C.c_int]
+    lib.fivonagi_set_collect.argtypes = [vp, C.c_int, C.POINTER(CollectOpts)]
+    lib.fivonagi_measure.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
     lib.fivonagi_last_error.argtypes = []
@@ -109,7 +116,8 @@ def load():
                  "fivonagi_step", "fivonagi_step_until", "fivonagi_enqueue", "fivonagi_sync",
                  "fivonagi_collect",
                  "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state",
-                 "fivonagi_autotune", "fivonagi_set_skew"):
+                 "fivonagi_autotune", "fivonagi_set_skew", "fivonagi_set_collect",
+                 "fivonagi_measure"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -289,6 +297,24 @@ class Context:
         _check(load().fivonagi_adjoint_state(
             self._h, inst, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in ls]))
         return np.stack(ls)
+
+    def set_collect(self, inst, jcut=2, mic=(0, 0, 0, 0), eta=1.0, on=True):
+        """Data collection on row jcut (apps/hifu-beam/data_collect.h)."""
+        if not on:
+            _check(load().fivonagi_set_collect(self._h, inst, None))
+            return
+        o = CollectOpts()
+        o.jcut, o.eta = int(jcut), float(eta)
+        for k in range(4):
+            o.mic[k] = int(mic[k])
+        _check(load().fivonagi_set_collect(self._h, inst, C.byref(o)))
+
+    def measure(self, inst=0):
+        """The measurement grid (ny, nx) of instance inst."""
+        p = self.problems[inst]
+        m = np.zeros((p.ny, p.nx), np.float32)
+        _check(load().fivonagi_measure(self._h, inst, m.ctypes.data_as(C.POINTER(C.c_float))))
+        return m
 
     def set_skew(self, skew, rotate=False):
         """Height skew of the work items (< 0: automatic) and the per-strip

@@ -27,6 +27,8 @@ extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkIt
                                      hipStream_t stream);
 extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, int phase,
                                        hipStream_t stream);
+extern "C" hipError_t fv_launch_collect(const fv::InstDesc* insts, int n_inst, int max_nx,
+                                        int phase, hipStream_t stream);
 extern "C" hipError_t fv_step_waves_per_cu(int* waves);
 extern "C" hipError_t fv_adj_march_forward(const fv::AdjDesc* ds, const fv::AdjItem* items,
                                            int n_items, int k, int slot, hipStream_t s);
@@ -109,6 +111,7 @@ struct fivonagi_ctx {
   int64_t attempts = 0;                    // step attempts launched (the phase is this mod 6)
   int graph_len = 0;
   int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
+  int collect_nx = 0;                      // widest instance with data collection (0: none)
   bool unlimited = false;                  // targets set to "run every attempt"
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -362,8 +365,22 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
         fv_launch_taylor_ghosts(c->d_desc, c->n_inst, c->taylor_ring, phase, c->stream);
     if (e != hipSuccess) return e;
   }
-  return fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(), phase,
-                        c->n_inst, c->d_ctl, c->d_sync, c->ext_any ? 1 : 0, c->stream);
+  hipError_t e = fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(),
+                                phase, c->n_inst, c->d_ctl, c->d_sync, c->ext_any ? 1 : 0,
+                                c->stream);
+  if (e != hipSuccess || c->collect_nx == 0) return e;
+  return fv_launch_collect(c->d_desc, c->n_inst, c->collect_nx, phase, c->stream);
+}
+
+// data collection overflow flags (any instance): FIVONAGI_ERR_COLLECT or 0
+int collect_status(fivonagi_ctx* c) {
+  for (int k = 0; k < c->n_inst; ++k) {
+    if (!c->desc[k].collect_err || !c->desc[k].measure) continue;
+    uint32_t v = 0;
+    HIP_TRY(hipMemcpy(&v, c->desc[k].collect_err, sizeof(v), hipMemcpyDeviceToHost));
+    if (v) return fail(FIVONAGI_ERR_COLLECT, "data collection: more than nx steps in a frame");
+  }
+  return 0;
 }
 
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
@@ -780,6 +797,10 @@ int fivonagi_step_until(fivonagi_ctx* c, double T_stop) {
     if (ctl[k].err) return fivonagi_sync(c);
   if (int rc = write_targets(c, std::vector<int64_t>(c->n_inst, INT64_MAX), T_stop)) return rc;
   c->unlimited = false;
+  if (c->collect_nx > 0) {   // a new frame of the data collection
+    for (int k = 0; k < c->n_inst; ++k) c->desc[k].n0frame = ctl[k].n_acc;
+    if (int rc = upload_tables(c)) return rc;
+  }
   // rounds of attempts sized by the slowest instance's remaining time over
   // its dt (+10%); an instance past T_stop skips the extra attempts
   for (int round = 0;; ++round) {
@@ -794,8 +815,51 @@ int fivonagi_step_until(fivonagi_ctx* c, double T_stop) {
     if (round > 10000) return fail(FIVONAGI_ERR_STALL, "T does not reach T_stop");
     if (int rc = launch_attempts(c, std::min<int64_t>(need, 1 << 20))) return rc;
     if (int rc = read_ctls(c, ctl)) return rc;
+    if (c->collect_nx > 0)
+      if (int rc = collect_status(c)) return rc;
   }
   return 0;
+}
+
+int fivonagi_set_collect(fivonagi_ctx* c, int inst, const fivonagi_collect_opts* o) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  auto& d = c->desc[inst];
+  const int nx = d.nx, ny = d.ny;
+  if (o) {
+    if (o->jcut < 0 || o->jcut >= ny || !(o->eta > 0))
+      return fail(FIVONAGI_ERR_ARG, "collect: jcut outside the grid or eta <= 0");
+    if (!d.measure) {
+      hipError_t e;
+      d.measure = (float*)c->dalloc(sizeof(float) * (size_t)nx * ny, &e);
+      if (e == hipSuccess) d.stage = (float*)c->dalloc(sizeof(float) * (size_t)nx, &e);
+      if (e == hipSuccess) d.collect_err = (uint32_t*)c->dalloc(sizeof(uint32_t), &e);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc (collect)");
+    }
+    HIP_TRY(hipMemsetAsync(d.measure, 0, sizeof(float) * (size_t)nx * ny, c->stream));
+    HIP_TRY(hipMemsetAsync(d.stage, 0, sizeof(float) * (size_t)nx, c->stream));
+    HIP_TRY(hipMemsetAsync(d.collect_err, 0, sizeof(uint32_t), c->stream));
+    d.jcut = o->jcut;
+    d.eta = o->eta;
+    for (int k = 0; k < 4; ++k) d.mic[k] = o->mic[k];
+    c->collect_nx = std::max(c->collect_nx, nx);
+    c->ext_any = true;
+  } else {
+    d.measure = nullptr;   // the buffers stay allocated until destroy
+    d.stage = nullptr;
+  }
+  c->drop_graph();         // the captured launches change
+  return upload_tables(c);
+}
+
+int fivonagi_measure(fivonagi_ctx* c, int inst, float* out) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  if (!out) return fail(FIVONAGI_ERR_ARG, "null output");
+  const auto& d = c->desc[inst];
+  if (!d.measure) return fail(FIVONAGI_ERR_ARG, "data collection is off for this instance");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(out, d.measure, sizeof(float) * (size_t)d.nx * d.ny,
+                    hipMemcpyDeviceToHost));
+  return collect_status(c);
 }
 
 int fivonagi_collect(fivonagi_ctx* c, int inst, float* q1, float* q2, float* q3, double* T,

@@ -419,6 +419,9 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     sx = uni(md_shift(Tn, D.md_t, D.md_vx, D.dx, P.mdx));
     sy = uni(md_shift(Tn, D.md_t, D.md_vy, D.dy, P.mdy));
   }
+  gfloat* __restrict__ stage = (gfloat*)D.stage;
+  const bool stage_on = kExt && stage != nullptr;
+  const int jcut = D.jcut;
   // HIFU: the packet runs while the window has not moved (boundary.h:82)
   const bool hdrive = hifu && P.mdx == 0 && it.x0 == 0;
   const double Td = P.T;
@@ -487,6 +490,8 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
         stg(o2, o, r2);
         stg(o3, o, r3);
       }
+      // data collection reads row jcut as it is before the shift
+      if (stage_on && s == jcut && out_lane) stg(stage, 4u * (uint32_t)i, mid.q1);
     }
   };
 
@@ -617,6 +622,62 @@ __global__ __launch_bounds__(64) void fv_settle_kernel(const InstDesc* __restric
   if (threadIdx.x == 0) D.ctl[(phase + 1) & 1] = P;
 }
 
+// Data collection of attempt a (apps/hifu-beam/data_collect.h:7-107), run
+// after its step launch: if the attempt was accepted, record its new T, its
+// cfl and row jcut of its new state (staged by the step before the window
+// shift, with the pre-shift MDX) into the instance's measurement grid.
+// blockIdx.y = instance; threads cover the columns (and, while T <= 0, the
+// whole grid, which the reference zeroes then).
+__global__ __launch_bounds__(256) void fv_collect_kernel(const InstDesc* __restrict__ insts,
+                                                         int phase) {
+  const InstDesc& D = insts[blockIdx.y];
+  float* __restrict__ m = D.measure;
+  if (m == nullptr) return;
+  // attempt a's control block (its step launch published it) and slots
+  const Fold F = fold_decide(fold_load(D.ctl + ((phase + 1) & 1), D.sync + phase % 3),
+                             D.cfl_max, D.dt_seq != nullptr);
+  if (F.state != kFoldAccept) return;
+  const Ctl& P = F.p;
+  const double T = P.T + P.dt;                       // update_T
+  const int nx = D.nx, ny = D.ny;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (T <= 0.0) {                                    // data_collect.h:27-30
+    for (int64_t k = tid; k < (int64_t)nx * ny; k += (int64_t)gridDim.x * blockDim.x)
+      m[k] = 0.0f;
+    return;
+  }
+  const int i = tid;
+  if (i >= nx) return;
+  const float u1 = D.stage[i];
+  const int mdx = P.mdx;
+  const int64_t nf = P.n_acc - D.n0frame;          // n - 1 - n0frame, n = n_acc + 1
+  if (nf >= nx) {                                   // data_collect.h:96
+    if (i == 0) aadd_relaxed(D.collect_err, 4u);
+    return;
+  }
+  for (int k = 0; k < 4; ++k) {                     // data_collect.h:51-76
+    if (i + mdx == D.mic[k]) {
+      float* row = m + (size_t)(2 + k) * nx;
+      if (nf == 0)
+        for (int ii = 0; ii < nx; ++ii) row[ii] = 0.0f;
+      row[nf] = u1;
+      break;
+    }
+  }
+  if (i == 0) {                                     // data_collect.h:78-94
+    m[nf] = (float)T;
+    m[nf + nx] = F.cfl;
+    if (nf == 0)
+      for (int k = 0; k < 4; ++k)
+        if (D.mic[k] - mdx < 0 || D.mic[k] - mdx >= nx)
+          for (int ii = 0; ii < nx; ++ii) m[(size_t)(2 + k) * nx + ii] = 0.0f;
+  }
+  // central peak only (data_collect.h:100-106)
+  if (fabs((double)(i + mdx) / D.eta - T) > 0.5) return;
+  const int64_t o = (int64_t)i + mdx + 6 * (int64_t)nx;
+  if (o < (int64_t)nx * ny) m[o] = fmaxf(u1, m[o]);
+}
+
 // Closed-form ghost ring of the Taylor-shock instances (apps/taylor-angle-xy/
 // boundary.h:38-55), written into the current input buffer before the step
 // reads it.  The values depend on the cell and on T only, never on the state,
@@ -664,6 +725,14 @@ extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_i
   const int blocks = (max_ring + 255) / 256;
   hipLaunchKernelGGL(fv::fv_taylor_ghost_kernel, dim3(blocks, n_inst), dim3(256), 0, stream,
                      insts, phase);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t fv_launch_collect(const fv::InstDesc* insts, int n_inst, int max_nx,
+                                        int phase, hipStream_t stream) {
+  if (n_inst <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fv::fv_collect_kernel, dim3((max_nx + 255) / 256, n_inst), dim3(256), 0,
+                     stream, insts, phase);
   return hipGetLastError();
 }
 

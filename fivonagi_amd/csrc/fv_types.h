@@ -84,7 +84,15 @@ struct InstDesc {
   // hifu_tab[4j..4j+3] = phi_j, nu_x, nu_y, 1 on aperture rows (0 elsewhere)
   const float* hifu_tab;
   float hifu_k;         // FREC/4: envelope argument per unit of T + phi
-  float pad4;
+  int32_t jcut;         // data collection: row (JCUT)
+  // data collection (apps/hifu-beam/data_collect.h): measure = nx*ny grid
+  // (null: off), stage = row jcut of the last attempt before the window shift
+  float* measure;
+  float* stage;
+  uint32_t* collect_err;  // 4 once a frame holds more than nx steps
+  int64_t n0frame;      // accepted steps when the current frame started
+  double eta;           // cells per length unit
+  int32_t mic[4];
 };
 
 // One wave's work: a 58-column strip of one instance, rows [y0, y1).

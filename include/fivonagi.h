@@ -44,7 +44,7 @@ enum {
 };
 enum {
   FIVONAGI_OK = 0, FIVONAGI_ERR_EIGEN = 2, FIVONAGI_ERR_ARG = 8,
-  FIVONAGI_ERR_DTSEQ = 9, FIVONAGI_ERR_STALL = 10
+  FIVONAGI_ERR_DTSEQ = 9, FIVONAGI_ERR_STALL = 10, FIVONAGI_ERR_COLLECT = 4
 };
 
 /* Per-instance parameters: the runtime form of apps/<app>/parameters.h
@@ -169,6 +169,26 @@ int fivonagi_step_adjoint(fivonagi_ctx *ctx, int64_t nsteps, const fivonagi_obje
                           double *grad_phase);
 /* dJ/d(state at the start of the last fivonagi_step_adjoint call), 3 grids. */
 int fivonagi_adjoint_state(fivonagi_ctx *ctx, int inst, float *l1, float *l2, float *l3);
+
+/* ---- data collection (apps/hifu-beam/data_collect.h:7-107) ---------------
+   After every accepted step, on row jcut of the new state (before the
+   moving-window shift), into a measurement grid of nx*ny floats laid out as
+   the reference's measure1: row 0 = T and row 1 = cfl of the frame's steps,
+   rows 2..5 = u1 at the four microphones (x index mic[k] of the unmoved
+   grid, i.e. i + MDX == mic[k]), from row 6 on = max of u1 over the steps
+   where |(i + MDX)/eta - T| <= 0.5, at flat index i + MDX + 6 nx.  Step n of a
+   frame is entry n - 1 - n0, n0 = the step count when the frame started
+   (fivonagi_step_until starts a frame).  While T <= 0 the grid is zeroed.
+   More than nx steps in one frame stops the call with code 4 (simError 4). */
+typedef struct {
+  int32_t jcut;            /* JCUT (2 in the hifu app)                        */
+  int32_t mic[4];          /* microphone x indices: round(ISPF*{.5,.9}), ISPF, round(ISPF*1.1) */
+  double eta;              /* ETA = NX/XMAX cells per length unit             */
+} fivonagi_collect_opts;
+/* opts = NULL switches collection off for the instance. */
+int fivonagi_set_collect(fivonagi_ctx *ctx, int inst, const fivonagi_collect_opts *opts);
+/* Copy the measurement grid (nx*ny floats) of instance inst to the host. */
+int fivonagi_measure(fivonagi_ctx *ctx, int inst, float *measure);
 
 /* clean_gpu (common/data_definitions.h:415-517) */
 void fivonagi_destroy(fivonagi_ctx *ctx);

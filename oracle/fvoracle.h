@@ -36,7 +36,7 @@ extern "C" {
 enum { ORC_BC_ZEROGRAD = 0, ORC_BC_EDGE_DRIVE = 1, ORC_BC_TAYLOR = 2, ORC_BC_HIFU = 3 };
 enum { ORC_CFL_REFERENCE_FOLD = 0, ORC_CFL_ALL = 1 };
 enum { ORC_FORM_LITERAL = 0, ORC_FORM_RHO1 = 1 };
-enum { ORC_OK = 0, ORC_ERR_DT = 1, ORC_ERR_EIGEN = 2, ORC_ERR_ARG = 8 };
+enum { ORC_OK = 0, ORC_ERR_DT = 1, ORC_ERR_EIGEN = 2, ORC_ERR_COLLECT = 4, ORC_ERR_ARG = 8 };
 
 typedef struct {
   int32_t nx, ny;               /* NX, NY incl. the 2-wide ghost frame        */
@@ -80,6 +80,17 @@ typedef struct {
      negated (the beam axis); the 2x2 corners are 0.
      hifu_ampl = AMPL, hifu_frec = FREC, hifu_a = ISPa, hifu_f = ISPF (points). */
   double hifu_ampl, hifu_frec, hifu_a, hifu_f;
+  /* data collection (apps/hifu-beam/data_collect.h:7-107), after every
+     accepted step, before the window shift: measure (nx*ny, NULL = off) as
+     the reference's measure1 -- row 0 T, row 1 cfl, rows 2..5 u1 at the
+     microphones (i + MDX == mic[k]), from row 6 on max u1 over
+     |(i + MDX)/eta - T| <= 0.5 at flat index i + MDX + 6 nx; entry
+     n - 1 - n0frame of the frame; zeroed while T <= 0; more than nx steps in
+     a frame stops the run with ORC_ERR_COLLECT (simError 4). */
+  double *measure;
+  int32_t jcut, mic[4], pad1;
+  double eta;
+  int64_t n0frame;
 } orc_params;
 
 typedef struct {

@@ -425,6 +425,46 @@ static void SFX(sweep)(SFX(ctx_t) *c, const orc_params *p, int axis, REAL dt) {
   SFX(write_limiters)(c, p, axis, c->B, c->A, dt);
 }
 
+/* dataCollect, apps/hifu-beam/data_collect.h:7-107, on the accepted state in
+   c->C at the updated T and n, before the window shift (main.cu:563-573).
+   Within one launch the threads touch disjoint entries, so the column loop
+   below is the same computation.  Returns 0 or ORC_ERR_COLLECT. */
+static int SFX(collect)(SFX(ctx_t) *c, const orc_params *p, const orc_state *st) {
+  double *m = p->measure;
+  const int NX = c->nx, NY = c->ny;
+  if (st->T <= 0) {                                    /* data_collect.h:27-30 */
+    for (size_t k = 0; k < c->n; ++k) m[k] = 0.0;
+    return 0;
+  }
+  const int64_t nf = st->n_accepted - 1 - p->n0frame;
+  if (nf >= NX) return ORC_ERR_COLLECT;               /* data_collect.h:96 */
+  const REAL *row = c->C[0] + (size_t)p->jcut * NX;
+  for (int i = 0; i < NX; ++i) {
+    const double u1 = (double)row[i];
+    for (int k = 0; k < 4; ++k) {                      /* data_collect.h:51-76 */
+      if (i + st->mdx == p->mic[k]) {
+        double *mr = m + (size_t)(2 + k) * NX;
+        if (nf == 0)
+          for (int ii = 0; ii < NX; ++ii) mr[ii] = 0.0;
+        mr[nf] = u1;
+        break;
+      }
+    }
+    if (i == 0) {                                      /* data_collect.h:78-94 */
+      m[nf] = st->T;
+      m[nf + NX] = st->last_cfl;
+      if (nf == 0)
+        for (int k = 0; k < 4; ++k)
+          if (p->mic[k] - st->mdx < 0 || p->mic[k] - st->mdx >= NX)
+            for (int ii = 0; ii < NX; ++ii) m[(size_t)(2 + k) * NX + ii] = 0.0;
+    }
+    if (fabs((double)(i + st->mdx) / p->eta - st->T) > 0.5) continue;   /* :100-103 */
+    const int64_t o = (int64_t)i + st->mdx + 6 * (int64_t)NX;
+    if (o < (int64_t)NX * NY) m[o] = u1 > m[o] ? u1 : m[o];
+  }
+  return 0;
+}
+
 /* moveDomain + updateMDXY, common/main.cu:109-176, on the accepted state in
    c->C at the updated T (main.cu:575-586): C(i,j) <- C(i+MDi, j+MDj) when
    that cell is on the grid (the reference checks only the upper bounds;
@@ -449,7 +489,8 @@ static void SFX(move_domain)(SFX(ctx_t) *c, const orc_params *p, orc_state *st) 
 }
 
 /* One step attempt on the state in c->C (the backup buffer holds the input,
-   main.cu:334).  Returns 0 accepted, 1 rejected, 2 eigenvalue error. */
+   main.cu:334).  Returns 0 accepted, 1 rejected, 2 eigenvalue error, 4 data
+   collection overflow. */
 static int SFX(attempt)(SFX(ctx_t) *c, const orc_params *p, orc_state *st,
                         double dt_used, int prescribed) {
   REAL dt = (REAL)dt_used;
@@ -468,6 +509,7 @@ static int SFX(attempt)(SFX(ctx_t) *c, const orc_params *p, orc_state *st,
     for (int f = 0; f < 3; ++f) memcpy(c->C[f], c->B[f], c->n * sizeof(REAL));
     st->T += (double)dt;                                  /* main.cu:232-238 */
     st->n_accepted++;
+    if (p->measure && SFX(collect)(c, p, st)) return 4;   /* main.cu:563-573 */
     if (p->md_on) SFX(move_domain)(c, p, st);             /* main.cu:575-586 */
   } else {
     st->n_rejected++;                                     /* clean retry */
@@ -503,6 +545,7 @@ static int SFX(run_impl)(const orc_params *p, REAL *q1, REAL *q2, REAL *q3,
     }
     int r = SFX(attempt)(&c, p, st, dt_used, prescribed);
     if (r == 2) { rc = ORC_ERR_EIGEN; st->error = rc; break; }
+    if (r == 4) { rc = ORC_ERR_COLLECT; st->error = rc; break; }
     consecutive_rej = (r == 1) ? consecutive_rej + 1 : 0;
     if (consecutive_rej > 200) { rc = ORC_ERR_DT; st->error = rc; break; }
   }

@@ -12,6 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR, BC_HIFU = 0, 1, 2, 3
+OK, ERR_DT, ERR_EIGEN, ERR_COLLECT, ERR_ARG = 0, 1, 2, 4, 8
 CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
 FORM_LITERAL, FORM_RHO1 = 0, 1
 
@@ -36,6 +37,9 @@ class Params(C.Structure):
         ("T_stop", C.c_double),
         ("hifu_ampl", C.c_double), ("hifu_frec", C.c_double),
         ("hifu_a", C.c_double), ("hifu_f", C.c_double),
+        ("measure", C.POINTER(C.c_double)),
+        ("jcut", C.c_int32), ("mic", C.c_int32 * 4), ("pad1", C.c_int32),
+        ("eta", C.c_double), ("n0frame", C.c_int64),
     ]
 
 
@@ -124,6 +128,11 @@ class Problem:
         self.moving = None if moving is None else tuple(float(v) for v in moving)
         self.T_stop = None if T_stop is None else float(T_stop)
         self.hifu = hifu  # dict(ampl, frec, a, f)
+        # data collection: dict(jcut, mic, eta); measure (ny, nx) float64 is
+        # the measurement grid the runs write into, n0frame the frame start
+        self.collect = None
+        self.measure = None
+        self.n0frame = 0
 
     def cparams(self):
         p = Params()
@@ -152,6 +161,14 @@ class Problem:
         if self.T_stop is not None:
             p.stop_at_T = 1
             p.T_stop = self.T_stop
+        if self.collect:
+            if self.measure is None:
+                self.measure = np.zeros((self.ny, self.nx))
+            c = self.collect
+            p.measure = _ptr(self.measure, C.c_double)
+            p.jcut, p.eta, p.n0frame = int(c["jcut"]), float(c["eta"]), int(self.n0frame)
+            for k in range(4):
+                p.mic[k] = int(c["mic"][k])
         return p
 
 

@@ -200,6 +200,67 @@ def test_hifu_app_matches_oracle(oracle, moving):
     assert max(e) < TOL
 
 
+def hifu_collect_opts(p):
+    """The hifu app's microphones and cut (data_collect.h:45-50, JCUT = 2,
+    ETA = NX/XMAX), for a grid of the app's physical size."""
+    F = np.float32(p.hifu["f"])
+    mic = [int(round(F * np.float32(0.5))), int(round(F * np.float32(0.9))), int(p.hifu["f"]),
+           int(round(F * np.float32(1.1)))]
+    return dict(jcut=2, mic=mic, eta=p.nx / 25.0)
+
+
+def test_data_collection_matches_oracle(oracle):
+    """dataCollect (apps/hifu-beam/data_collect.h:7-107) over frames of the
+    hifu app with its moving window: T and cfl of every step, the
+    microphones and the running u1 maximum of the central peak."""
+    prob = wl.hifu(nx=128, ny=256, ispam=10.0)
+    opts = hifu_collect_opts(prob)
+    T_ends = [prob.T0 + k for k in range(1, 30)]
+    with fa.Context(prob) as ctx:
+        ctx.set_collect(0, **opts)
+        for Te in T_ends:
+            ctx.step_until(Te)
+        m = ctx.measure(0)
+        info = ctx.info(0)
+    op = to_oracle(prob)
+    op.collect = opts
+    q, st = prob.q0.astype(np.float64), None
+    for Te in T_ends:
+        op.T_stop = Te
+        op.n0frame = 0 if st is None else st.n_accepted
+        q, st, rc = oracle.run(op, q, 10 ** 6, state=st)
+        assert rc == 0
+    mo = op.measure
+    assert info["n_accepted"] == st.n_accepted and info["md_x"] == st.mdx > 0
+    nf = st.n_accepted - 1 - op.n0frame + 1            # steps in the last frame
+    np.testing.assert_allclose(m[0, :nf], mo[0, :nf], rtol=1e-6)
+    np.testing.assert_allclose(m[1, :nf], mo[1, :nf], rtol=1e-5)
+    # microphones and u1max sample the rho' field: relative to its peak on the
+    # cut (the u1max maximum), as the field parity is relative to the field max
+    den = np.abs(mo[6:]).max()
+    assert den > 0 and np.abs(mo[2]).max() > 0     # the first microphone heard the beam
+    e_mic = [np.abs(m[2 + k] - mo[2 + k]).max() / den for k in range(4)]
+    e_max = np.abs(m[6:] - mo[6:]).max() / den
+    report("data collection mic1 mic2 u1max", [e_mic[0], e_mic[1], e_max])
+    assert max(e_mic) < 1e-5 and e_max < 1e-5
+
+
+def test_data_collection_frame_overflow(oracle):
+    """More than nx steps in one frame is simError 4 (data_collect.h:96)."""
+    prob = wl.hifu(nx=64, ny=128, ispam=10.0)
+    opts = hifu_collect_opts(prob)
+    with fa.Context(prob) as ctx:
+        ctx.set_collect(0, **opts)
+        ctx.step_until(prob.T0 + 0.5)               # T > 0 is 30+ time units away
+        with pytest.raises(fa.FivonagiError) as ei:
+            ctx.step_until(prob.T0 + 40.0)
+        assert ei.value.code == fa.ERR_COLLECT
+    op = to_oracle(prob, T_stop=prob.T0 + 40.0)
+    op.collect = opts
+    q, st, rc = oracle.run(op, prob.q0.astype(np.float64), 10 ** 6)
+    assert rc == oracle.ERR_COLLECT
+
+
 def test_moving_window_in_a_batch(oracle):
     """A batch where one instance moves and one does not (the multi-instance
     form of the moving-window kernel)."""
