@@ -153,6 +153,23 @@ def hifu(etac=1, ispam=30.0, cflw=0.9, nx=None, ny=None, moving=True,
                    moving=md, hifu=h)
 
 
+def hifu_collect(prob):
+    """The hifu app's data-collection options (apps/hifu-beam/data_collect.h:
+    45-50, parameters.h): JCUT = 2, ETA = NX/XMAX, microphones at
+    round(ISPF*0.5f), round(ISPF*0.9f), ISPF, round(ISPF*1.1f), with ISPF the
+    float the FP32 build holds (50*ISPlambda/(XMAX/(NX-1)) rounded to float;
+    int() truncates, C round() rounds halves away from zero)."""
+    f32 = np.float32
+    ispf = f32(50.0 * 1.0 / float(f32(25.0) / f32(prob.nx - 1)))
+
+    def c_round(v):
+        return int(math.floor(float(v) + 0.5))
+
+    mic = [c_round(ispf * f32(0.5)), c_round(ispf * f32(0.9)), int(ispf),
+           c_round(ispf * f32(1.1))]
+    return dict(jcut=2, mic=mic, eta=prob.nx / 25.0)
+
+
 def aperture_params(ny, n_rows, rng, amp=0.02, halfwidth_rows=None, noise=0.05):
     """Edge-drive parameters on an aperture of n_rows rows centred on the edge:
     amp_j = amp * (g_j + noise * N(0,1)), phase_j = noise * N(0,1), zero

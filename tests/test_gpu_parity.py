@@ -203,10 +203,7 @@ def test_hifu_app_matches_oracle(oracle, moving):
 def hifu_collect_opts(p):
     """The hifu app's microphones and cut (data_collect.h:45-50, JCUT = 2,
     ETA = NX/XMAX), for a grid of the app's physical size."""
-    F = np.float32(p.hifu["f"])
-    mic = [int(round(F * np.float32(0.5))), int(round(F * np.float32(0.9))), int(p.hifu["f"]),
-           int(round(F * np.float32(1.1)))]
-    return dict(jcut=2, mic=mic, eta=p.nx / 25.0)
+    return wl.hifu_collect(p)
 
 
 def test_data_collection_matches_oracle(oracle):
@@ -259,6 +256,28 @@ def test_data_collection_frame_overflow(oracle):
     op.collect = opts
     q, st, rc = oracle.run(op, prob.q0.astype(np.float64), 10 ** 6)
     assert rc == oracle.ERR_COLLECT
+
+
+def test_hifu_frames_export(tmp_path):
+    """The hifu app end to end: calcFrame's frame loop with data collection
+    and the reference's exports (fivonagi_amd.export.hifu_frames)."""
+    from fivonagi_amd.export import hifu_frames
+    prob = wl.hifu(nx=128, ny=256, ispam=10.0)
+    with fa.Context(prob) as ctx:
+        hifu_frames(ctx, prob, str(tmp_path), frames=25)
+        q, st = ctx.collect(0)
+        m = ctx.measure(0)
+        md = ctx.info(0)["md_x"]
+    assert st["T"] >= prob.T0 + 25 and md > 0
+    rows = np.loadtxt(tmp_path / "u1-cut-0024.dat")
+    np.testing.assert_allclose(rows[:, 1], q[0][2], rtol=1e-5, atol=1e-12)
+    umax = np.loadtxt(tmp_path / "u1max.dat")
+    assert umax.shape == (prob.nx + md, 2)
+    np.testing.assert_allclose(umax[:, 1], m.reshape(-1)[6 * prob.nx:7 * prob.nx + md],
+                               rtol=1e-5, atol=1e-12)
+    mic = np.loadtxt(tmp_path / "microphone_1.dat", comments="#")
+    assert mic.shape[0] == st["n_accepted"] and mic[-1, 0] == st["n_accepted"]
+    assert np.abs(mic[:, 2]).max() > 0
 
 
 def test_moving_window_in_a_batch(oracle):
