@@ -173,6 +173,58 @@ def bench_batched(args, hip):
     }
 
 
+def bench_hifu(args, hip):
+    """The hifu-beam app at ETAC = 4 (2048x4096, ISPAM = 30): calcFrame's frame
+    loop (fivonagi_step_until per frame of DTPRINT = 1) with the moving window
+    and the data collection on; `steps` is a minimum, whole frames are
+    timed (their host synchronisations included)."""
+    import fivonagi_amd as fa
+    from fivonagi_amd import workloads as wl
+    prob = wl.hifu(4)
+    stream = hip.stream()
+    ctx = fa.Context(prob)
+    ctx.set_stream(stream.value)
+    ctx.set_collect(0, **wl.hifu_collect(prob))
+    # warm-up: the packet is launched at T ~ 0; start timing past MDT so the
+    # window moves and the collection records
+    T = prob.T0
+    while T < prob.moving[2] + 1.0:      # frames of DTPRINT = 1, as calcFrame
+        T += 1.0
+        ctx.step_until(T)
+    before = ctx.info(0)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    frames = 0
+    while True:
+        T += 1.0
+        ctx.step_until(T)
+        frames += 1
+        if ctx.info(0)["n_accepted"] - before["n_accepted"] >= args.steps:
+            break
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1)
+    after = ctx.info(0)
+    steps = after["n_accepted"] - before["n_accepted"]
+    cells = prob.nx * prob.ny * steps
+    value = cells / (ms / 1e3) / 1e9
+    achieved = BYTES_PER_CELL * prob.nx * prob.ny / (ms / steps / 1e3) / 1e9
+    ctx.close()
+    return {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": before["n_accepted"], "ms_per_step": round(ms / steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "hifu-beam app ETAC=4 (2048x4096, ISPAM=30): frame loop, "
+                               "moving window, data collection",
+                   "frames": frames, "T_start": before["T"], "T_end": after["T"],
+                   "md_x": after["md_x"], "rows_per_wave": after["rows_per_wave"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": KERNEL},
+        "cpu_baseline": None,
+    }
+
+
 def bench_adjoint(args, hip):
     """One gradient of J (time-integrated rho'^2 over a 64x64 box at x=1500)
     with respect to 512 edge parameters on 2048x1024: forward (recording dt)
@@ -252,7 +304,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain copy of known size (counter calibration)")
     ap.add_argument("--workload", default="edge_drive",
-                    choices=["edge_drive", "batched", "adjoint", "batched_adjoint"])
+                    choices=["edge_drive", "batched", "adjoint", "batched_adjoint", "hifu"])
     ap.add_argument("--plain-instances", type=int, default=32,
                     help="batched_adjoint: plain solves timed (scaled to 32)")
     args = ap.parse_args()
@@ -265,7 +317,7 @@ def main():
             sys.exit(2)
         hip = Hip()
         fn = {"batched": bench_batched, "adjoint": bench_adjoint,
-              "batched_adjoint": bench_batched_adjoint}[args.workload]
+              "batched_adjoint": bench_batched_adjoint, "hifu": bench_hifu}[args.workload]
         print(json.dumps(fn(args, hip)), flush=True)
         return
 
