@@ -225,6 +225,53 @@ def bench_hifu(args, hip):
     }
 
 
+def bench_taylor_study(args, hip):
+    """The taylor-angle-xy convergence study as the reference runs it
+    (apps/taylor-angle-xy/run.sh): 5 resolutions (NX = 64..1024) x 5 CFLs x 5
+    angles, each run to T = 100 with the app's moving window, data collection
+    and exports (no whole-grid files), in FP32 and in FP64, then the errors
+    (plots/error.py) and the convergence tables (plots/error_table_*.py).
+    Each precision's 125 runs are one batched Context.  value = cell updates
+    per second of the stepping (step_until calls, their host synchronisations
+    included); the exports' host time is reported beside it."""
+    import shutil
+    import tempfile
+    from fivonagi_amd import taylor_study as ts
+    root = args.study_dir or tempfile.mkdtemp(prefix="taylor_study_")
+    res, dep = os.path.join(root, "results"), os.path.join(root, "deploy")
+    os.makedirs(res, exist_ok=True)
+    os.makedirs(dep, exist_ok=True)
+    t0 = time.perf_counter()
+    stats = ts.run_sweep(res, log=lambda m: print(m, file=sys.stderr, flush=True))
+    err = ts.extract_errors(res, dep)
+    ts.write_tables(dep, err)
+    wall = time.perf_counter() - t0
+    rates = {}
+    for pi, p in enumerate(ts.PRECISIONS):
+        for ei, et in enumerate(ts.ERROR_TYPES):
+            rr = [ts._rates(err, ts.ETAS, ci, pi, ai, ei)[1]
+                  for ai in range(len(ts.ANGLES)) for ci in range(len(ts.CFLS))]
+            rates[f"p{p}_{et}_rate_reg_mean"] = round(float(np.mean(rr)), 4)
+    cells = sum(v["cell_updates"] for v in stats.values())
+    gpu_s = sum(v["gpu_s"] for v in stats.values())
+    if not args.study_dir:
+        shutil.rmtree(root, ignore_errors=True)
+    per_p = {f"p{p}": {"runs": v["instances"], "cell_updates": v["cell_updates"],
+                       "stepping_s": round(v["gpu_s"], 3), "wall_s": round(v["wall_s"], 3),
+                       "gcell_per_s": round(v["cell_updates"] / v["gpu_s"] / 1e9, 3)}
+             for p, v in stats.items()}
+    return {
+        "metric": METRIC, "value": round(cells / gpu_s / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "taylor-angle-xy convergence study: eta 5..82 x cfl 60..99 x "
+                               "5 angles x p1/p2 to T=100, exports, errors, tables",
+                   "per_precision": per_p, "seconds_total": round(wall, 2), **rates},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def bench_adjoint(args, hip):
     """One gradient of J (time-integrated rho'^2 over a 64x64 box at x=1500)
     with respect to 512 edge parameters on 2048x1024: forward (recording dt)
@@ -300,11 +347,17 @@ def main():
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--rows-per-wave", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp64", action="store_true",
+                    help="edge_drive in FP64 (the reference's PRECISION=2); not the headline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain copy of known size (counter calibration)")
     ap.add_argument("--workload", default="edge_drive",
-                    choices=["edge_drive", "batched", "adjoint", "batched_adjoint", "hifu"])
+                    choices=["edge_drive", "batched", "adjoint", "batched_adjoint", "hifu",
+                             "taylor_study"])
+    ap.add_argument("--study-dir", default=None,
+                    help="taylor_study: keep the runs and tables here (default: a temporary "
+                         "directory, removed)")
     ap.add_argument("--plain-instances", type=int, default=32,
                     help="batched_adjoint: plain solves timed (scaled to 32)")
     args = ap.parse_args()
@@ -317,7 +370,8 @@ def main():
             sys.exit(2)
         hip = Hip()
         fn = {"batched": bench_batched, "adjoint": bench_adjoint,
-              "batched_adjoint": bench_batched_adjoint, "hifu": bench_hifu}[args.workload]
+              "batched_adjoint": bench_batched_adjoint, "hifu": bench_hifu,
+              "taylor_study": bench_taylor_study}[args.workload]
         print(json.dumps(fn(args, hip)), flush=True)
         return
 
@@ -336,7 +390,7 @@ def main():
         sys.exit(1)
     prob = wl.edge_drive(args.nx, args.ny, rows_per_wave=args.rows_per_wave)
     stream = hip.stream()
-    ctx = fa.Context(prob)
+    ctx = fa.Context(prob, fp64=args.fp64)
     ctx.set_stream(stream.value)
     info0 = ctx.info(0)
 
@@ -361,9 +415,10 @@ def main():
     value = cells * accepted / secs / 1e9
     ms_per_step = ms / args.steps
     # dominant (only) kernel: one launch per attempt
-    achieved = BYTES_PER_CELL * cells / (ms_per_step / 1e3) / 1e9
+    bpc = BYTES_PER_CELL * (2 if args.fp64 else 1)
+    achieved = bpc * cells / (ms_per_step / 1e3) / 1e9
     tag = f"edge_drive_{prob.nx}x{prob.ny}"
-    traffic = traffic_from_profiles(tag)
+    traffic = None if args.fp64 else traffic_from_profiles(tag)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -375,10 +430,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64" if args.fp64 else "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"edge_drive {prob.nx}x{prob.ny} fp32 forward (A=0.02, aperture "
+            "workload": f"edge_drive {prob.nx}x{prob.ny} {'fp64' if args.fp64 else 'fp32'} "
+                        f"forward (A=0.02, aperture "
                         f"half-width 200 cells, 20 cells/wavelength, beta=3.5, delta=5e-5, CFL 0.9)",
             "accepted_steps": accepted, "attempts": attempts,
             "rows_per_wave": after["rows_per_wave"], "skew": round(after["skew"], 3),
@@ -391,7 +447,7 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel": KERNEL,
-            "bytes_per_launch": BYTES_PER_CELL * cells,
+            "bytes_per_launch": bpc * cells,
             "avg_launch_ms": round(ms_per_step, 5),
         },
         "cpu_baseline": None,

@@ -17,11 +17,13 @@ CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
 OK, ERR_EIGEN, ERR_COLLECT, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 4, 8, 9, 10
 
 EXPORTS = (
-    "fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence", "fivonagi_step",
-    "fivonagi_step_until",
+    "fivonagi_init", "fivonagi_init_f64", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
+    "fivonagi_step", "fivonagi_step_until", "fivonagi_collect_f64",
     "fivonagi_enqueue", "fivonagi_sync", "fivonagi_collect", "fivonagi_info",
     "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_set_skew",
-    "fivonagi_set_collect", "fivonagi_measure", "fivonagi_destroy", "fivonagi_last_error",
+    "fivonagi_set_collect", "fivonagi_measure", "fivonagi_measure_f64", "fivonagi_measure_rows",
+    "fivonagi_measure_rows_f64", "fivonagi_cut_errors", "fivonagi_cut_errors_f64",
+    "fivonagi_destroy", "fivonagi_last_error",
 )
 
 
@@ -55,6 +57,7 @@ class Info(C.Structure):
         ("n_items", C.c_int32), ("rows_per_wave", C.c_int32),
         ("skew", C.c_double), ("rotate", C.c_int32), ("pad", C.c_int32),
         ("md_x", C.c_int32), ("md_y", C.c_int32),
+        ("fp64", C.c_int32), ("pad2", C.c_int32),
     ]
 
 
@@ -66,7 +69,8 @@ class Objective(C.Structure):
 
 class CollectOpts(C.Structure):
     """fivonagi_collect_opts (include/fivonagi.h)."""
-    _fields_ = [("jcut", C.c_int32), ("mic", C.c_int32 * 4), ("eta", C.c_double)]
+    _fields_ = [("jcut", C.c_int32), ("mic", C.c_int32 * 4), ("eta", C.c_double),
+                ("cut_half", C.c_int32), ("pad", C.c_int32), ("tan_theta", C.c_double)]
 
 
 class AdjointOpts(C.Structure):
@@ -90,6 +94,8 @@ def load():
     P = C.POINTER(Params)
     vp = C.c_void_p
     lib.fivonagi_init.argtypes = [C.POINTER(vp), P, C.c_int, C.POINTER(C.POINTER(C.c_float))]
+    lib.fivonagi_init_f64.argtypes = [C.POINTER(vp), P, C.c_int,
+                                      C.POINTER(C.POINTER(C.c_double))]
     lib.fivonagi_set_stream.argtypes = [vp, vp]
     lib.fivonagi_set_dt_sequence.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_int64]
     lib.fivonagi_step.argtypes = [vp, C.c_int64]
@@ -100,6 +106,7 @@ def load():
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int64)
     lib.fivonagi_collect.argtypes = [vp, C.c_int, fp, fp, fp, dp, dp, ip, ip]
+    lib.fivonagi_collect_f64.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, ip, ip]
     lib.fivonagi_info.argtypes = [vp, C.c_int, C.POINTER(Info)]
     lib.fivonagi_step_adjoint.argtypes = [vp, C.c_int64, C.POINTER(Objective),
                                           C.POINTER(AdjointOpts), dp, dp, dp]
@@ -108,16 +115,23 @@ def load():
     lib.fivonagi_set_skew.argtypes = [vp, C.c_double, C.c_int]
     lib.fivonagi_set_collect.argtypes = [vp, C.c_int, C.POINTER(CollectOpts)]
     lib.fivonagi_measure.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
+    lib.fivonagi_measure_f64.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    lib.fivonagi_measure_rows.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    lib.fivonagi_measure_rows_f64.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    lib.fivonagi_cut_errors.argtypes = [fp, fp, C.c_int, fp]
+    lib.fivonagi_cut_errors_f64.argtypes = [dp, dp, C.c_int, fp]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
     lib.fivonagi_last_error.argtypes = []
     lib.fivonagi_last_error.restype = C.c_char_p
-    for name in ("fivonagi_init", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
+    for name in ("fivonagi_init", "fivonagi_init_f64", "fivonagi_collect_f64",
+                 "fivonagi_set_stream", "fivonagi_set_dt_sequence",
                  "fivonagi_step", "fivonagi_step_until", "fivonagi_enqueue", "fivonagi_sync",
                  "fivonagi_collect",
                  "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state",
                  "fivonagi_autotune", "fivonagi_set_skew", "fivonagi_set_collect",
-                 "fivonagi_measure"):
+                 "fivonagi_measure", "fivonagi_measure_f64", "fivonagi_measure_rows",
+                 "fivonagi_measure_rows_f64", "fivonagi_cut_errors", "fivonagi_cut_errors_f64"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -157,6 +171,7 @@ class Problem:
         # BC_HIFU: dict(ampl=AMPL, frec=FREC, a=ISPa, f=ISPF)
         self.hifu = hifu
         self.q0 = np.ascontiguousarray(q0, np.float32)
+        self.q0_f64 = np.ascontiguousarray(q0, np.float64)   # for FP64 contexts
         assert self.q0.shape == (3, self.ny, self.nx), self.q0.shape
 
     def cparams(self):
@@ -176,6 +191,11 @@ class Problem:
         if self.moving is not None:
             p.md_on = 1
             p.md_vx, p.md_vy, p.md_t = self.moving
+        elif self.taylor and "mdt" in self.taylor:
+            # the app's MDVX, MDVY, MDT locate the measured cut also when the
+            # window stays put
+            t = self.taylor
+            p.md_vx, p.md_vy, p.md_t = t["mdvx"], t["mdvy"], t["mdt"]
         if self.hifu:
             h = self.hifu
             p.hifu_ampl, p.hifu_frec, p.hifu_a, p.hifu_f = h["ampl"], h["frec"], h["a"], h["f"]
@@ -185,21 +205,26 @@ class Problem:
 class Context:
     """A fivonagi_ctx: one or more instances advanced together on one stream."""
 
-    def __init__(self, problems):
+    def __init__(self, problems, fp64=False):
+        """fp64: state and arithmetic in double (fivonagi_init_f64), from each
+        problem's unrounded initial state."""
         if isinstance(problems, Problem):
             problems = [problems]
         self.problems = list(problems)
+        self.fp64 = bool(fp64)
         lib = load()
         n = len(self.problems)
         parr = (Params * n)(*[p.cparams() for p in self.problems])
         qs = []
         for p in self.problems:
-            qs += [np.ascontiguousarray(p.q0[k]) for k in range(3)]
+            src = p.q0_f64 if self.fp64 else p.q0
+            qs += [np.ascontiguousarray(src[k]) for k in range(3)]
         self._keep = qs
-        qptr = (C.POINTER(C.c_float) * (3 * n))(
-            *[q.ctypes.data_as(C.POINTER(C.c_float)) for q in qs])
+        ct = C.c_double if self.fp64 else C.c_float
+        qptr = (C.POINTER(ct) * (3 * n))(*[q.ctypes.data_as(C.POINTER(ct)) for q in qs])
         h = C.c_void_p()
-        _check(lib.fivonagi_init(C.byref(h), parr, n, qptr))
+        init = lib.fivonagi_init_f64 if self.fp64 else lib.fivonagi_init
+        _check(init(C.byref(h), parr, n, qptr))
         self._h = h
         self._dtseq = {}
 
@@ -250,14 +275,16 @@ class Context:
         return load().fivonagi_sync(self._h)
 
     def collect(self, inst=0):
+        """(q (3, ny, nx), state): float64 for FP64 contexts, else float32."""
         p = self.problems[inst]
-        q = np.zeros((3, p.ny, p.nx), np.float32)
+        dtype, ct = (np.float64, C.c_double) if self.fp64 else (np.float32, C.c_float)
+        fn = load().fivonagi_collect_f64 if self.fp64 else load().fivonagi_collect
+        q = np.zeros((3, p.ny, p.nx), dtype)
         T, dt = C.c_double(), C.c_double()
         na, nr = C.c_int64(), C.c_int64()
         fpp = [np.ascontiguousarray(q[k]) for k in range(3)]
-        _check(load().fivonagi_collect(self._h, inst,
-                                       *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in fpp],
-                                       C.byref(T), C.byref(dt), C.byref(na), C.byref(nr)))
+        _check(fn(self._h, inst, *[a.ctypes.data_as(C.POINTER(ct)) for a in fpp],
+                  C.byref(T), C.byref(dt), C.byref(na), C.byref(nr)))
         return np.stack(fpp), dict(T=T.value, dt=dt.value, n_accepted=na.value,
                                    n_rejected=nr.value)
 
@@ -298,8 +325,11 @@ class Context:
             self._h, inst, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in ls]))
         return np.stack(ls)
 
-    def set_collect(self, inst, jcut=2, mic=(0, 0, 0, 0), eta=1.0, on=True):
-        """Data collection on row jcut (apps/hifu-beam/data_collect.h)."""
+    def set_collect(self, inst, jcut=2, mic=(0, 0, 0, 0), eta=1.0, on=True, cut_half=0,
+                    tan_theta=0.0):
+        """Data collection (apps/hifu-beam/data_collect.h on row jcut; for a
+        Taylor instance apps/taylor-angle-xy/data_collect.h, the cut of
+        2 cut_half + 1 points along the propagation direction)."""
         if not on:
             _check(load().fivonagi_set_collect(self._h, inst, None))
             return
@@ -307,13 +337,22 @@ class Context:
         o.jcut, o.eta = int(jcut), float(eta)
         for k in range(4):
             o.mic[k] = int(mic[k])
+        o.cut_half, o.tan_theta = int(cut_half), float(tan_theta)
         _check(load().fivonagi_set_collect(self._h, inst, C.byref(o)))
 
-    def measure(self, inst=0):
-        """The measurement grid (ny, nx) of instance inst."""
+    def measure(self, inst=0, rows=None):
+        """The measurement grid (ny, nx) of instance inst (float64 in an FP64
+        context); rows: only the first rows rows."""
         p = self.problems[inst]
-        m = np.zeros((p.ny, p.nx), np.float32)
-        _check(load().fivonagi_measure(self._h, inst, m.ctypes.data_as(C.POINTER(C.c_float))))
+        rows = p.ny if rows is None else int(rows)
+        if self.fp64:
+            m = np.zeros((rows, p.nx), np.float64)
+            _check(load().fivonagi_measure_rows_f64(self._h, inst, rows,
+                                                    m.ctypes.data_as(C.POINTER(C.c_double))))
+        else:
+            m = np.zeros((rows, p.nx), np.float32)
+            _check(load().fivonagi_measure_rows(self._h, inst, rows,
+                                                m.ctypes.data_as(C.POINTER(C.c_float))))
         return m
 
     def set_skew(self, skew, rotate=False):
@@ -326,3 +365,24 @@ class Context:
         h = C.c_int32()
         _check(load().fivonagi_autotune(self._h, int(trial_steps), C.byref(h)))
         return h.value
+
+
+def cut_errors(numeric, analytic):
+    """Relative L1, L2, Linf errors of a Taylor cut as the reference's export
+    computes them (float accumulators; fivonagi_cut_errors).  numeric and
+    analytic: the cut's entries, float32 (FP32 run) or float64 (FP64 run)."""
+    numeric = np.ascontiguousarray(numeric)
+    analytic = np.ascontiguousarray(analytic, numeric.dtype)
+    e = np.zeros(3, np.float32)
+    ef = e.ctypes.data_as(C.POINTER(C.c_float))
+    if numeric.dtype == np.float64:
+        dp = C.POINTER(C.c_double)
+        _check(load().fivonagi_cut_errors_f64(numeric.ctypes.data_as(dp),
+                                              analytic.ctypes.data_as(dp), len(numeric), ef))
+    else:
+        numeric = numeric.astype(np.float32)
+        analytic = analytic.astype(np.float32)
+        fp = C.POINTER(C.c_float)
+        _check(load().fivonagi_cut_errors(numeric.ctypes.data_as(fp),
+                                          analytic.ctypes.data_as(fp), len(numeric), ef))
+    return e

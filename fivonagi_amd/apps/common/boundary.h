@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../na/solver.h"
+
 namespace fvbc {
 
 struct Source {       // where a ghost cell's value comes from
@@ -52,12 +54,12 @@ __device__ __forceinline__ Source map(int kind, int i, int j, int nx, int ny) {
 // Left-edge drive: p = amp*sin(theta + phase) = ampc*sin(theta) + amps*cos(theta)
 // with ampc = amp*cos(phase), amps = amp*sin(phase) precomputed per row and
 // sin/cos(theta = omega*T) computed once per step.
-__device__ __forceinline__ void drive(float ampc, float amps, float sin_t, float cos_t,
-                                      float& q1, float& q2, float& q3) {
-  const float p = fmaf(ampc, sin_t, amps * cos_t);
+template <typename R>
+__device__ __forceinline__ void drive(R ampc, R amps, R sin_t, R cos_t, R& q1, R& q2, R& q3) {
+  const R p = fvna::fma_(ampc, sin_t, amps * cos_t);
   q1 = p;
-  q2 = p * (p + 1.0f);
-  q3 = 0.0f;
+  q2 = p * (p + R(1));
+  q3 = R(0);
 }
 
 // HIFU focused packet (apps/hifu-beam/boundary.h:10-17, 84-93):
@@ -65,26 +67,26 @@ __device__ __forceinline__ void drive(float ampc, float amps, float sin_t, float
 // AMPL sin(ISPW (T + phi)) = ampc sin(ISPW T) + amps cos(ISPW T) from the
 // per-row ampc = AMPL cos(ISPW phi), amps = AMPL sin(ISPW phi);
 // u1 = pack, (u2, u3) = nu pack (pack + 1).
-__device__ __forceinline__ void hifu_packet(float ampc, float amps, float sin_t, float cos_t,
-                                            float opT, float nux, float nuy, float& q1,
-                                            float& q2, float& q3) {
-  const float o2 = opT * opT, o4 = o2 * o2, o8 = o4 * o4;
-  const float p = fmaf(ampc, sin_t, amps * cos_t) * expf(-(o8 * o2));
-  const float pp = p * (p + 1.0f);
+template <typename R>
+__device__ __forceinline__ void hifu_packet(R ampc, R amps, R sin_t, R cos_t, R opT, R nux, R nuy,
+                                            R& q1, R& q2, R& q3) {
+  const R o2 = opT * opT, o4 = o2 * o2, o8 = o4 * o4;
+  const R p = fvna::fma_(ampc, sin_t, amps * cos_t) * exp(-(o8 * o2));
+  const R pp = p * (p + R(1));
   q1 = p;
   q2 = nux * pp;
   q3 = nuy * pp;
 }
 
+template <typename R>
 __device__ __forceinline__ void taylor(double rhoa, double c, double cs, double sn,
                                        double yoff, int jc, double dx, double dy,
-                                       double T, int i, int j, float& q1, float& q2,
-                                       float& q3) {
+                                       double T, int i, int j, R& q1, R& q2, R& q3) {
   const double arg = c * ((double)i * dx * cs + ((double)(j - jc) * dy + yoff) * sn - T);
   const double r = rhoa * tanh(arg);
-  q1 = (float)r;
-  q2 = (float)(r * cs);
-  q3 = (float)(r * sn);
+  q1 = (R)r;
+  q2 = (R)(r * cs);
+  q3 = (R)(r * sn);
 }
 
 }  // namespace fvbc

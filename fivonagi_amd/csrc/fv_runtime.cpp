@@ -23,13 +23,13 @@
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
-                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int md,
-                                     hipStream_t stream);
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int ext,
+                                     int fp64, hipStream_t stream);
 extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, int phase,
                                        hipStream_t stream);
 extern "C" hipError_t fv_launch_collect(const fv::InstDesc* insts, int n_inst, int max_nx,
-                                        int phase, hipStream_t stream);
-extern "C" hipError_t fv_step_waves_per_cu(int* waves);
+                                        int phase, int fp64, hipStream_t stream);
+extern "C" hipError_t fv_step_waves_per_cu(int* waves, int fp64);
 extern "C" hipError_t fv_adj_march_forward(const fv::AdjDesc* ds, const fv::AdjItem* items,
                                            int n_items, int k, int slot, hipStream_t s);
 extern "C" hipError_t fv_adj_march_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items_x,
@@ -92,6 +92,7 @@ struct fivonagi_ctx {
   std::vector<fv::InstDesc> desc;          // host copy of the instance table
   std::vector<fv::WorkItem> items;
   std::vector<uint64_t> rowmask;           // CFL lane mask per item row (fv::WorkItem::mrow)
+  bool fp64 = false;                       // state and arithmetic in double
   bool md_any = false;                     // some instance has a moving window
   bool ext_any = false;                    // ... or the HIFU boundary (extended kernel)
   float skew = -1.0f;                      // height skew of the work items (< 0: automatic)
@@ -116,6 +117,13 @@ struct fivonagi_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<float*> adj_lam;             // dJ/dq0 of the last adjoint call (per instance)
+  struct CollectBufs { void* measure = nullptr; void* stage = nullptr; };
+  std::vector<CollectBufs> collect_bufs;   // per instance, kept once allocated
+  uint32_t* d_collect_err = nullptr;       // collection overflow flag per instance
+  // control blocks as the host last read them; valid until the next launch
+  // or host write (read_ctls, launch_attempts, write_ctls)
+  std::vector<fv::Ctl> ctl_cache;
+  bool ctl_cache_ok = false;
 
   void* dalloc(size_t bytes, hipError_t* e) {
     void* p = nullptr;
@@ -160,12 +168,12 @@ int auto_rows(const std::vector<fivonagi_params>& ps, long target_waves) {
 }
 
 // CUs of the device and resident step waves per CU (4 SIMDs per CU)
-void device_shape(int device, long* cus, int* per_cu) {
+void device_shape(int device, bool fp64, long* cus, int* per_cu) {
   int n = 0, w = 0;
   if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      fv_step_waves_per_cu(&w) != hipSuccess || n <= 0 || w <= 0) {
-    n = 256;   // MI355X: 256 CUs, 4 waves of the step kernel per SIMD
-    w = 16;
+      fv_step_waves_per_cu(&w, fp64 ? 1 : 0) != hipSuccess || n <= 0 || w <= 0) {
+    n = 256;   // MI355X: 256 CUs, 4 waves of the FP32 step kernel per SIMD, 2 of FP64
+    w = fp64 ? 8 : 16;
   }
   *cus = n;
   *per_cu = w;
@@ -290,6 +298,7 @@ void build_items(fivonagi_ctx* c, int auto_h, float skew, bool rotate) {
 // old grid size, so it is dropped
 int upload_items(fivonagi_ctx* c) {
   c->drop_graph();
+  c->ctl_cache_ok = false;
   if (c->d_items) (void)hipFree(c->d_items);
   c->d_items = nullptr;
   if (c->d_rowmask) (void)hipFree(c->d_rowmask);
@@ -307,6 +316,7 @@ int upload_items(fivonagi_ctx* c) {
 }
 
 int upload_tables(fivonagi_ctx* c) {
+  c->ctl_cache_ok = false;   // the settled control blocks depend on the descriptors
   HIP_TRY(hipMemcpyAsync(c->d_desc, c->desc.data(), sizeof(fv::InstDesc) * c->n_inst,
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -329,6 +339,10 @@ void set_ctl_derived(fv::Ctl& t, const fv::InstDesc& d) {
 // folds that attempt's reduction into block (a+1)%2 (a = next attempt index),
 // which is where the next launch will store the same result.
 int read_ctls(fivonagi_ctx* c, std::vector<fv::Ctl>& out) {
+  if (c->ctl_cache_ok) {
+    out = c->ctl_cache;
+    return 0;
+  }
   const int phase = (int)(c->attempts % 6);
   HIP_TRY(fv_launch_settle(c->d_desc, c->n_inst, phase, c->stream));
   std::vector<fv::Ctl> both(2 * (size_t)c->n_inst);
@@ -338,12 +352,15 @@ int read_ctls(fivonagi_ctx* c, std::vector<fv::Ctl>& out) {
   out.resize(c->n_inst);
   const int b = (phase + 1) & 1;
   for (int k = 0; k < c->n_inst; ++k) out[k] = both[2 * (size_t)k + b];
+  c->ctl_cache = out;
+  c->ctl_cache_ok = true;
   return 0;
 }
 
 // Replace the control blocks the next attempt starts from: block a%2, with
 // the reduction of attempt a-1 marked as already folded in.
 int write_ctls(fivonagi_ctx* c, const std::vector<fv::Ctl>& in) {
+  c->ctl_cache_ok = false;
   const int phase = (int)(c->attempts % 6);
   const uint32_t zero = 0u;
   for (int k = 0; k < c->n_inst; ++k) {
@@ -367,23 +384,27 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
   }
   hipError_t e = fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(),
                                 phase, c->n_inst, c->d_ctl, c->d_sync, c->ext_any ? 1 : 0,
-                                c->stream);
+                                c->fp64 ? 1 : 0, c->stream);
   if (e != hipSuccess || c->collect_nx == 0) return e;
-  return fv_launch_collect(c->d_desc, c->n_inst, c->collect_nx, phase, c->stream);
+  return fv_launch_collect(c->d_desc, c->n_inst, c->collect_nx, phase, c->fp64 ? 1 : 0,
+                           c->stream);
 }
 
-// data collection overflow flags (any instance): FIVONAGI_ERR_COLLECT or 0
+// data collection overflow flags (any instance collecting): FIVONAGI_ERR_COLLECT or 0
 int collect_status(fivonagi_ctx* c) {
-  for (int k = 0; k < c->n_inst; ++k) {
-    if (!c->desc[k].collect_err || !c->desc[k].measure) continue;
-    uint32_t v = 0;
-    HIP_TRY(hipMemcpy(&v, c->desc[k].collect_err, sizeof(v), hipMemcpyDeviceToHost));
-    if (v) return fail(FIVONAGI_ERR_COLLECT, "data collection: more than nx steps in a frame");
-  }
+  if (!c->d_collect_err) return 0;
+  std::vector<uint32_t> v(c->n_inst);
+  HIP_TRY(hipMemcpyAsync(v.data(), c->d_collect_err, sizeof(uint32_t) * c->n_inst,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->n_inst; ++k)
+    if (v[k] && c->desc[k].measure)
+      return fail(FIVONAGI_ERR_COLLECT, "data collection: more than nx steps in a frame");
   return 0;
 }
 
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
+  c->ctl_cache_ok = false;
   const int G = 30;   // attempts per captured graph (a multiple of 6: phases 0..5 repeat)
   // single launches up to a phase-0 boundary, then whole graphs, then singles
   while (n > 0 && (c->attempts % 6 != 0 || n < G)) {
@@ -446,8 +467,9 @@ extern "C" {
 
 const char* fivonagi_last_error(void) { return g_err.c_str(); }
 
-int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
-                  const float* const* q_host) {
+// fivonagi_init / fivonagi_init_f64: q_host holds float or double grids.
+static int init_impl(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
+                     const void* const* q_host, bool f64) {
   if (!out || !p || !q_host || n_inst <= 0) return fail(FIVONAGI_ERR_ARG, "null argument");
   *out = nullptr;
   for (int k = 0; k < n_inst; ++k) {
@@ -470,6 +492,8 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   }
   auto* c = new fivonagi_ctx();
   c->n_inst = n_inst;
+  c->fp64 = f64;
+  const size_t es = f64 ? sizeof(double) : sizeof(float);
   c->params.assign(p, p + n_inst);
   c->desc.resize(n_inst);
   c->dt_seq.resize(n_inst);
@@ -493,22 +517,26 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
     auto& d = c->desc[k];
     std::memset(&d, 0, sizeof(d));
     const size_t n = (size_t)q.nx * q.ny;
-    float* buf = (float*)c->dalloc(sizeof(float) * n * 6, &e);
+    char* buf = (char*)c->dalloc(es * n * 6, &e);
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc state"); }
     for (int b = 0; b < 2; ++b)
-      for (int f = 0; f < 3; ++f) d.q[b][f] = buf + (size_t)(3 * b + f) * n;
+      for (int f = 0; f < 3; ++f) d.q[b][f] = (float*)(buf + es * (size_t)(3 * b + f) * n);
     for (int f = 0; f < 3; ++f) {
-      e = hipMemcpy(d.q[0][f], q_host[3 * k + f], sizeof(float) * n, hipMemcpyHostToDevice);
+      e = hipMemcpy(d.q[0][f], q_host[3 * k + f], es * n, hipMemcpyHostToDevice);
       if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy state"); }
-      e = hipMemcpy(d.q[1][f], q_host[3 * k + f], sizeof(float) * n, hipMemcpyHostToDevice);
+      e = hipMemcpy(d.q[1][f], q_host[3 * k + f], es * n, hipMemcpyHostToDevice);
       if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy state"); }
     }
+    d.fp64 = f64 ? 1 : 0;
+    d.hbeta_d = q.beta - 1.0;
+    d.hifu_k_d = q.hifu_frec / 4.0;
     // edge drive amplitude/phase folded into amp*cos(phase), amp*sin(phase);
     // the HIFU aperture's AMPL and phase delay ISPW*phi_j the same way
-    std::vector<float> ec(q.ny, 0.f), es(q.ny, 0.f), tab;
+    // (double here, uploaded in the context's precision)
+    std::vector<double> ec(q.ny, 0.0), es_(q.ny, 0.0), tab;
     const bool hifu = q.bc == FIVONAGI_BC_HIFU;
     const double ispw = 2.0 * 3.14159265358979323846 * q.hifu_frec;   // ISPW
-    if (hifu) tab.assign(4 * (size_t)q.ny, 0.f);
+    if (hifu) tab.assign(4 * (size_t)q.ny, 0.0);
     for (int j = 0; j < q.ny; ++j) {
       if (hifu) {
         // apps/hifu-beam/boundary.h:79-90
@@ -518,38 +546,40 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
           const double F = q.hifu_f, jp2 = (double)jp * jp;
           const double nd = std::sqrt(jp2 + F * F);
           const double phi = (q.dx * jp2) / (2.0 * F);
-          ec[j] = (float)(q.hifu_ampl * std::cos(ispw * phi));
-          es[j] = (float)(q.hifu_ampl * std::sin(ispw * phi));
-          tab[4 * (size_t)j + 0] = (float)phi;
-          tab[4 * (size_t)j + 1] = (float)(F / nd);
-          tab[4 * (size_t)j + 2] = (float)(-jp / nd);
-          tab[4 * (size_t)j + 3] = 1.f;
+          ec[j] = q.hifu_ampl * std::cos(ispw * phi);
+          es_[j] = q.hifu_ampl * std::sin(ispw * phi);
+          tab[4 * (size_t)j + 0] = phi;
+          tab[4 * (size_t)j + 1] = F / nd;
+          tab[4 * (size_t)j + 2] = -jp / nd;
+          tab[4 * (size_t)j + 3] = 1.0;
         }
         continue;
       }
       const double a = q.edge_amp ? q.edge_amp[j] : 0.0;
       const double ph = q.edge_phase ? q.edge_phase[j] : 0.0;
-      ec[j] = (float)(a * std::cos(ph));
-      es[j] = (float)(a * std::sin(ph));
+      ec[j] = a * std::cos(ph);
+      es_[j] = a * std::sin(ph);
     }
+    // a table in the context's precision on the device
+    auto upload = [&](const std::vector<double>& v, const void** dst) -> hipError_t {
+      hipError_t r;
+      void* t = c->dalloc(es * v.size(), &r);
+      if (r != hipSuccess) return r;
+      *dst = t;
+      if (f64) return hipMemcpy(t, v.data(), es * v.size(), hipMemcpyHostToDevice);
+      std::vector<float> vf(v.begin(), v.end());
+      return hipMemcpy(t, vf.data(), es * v.size(), hipMemcpyHostToDevice);
+    };
     d.hifu_tab = nullptr;
     d.hifu_k = (float)(q.hifu_frec / 4.0);
     if (hifu) {
-      float* t = (float*)c->dalloc(sizeof(float) * tab.size(), &e);
-      if (e == hipSuccess)
-        e = hipMemcpy(t, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice);
+      e = upload(tab, &d.hifu_tab);
       if (e != hipSuccess) { delete c; return hip_fail(e, "hifu table"); }
-      d.hifu_tab = t;
       c->ext_any = true;
     }
-    float* edge = (float*)c->dalloc(sizeof(float) * 2 * q.ny, &e);
-    if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc edge"); }
-    e = hipMemcpy(edge, ec.data(), sizeof(float) * q.ny, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(edge + q.ny, es.data(), sizeof(float) * q.ny, hipMemcpyHostToDevice);
-    if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy edge"); }
-    d.edge_c = edge;
-    d.edge_s = edge + q.ny;
+    e = upload(ec, &d.edge_c);
+    if (e == hipSuccess) e = upload(es_, &d.edge_s);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "edge tables"); }
     // CFL subset
     d.cfl_mask = fv::kCflAll;
     if (q.cfl_mode == FIVONAGI_CFL_REFERENCE_FOLD) {
@@ -599,7 +629,7 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * 2 * n_inst, hipMemcpyHostToDevice);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
 
-  device_shape(c->device, &c->cus, &c->per_cu);
+  device_shape(c->device, c->fp64, &c->cus, &c->per_cu);
   build_items(c, auto_rows(c->params, c->cus * c->per_cu), c->skew, c->rotate);
   c->d_desc = (fv::InstDesc*)c->dalloc(sizeof(fv::InstDesc) * n_inst, &e);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc desc"); }
@@ -607,6 +637,16 @@ int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   if (rc) { delete c; return rc; }
   *out = c;
   return 0;
+}
+
+int fivonagi_init(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
+                  const float* const* q_host) {
+  return init_impl(out, p, n_inst, (const void* const*)q_host, false);
+}
+
+int fivonagi_init_f64(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
+                      const double* const* q_host) {
+  return init_impl(out, p, n_inst, (const void* const*)q_host, true);
 }
 
 int fivonagi_set_stream(fivonagi_ctx* c, void* s) {
@@ -659,11 +699,12 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
   };
   for (int k = 0; k < c->n_inst; ++k) {
     const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
-    hipError_t e = hipMalloc(&keep[k], sizeof(float) * 3 * n);
+    const size_t es = c->fp64 ? sizeof(double) : sizeof(float);
+    hipError_t e = hipMalloc(&keep[k], es * 3 * n);
     if (e != hipSuccess) { release(); return hip_fail(e, "hipMalloc (autotune)"); }
     for (int f = 0; f < 3; ++f)
-      (void)hipMemcpyAsync(keep[k] + (size_t)f * n, c->desc[k].q[saved[k].cur][f],
-                           sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
+      (void)hipMemcpyAsync((char*)keep[k] + es * (size_t)f * n, c->desc[k].q[saved[k].cur][f],
+                           es * n, hipMemcpyDeviceToDevice, c->stream);
   }
   const int auto_h = auto_rows(c->params, c->cus * c->per_cu);
   std::vector<int> cands = {auto_h, 8, 12, 16, 20, 24, 32, 40, 48, 64, 96, 128};
@@ -717,8 +758,9 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
   for (int k = 0; k < c->n_inst; ++k) {
     const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
     for (int f = 0; f < 3; ++f)
-      (void)hipMemcpyAsync(c->desc[k].q[saved[k].cur][f], keep[k] + (size_t)f * n,
-                           sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
+      (void)hipMemcpyAsync(c->desc[k].q[saved[k].cur][f],
+                           (char*)keep[k] + (c->fp64 ? 8 : 4) * (size_t)f * n,
+                           (c->fp64 ? 8 : 4) * n, hipMemcpyDeviceToDevice, c->stream);
   }
   hipError_t es = hipStreamSynchronize(c->stream);
   int rc3 = (rc || rc2 || es != hipSuccess) ? 0 : write_ctls(c, saved);
@@ -825,60 +867,165 @@ int fivonagi_set_collect(fivonagi_ctx* c, int inst, const fivonagi_collect_opts*
   if (int rc = check_ctx(c, inst)) return rc;
   auto& d = c->desc[inst];
   const int nx = d.nx, ny = d.ny;
+  const size_t es = c->fp64 ? sizeof(double) : sizeof(float);
+  const bool taylor = d.bc == fv::kBcTaylor;
   if (o) {
-    if (o->jcut < 0 || o->jcut >= ny || !(o->eta > 0))
-      return fail(FIVONAGI_ERR_ARG, "collect: jcut outside the grid or eta <= 0");
-    if (!d.measure) {
+    if (!(o->eta > 0)) return fail(FIVONAGI_ERR_ARG, "collect: eta <= 0");
+    if (!taylor && (o->jcut < 0 || o->jcut >= ny))
+      return fail(FIVONAGI_ERR_ARG, "collect: jcut outside the grid");
+    if (taylor && (o->cut_half < 0 || 2 * (int64_t)o->cut_half >= nx))
+      return fail(FIVONAGI_ERR_ARG, "collect: the Taylor cut (2 cut_half + 1 entries) must fit a row");
+    if (c->collect_bufs.size() != (size_t)c->n_inst) c->collect_bufs.resize(c->n_inst);
+    auto& b = c->collect_bufs[inst];
+    if (!c->d_collect_err) {
       hipError_t e;
-      d.measure = (float*)c->dalloc(sizeof(float) * (size_t)nx * ny, &e);
-      if (e == hipSuccess) d.stage = (float*)c->dalloc(sizeof(float) * (size_t)nx, &e);
-      if (e == hipSuccess) d.collect_err = (uint32_t*)c->dalloc(sizeof(uint32_t), &e);
+      c->d_collect_err = (uint32_t*)c->dalloc(sizeof(uint32_t) * c->n_inst, &e);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc (collect)");
+      HIP_TRY(hipMemsetAsync(c->d_collect_err, 0, sizeof(uint32_t) * c->n_inst, c->stream));
+    }
+    if (!b.measure) {
+      hipError_t e;
+      b.measure = c->dalloc(es * (size_t)nx * ny, &e);
+      if (e == hipSuccess) b.stage = c->dalloc(es * (size_t)nx, &e);
       if (e != hipSuccess) return hip_fail(e, "hipMalloc (collect)");
     }
-    HIP_TRY(hipMemsetAsync(d.measure, 0, sizeof(float) * (size_t)nx * ny, c->stream));
-    HIP_TRY(hipMemsetAsync(d.stage, 0, sizeof(float) * (size_t)nx, c->stream));
-    HIP_TRY(hipMemsetAsync(d.collect_err, 0, sizeof(uint32_t), c->stream));
+    HIP_TRY(hipMemsetAsync(b.measure, 0, es * (size_t)nx * ny, c->stream));
+    HIP_TRY(hipMemsetAsync(b.stage, 0, es * (size_t)nx, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_collect_err + inst, 0, sizeof(uint32_t), c->stream));
+    d.measure = b.measure;
+    d.stage = taylor ? nullptr : b.stage;   // the Taylor cut reads the output state
+    d.collect_err = c->d_collect_err + inst;
     d.jcut = o->jcut;
     d.eta = o->eta;
     for (int k = 0; k < 4; ++k) d.mic[k] = o->mic[k];
+    d.cut_half = o->cut_half;
+    d.tan_theta = o->tan_theta;
     c->collect_nx = std::max(c->collect_nx, nx);
-    c->ext_any = true;
+    if (!taylor) c->ext_any = true;        // the staged row comes from the extended kernel
   } else {
-    d.measure = nullptr;   // the buffers stay allocated until destroy
+    d.measure = nullptr;   // the buffers stay allocated for a later call
     d.stage = nullptr;
   }
   c->drop_graph();         // the captured launches change
   return upload_tables(c);
 }
 
-int fivonagi_measure(fivonagi_ctx* c, int inst, float* out) {
+}  // extern "C"
+
+// measurement grid of instance inst to a host array of type H (the
+// context's precision)
+template <typename H>
+static int measure_impl(fivonagi_ctx* c, int inst, int rows, H* out) {
   if (int rc = check_ctx(c, inst)) return rc;
   if (!out) return fail(FIVONAGI_ERR_ARG, "null output");
+  if (c->fp64 != (sizeof(H) == sizeof(double)))
+    return fail(FIVONAGI_ERR_ARG, c->fp64 ? "FP64 context: use the _f64 form"
+                                          : "FP32 context: use the float form");
   const auto& d = c->desc[inst];
   if (!d.measure) return fail(FIVONAGI_ERR_ARG, "data collection is off for this instance");
+  if (rows < 0 || rows > d.ny) return fail(FIVONAGI_ERR_ARG, "rows outside 0..ny");
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(out, d.measure, sizeof(float) * (size_t)d.nx * d.ny,
-                    hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out, d.measure, sizeof(H) * (size_t)d.nx * rows, hipMemcpyDeviceToHost));
   return collect_status(c);
 }
 
-int fivonagi_collect(fivonagi_ctx* c, int inst, float* q1, float* q2, float* q3, double* T,
-                     double* dt, int64_t* n_acc, int64_t* n_rej) {
+// The relative errors of apps/taylor-angle-xy/data_export.h:96-122 in its own
+// types: float accumulators, |a - n| and the maxima in the state's type T,
+// squares through pow(T, 2) (double), sqrt of the float ratio in double.
+template <typename T>
+static int cut_errors_impl(const T* numeric, const T* analytic, int length, float* e) {
+  if (!numeric || !analytic || !e || length < 0) return fail(FIVONAGI_ERR_ARG, "cut errors");
+  float l1a = 0, l1b = 0, l2a = 0, l2b = 0, lia = 0, lib = 0;
+  for (int i = 0; i < length; ++i) {
+    const T d = std::abs(analytic[i] - numeric[i]);
+    const T a = std::abs(analytic[i]);
+    l1a += d;
+    l1b += a;
+    l2a += (double)d * (double)d;
+    l2b += (double)a * (double)a;
+    lia = (float)(d > (T)lia ? d : (T)lia);
+    lib = (float)(a > (T)lib ? a : (T)lib);
+  }
+  e[0] = l1a / l1b;
+  e[1] = (float)std::sqrt((double)(l2a / l2b));
+  e[2] = lia / lib;
+  return 0;
+}
+
+// state of instance inst to host arrays of type H (float or double),
+// converting when the context's precision differs
+template <typename H>
+static int collect_impl(fivonagi_ctx* c, int inst, H* q1, H* q2, H* q3, double* T, double* dt,
+                        int64_t* n_acc, int64_t* n_rej) {
   if (int rc = check_ctx(c, inst)) return rc;
   std::vector<fv::Ctl> ctl;
   if (int rc = read_ctls(c, ctl)) return rc;
   const auto& t = ctl[inst];
   const auto& d = c->desc[inst];
   const size_t n = (size_t)d.nx * d.ny;
-  float* dst[3] = {q1, q2, q3};
-  for (int f = 0; f < 3; ++f)
-    if (dst[f])
-      HIP_TRY(hipMemcpy(dst[f], d.q[t.cur][f], sizeof(float) * n, hipMemcpyDeviceToHost));
+  H* dst[3] = {q1, q2, q3};
+  const bool same = c->fp64 == (sizeof(H) == sizeof(double));
+  std::vector<double> vd;
+  std::vector<float> vf;
+  for (int f = 0; f < 3; ++f) {
+    if (!dst[f]) continue;
+    if (same) {
+      HIP_TRY(hipMemcpy(dst[f], d.q[t.cur][f], sizeof(H) * n, hipMemcpyDeviceToHost));
+    } else if (c->fp64) {
+      vd.resize(n);
+      HIP_TRY(hipMemcpy(vd.data(), d.q[t.cur][f], sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (size_t o = 0; o < n; ++o) dst[f][o] = (H)vd[o];
+    } else {
+      vf.resize(n);
+      HIP_TRY(hipMemcpy(vf.data(), d.q[t.cur][f], sizeof(float) * n, hipMemcpyDeviceToHost));
+      for (size_t o = 0; o < n; ++o) dst[f][o] = (H)vf[o];
+    }
+  }
   if (T) *T = t.T;
   if (dt) *dt = t.dt;
   if (n_acc) *n_acc = t.n_acc;
   if (n_rej) *n_rej = t.n_rej;
   return 0;
+}
+
+extern "C" {
+
+int fivonagi_collect(fivonagi_ctx* c, int inst, float* q1, float* q2, float* q3, double* T,
+                     double* dt, int64_t* n_acc, int64_t* n_rej) {
+  return collect_impl<float>(c, inst, q1, q2, q3, T, dt, n_acc, n_rej);
+}
+
+int fivonagi_collect_f64(fivonagi_ctx* c, int inst, double* q1, double* q2, double* q3,
+                         double* T, double* dt, int64_t* n_acc, int64_t* n_rej) {
+  return collect_impl<double>(c, inst, q1, q2, q3, T, dt, n_acc, n_rej);
+}
+
+int fivonagi_measure(fivonagi_ctx* c, int inst, float* out) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  return measure_impl<float>(c, inst, c->desc[inst].ny, out);
+}
+
+int fivonagi_measure_f64(fivonagi_ctx* c, int inst, double* out) {
+  if (int rc = check_ctx(c, inst)) return rc;
+  return measure_impl<double>(c, inst, c->desc[inst].ny, out);
+}
+
+int fivonagi_measure_rows(fivonagi_ctx* c, int inst, int rows, float* out) {
+  return measure_impl<float>(c, inst, rows, out);
+}
+
+int fivonagi_measure_rows_f64(fivonagi_ctx* c, int inst, int rows, double* out) {
+  return measure_impl<double>(c, inst, rows, out);
+}
+
+int fivonagi_cut_errors(const float* numeric, const float* analytic, int length,
+                        float errors[3]) {
+  return cut_errors_impl<float>(numeric, analytic, length, errors);
+}
+
+int fivonagi_cut_errors_f64(const double* numeric, const double* analytic, int length,
+                            float errors[3]) {
+  return cut_errors_impl<double>(numeric, analytic, length, errors);
 }
 
 int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
@@ -900,6 +1047,7 @@ int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
   info->rotate = c->rotate_used ? 1 : 0;
   info->md_x = t.mdx;
   info->md_y = t.mdy;
+  info->fp64 = c->fp64 ? 1 : 0;
   // mean rows per work item (heights differ when the split is skewed)
   const auto& p = c->params[inst];
   const int strips = (p.nx + fv::kStripCols - 1) / fv::kStripCols;
@@ -939,6 +1087,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   if (!c || !obj || !J || nsteps <= 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
   if (nsteps > (1 << 30)) return fail(FIVONAGI_ERR_ARG, "too many steps");
   if (c->md_any) return fail(FIVONAGI_ERR_ARG, "the adjoint does not support a moving window");
+  if (c->fp64) return fail(FIVONAGI_ERR_ARG, "the adjoint runs on FP32 contexts");
   for (const auto& p : c->params)
     if (p.bc == FIVONAGI_BC_HIFU)
       return fail(FIVONAGI_ERR_ARG, "the adjoint does not support the HIFU boundary");

@@ -41,6 +41,13 @@ namespace fv {
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) const float cgfloat;
 typedef __attribute__((address_space(4))) const float kfloat;   // read-only: scalar loads
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const double cgdouble;
+typedef __attribute__((address_space(4))) const double kdouble;
+// the same three per real type R (float: the FP32 path, double: FP64)
+template <typename R> struct Mem;
+template <> struct Mem<float> { typedef gfloat g; typedef cgfloat cg; typedef kfloat k; };
+template <> struct Mem<double> { typedef gdouble g; typedef cgdouble cg; typedef kdouble k; };
 
 // DPP cross-lane moves over the whole wave64 (gfx9 wave_shr:1 / wave_shl:1).
 // shr1: lane l receives lane l-1's value (lane 0 receives 0).
@@ -53,6 +60,17 @@ __device__ __forceinline__ float shr1(float v) {
 __device__ __forceinline__ float shl1(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
+// 64-bit values move as two 32-bit halves
+__device__ __forceinline__ double dpp64(double v, int ctrl_shr) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)(ctrl_shr ? __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x138, 0xf, 0xf, true)
+                                          : __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x130, 0xf, 0xf, true));
+  const uint32_t hi = (uint32_t)(ctrl_shr ? __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x138, 0xf, 0xf, true)
+                                          : __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x130, 0xf, 0xf, true));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ double shr1(double v) { return dpp64(v, 1); }
+__device__ __forceinline__ double shl1(double v) { return dpp64(v, 0); }
 
 typedef __attribute__((address_space(1))) char gchar;
 typedef __attribute__((address_space(1))) const char cgchar;
@@ -65,6 +83,12 @@ __device__ __forceinline__ float ldg(cgfloat* base, uint32_t off) {
 }
 __device__ __forceinline__ void stg(gfloat* base, uint32_t off, float v) {
   *(gfloat*)((gchar*)base + off) = v;
+}
+__device__ __forceinline__ double ldg(cgdouble* base, uint32_t off) {
+  return *(cgdouble*)((cgchar*)base + off);
+}
+__device__ __forceinline__ void stg(gdouble* base, uint32_t off, double v) {
+  *(gdouble*)((gchar*)base + off) = v;
 }
 
 // s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding)
@@ -80,6 +104,9 @@ __device__ __forceinline__ int64_t uni64(int64_t v) {
 __device__ __forceinline__ float unif(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
+__device__ __forceinline__ double unif(double v) {
+  return __longlong_as_double(uni64(__double_as_longlong(v)));
+}
 
 // v on the lanes whose bit of the uniform 64-bit mask is set, 0 elsewhere
 // (one v_cndmask with the mask as the lane condition).
@@ -88,17 +115,28 @@ __device__ __forceinline__ float lane_sel(uint64_t mask, float v) {
   asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(d) : "v"(v), "s"(mask));
   return d;
 }
+__device__ __forceinline__ double lane_sel(uint64_t mask, double v) {
+  return ((mask >> __lane_id()) & 1ull) ? v : 0.0;
+}
 
 // Pins a value in a register here: its computation cannot sink into a
 // branch that follows, so a cross-lane move feeding it stays in its block
 // and folds into its user.
 __device__ __forceinline__ void pin(float& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
   return v;
 }
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double fmax_(double a, double b) { return fmax(a, b); }
 
 __device__ __forceinline__ uint32_t amax_relaxed(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -110,23 +148,70 @@ __device__ __forceinline__ uint32_t axchg_relaxed(uint32_t* p, uint32_t v) {
   return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The per-group cfl maximum of an instance, as float or double bits
+// (non-negative values order like their bit patterns).
+template <typename R> struct CflSlot;
+template <> struct CflSlot<float> {
+  typedef uint32_t bits;
+  static __device__ __forceinline__ bits load(const GroupSlot& g) { return g.cfl_bits; }
+  static __device__ __forceinline__ void clear(GroupSlot& g) { g.cfl_bits = 0u; }
+  static __device__ __forceinline__ void fold_in(GroupSlot& g, float v) {
+    __hip_atomic_fetch_max(&g.cfl_bits, __float_as_uint(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+  static __device__ __forceinline__ float value(bits b) { return __uint_as_float(b); }
+  static __device__ __forceinline__ bits lane_max(bits g) {
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) m = max(m, (uint32_t)__builtin_amdgcn_readlane(g, k));
+    return m;
+  }
+};
+template <> struct CflSlot<double> {
+  typedef uint64_t bits;
+  static __device__ __forceinline__ bits load(const GroupSlot& g) { return g.cfl_bits64; }
+  static __device__ __forceinline__ void clear(GroupSlot& g) { g.cfl_bits64 = 0ull; }
+  static __device__ __forceinline__ void fold_in(GroupSlot& g, double v) {
+    __hip_atomic_fetch_max(&g.cfl_bits64, (uint64_t)__double_as_longlong(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+  static __device__ __forceinline__ double value(bits b) {
+    return __longlong_as_double((long long)b);
+  }
+  static __device__ __forceinline__ bits lane_max(bits g) {
+    uint64_t m = 0ull;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) {
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)g, k);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(g >> 32), k);
+      m = max(m, ((uint64_t)hi << 32) | lo);
+    }
+    return m;
+  }
+};
+
 // What attempt a's reduction slots say about it, with the control block it
 // ran with.  fold_load issues every load before anything is decided, so the
 // fold costs one memory round trip.
+// R: the instance's real type (its cfl reduction is in R).
+template <typename R>
 struct FoldRaw {
   Ctl p;          // control block attempt a ran with
-  uint32_t ran, errf, gbits;
+  uint32_t ran, errf;
+  typename CflSlot<R>::bits gbits;
 };
+template <typename R>
 struct Fold {
   Ctl p;
-  float cfl;      // max cfl over the counted cells of attempt a
+  R cfl;          // max cfl over the counted cells of attempt a
   int state;      // kFoldIdle, kFoldError, kFoldAccept, kFoldReject
 };
 constexpr int kFoldIdle = 0, kFoldError = 1, kFoldAccept = 2, kFoldReject = 3;
 
-__device__ __forceinline__ FoldRaw fold_load(const Ctl* __restrict__ c,
-                                             const SyncSlots* __restrict__ sl) {
-  FoldRaw r;
+template <typename R>
+__device__ __forceinline__ FoldRaw<R> fold_load(const Ctl* __restrict__ c,
+                                                const SyncSlots* __restrict__ sl) {
+  FoldRaw<R> r;
   Ctl& p = r.p;
   p.dt = c->dt; p.T = c->T; p.last_cfl = c->last_cfl;
   p.drive_s = c->drive_s; p.drive_c = c->drive_c;
@@ -139,31 +224,29 @@ __device__ __forceinline__ FoldRaw fold_load(const Ctl* __restrict__ c,
   r.errf = sl->err_flag;
   // the group maxima with one vector load (lane l reads group l % kGroups)
   // rather than kGroups scalar loads
-  r.gbits = sl->grp[__lane_id() & (kGroups - 1)].cfl_bits;
+  r.gbits = CflSlot<R>::load(sl->grp[__lane_id() & (kGroups - 1)]);
   return r;
 }
 
-__device__ __forceinline__ Fold fold_decide(const FoldRaw& r, double cfl_max, bool presc) {
-  Fold f;
+template <typename R>
+__device__ __forceinline__ Fold<R> fold_decide(const FoldRaw<R>& r, double cfl_max, bool presc) {
+  Fold<R> f;
   f.p = r.p;
   f.p.n_acc = uni64(r.p.n_acc); f.p.n_rej = uni64(r.p.n_rej); f.p.n_att = uni64(r.p.n_att);
   f.p.target_acc = uni64(r.p.target_acc);
   f.p.cur = uni(r.p.cur); f.p.err = uni(r.p.err);
   f.p.mdx = uni(r.p.mdx); f.p.mdy = uni(r.p.mdy);
-  // non-negative floats order like their bit patterns
-  uint32_t mbits = 0u;
-#pragma unroll
-  for (int k = 0; k < kGroups; ++k) mbits = max(mbits, (uint32_t)__builtin_amdgcn_readlane(r.gbits, k));
-  const float cfl32 = __uint_as_float(mbits);
-  f.cfl = cfl32;
-  const bool acc = presc || ((double)cfl32 < cfl_max);
+  const R cfl = CflSlot<R>::value(CflSlot<R>::lane_max(r.gbits));
+  f.cfl = cfl;
+  const bool acc = presc || ((double)cfl < cfl_max);
   f.state = !uni(r.ran) ? kFoldIdle
                         : (uni(r.errf) ? kFoldError : (acc ? kFoldAccept : kFoldReject));
   return f;
 }
 
 // Ping-pong index attempt a+1 reads (known before the rest of advance()).
-__device__ __forceinline__ int next_cur(const Fold& f) {
+template <typename R>
+__device__ __forceinline__ int next_cur(const Fold<R>& f) {
   return f.state == kFoldAccept ? (f.p.cur ^ 1) : f.p.cur;
 }
 
@@ -177,7 +260,8 @@ __device__ __forceinline__ int md_shift(double T, double md_t, double v, double 
 // Control block of attempt a+1 from attempt a's fold: common/main.cu:199-230
 // with a clean retry on rejection.  log: record dt and T of an accepted step
 // (adjoint runs); every writer records the same values.
-__device__ Ctl advance(const InstDesc& D, const Fold& f, bool log) {
+template <typename R>
+__device__ Ctl advance(const InstDesc& D, const Fold<R>& f, bool log) {
   const Ctl& p = f.p;
   Ctl o = p;
   if (f.state == kFoldIdle) return o;   // attempt a did not run (instance done or stopped)
@@ -227,34 +311,48 @@ __device__ Ctl advance(const InstDesc& D, const Fold& f, bool log) {
   // drive phase omega*T in turns, reduced in double to [0, 1); the step uses
   // sin/cos in FP32
   const double turns = D.omega_turns * T;
-  float fs, fc;
-  sincospif((float)(2.0 * (turns - floor(turns))), &fs, &fc);
-  o.drive_s = fs;
-  o.drive_c = fc;
+  if (sizeof(R) == 4) {
+    float fs, fc;
+    sincospif((float)(2.0 * (turns - floor(turns))), &fs, &fc);
+    o.drive_s = fs;
+    o.drive_c = fc;
+  } else {                      // FP64 instances: the drive phase in double
+    double ds, dc;
+    sincospi(2.0 * (turns - floor(turns)), &ds, &dc);
+    o.drive_s = ds;
+    o.drive_c = dc;
+  }
   return o;
 }
 
 // Control block of the attempt a launch runs (phase = a mod 6), read-only.
-__device__ __forceinline__ Ctl current(const InstDesc& D, int phase, bool log) {
+template <typename R>
+__device__ __forceinline__ Ctl current_t(const InstDesc& D, int phase, bool log) {
   return advance(D,
-                 fold_decide(fold_load(D.ctl + (phase & 1), D.sync + (phase + 2) % 3),
+                 fold_decide(fold_load<R>(D.ctl + (phase & 1), D.sync + (phase + 2) % 3),
                              D.cfl_max, D.dt_seq != nullptr),
                  log);
 }
+__device__ __forceinline__ Ctl current(const InstDesc& D, int phase, bool log) {
+  return D.fp64 ? current_t<double>(D, phase, log) : current_t<float>(D, phase, log);
+}
 
+template <typename R>
 struct Row {         // one cell of a row as the y sweep sees it
-  float q1, q2, q3;  // rho', rho u, rho v
-  float un, ut;      // y normal velocity v = q3/rho, tangential u = q2/rho
+  R q1, q2, q3;      // rho', rho u, rho v
+  R un, ut;          // y normal velocity v = q3/rho, tangential u = q2/rho
 };
 
+template <typename R>
 struct YI {          // y interface with its limited flux towards the + cell
-  fvna::Iface f;
-  float dl[3];       // W_p . W_p of the interface below (one per wave family)
-  float P[3];        // apdq - fadd, in (density, normal, tangential) order
+  fvna::IfaceT<R> f;
+  R dl[3];           // W_p . W_p of the interface below (one per wave family)
+  R P[3];            // apdq - fadd, in (density, normal, tangential) order
 };
 
+template <typename R>
 struct XY {          // one cell of q_xy, with its velocities for the source
-  float q1, q2, q3, u, v;
+  R q1, q2, q3, u, v;
 };
 
 // kOne: the launch covers a single instance, whose control block and
@@ -272,7 +370,9 @@ struct XY {          // one cell of q_xy, with its velocities for the source
 //  - HIFU boundary (apps/hifu-beam/boundary.h:20-133): rows 0, 1 read rows
 //    4, 3 with rho v negated, the 2x2 corners are 0, and the aperture rows of
 //    the left ghost columns take the focused packet while MDX == 0.
-template <bool kOne, bool kExt>
+// R: float (FP32 path) or double (FP64 instances: state, tables, arithmetic
+// and the cfl reduction in double; the reference's PRECISION=2).
+template <typename R, bool kOne, bool kExt>
 __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
                                                      const WorkItem* __restrict__ items,
                                                      const uint64_t* __restrict__ rowmask,
@@ -301,33 +401,36 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // pointer by the buffer index once accept/reject is known
   const int nx = D.nx, ny = D.ny;
   const int bc = D.bc;
-  float* qb[2][3];
+  typedef typename Mem<R>::g gR;
+  typedef typename Mem<R>::cg cgR;
+  typedef typename Mem<R>::k kR;
+  R* qb[2][3];
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int f = 0; f < 3; ++f) qb[h][f] = D.q[h][f];
+    for (int f = 0; f < 3; ++f) qb[h][f] = (R*)D.q[h][f];
   double cfl_max = D.cfl_max;
   const double* dt_seq = D.dt_seq;
-  const float* ec_p = D.edge_c;
-  const float* es_p = D.edge_s;
-  const FoldRaw FR = fold_load(kOne ? ctl_in : D.ctl + (phase & 1),
-                               kOne ? sync_in : D.sync + (phase + 2) % 3);
+  const R* ec_p = (const R*)D.edge_c;
+  const R* es_p = (const R*)D.edge_s;
+  const FoldRaw<R> FR = fold_load<R>(kOne ? ctl_in : D.ctl + (phase & 1),
+                                     kOne ? sync_in : D.sync + (phase + 2) % 3);
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int f = 0; f < 3; ++f) asm volatile("" : "+s"(qb[h][f]));
   asm volatile("" : "+s"(cfl_max), "+s"(dt_seq), "+s"(ec_p), "+s"(es_p));
-  const Fold F = fold_decide(FR, cfl_max, dt_seq != nullptr);
+  const Fold<R> F = fold_decide(FR, cfl_max, dt_seq != nullptr);
   const int cur = next_cur(F);
 
-  cgfloat* __restrict__ in1 = (cgfloat*)(cur ? qb[1][0] : qb[0][0]);
-  cgfloat* __restrict__ in2 = (cgfloat*)(cur ? qb[1][1] : qb[0][1]);
-  cgfloat* __restrict__ in3 = (cgfloat*)(cur ? qb[1][2] : qb[0][2]);
-  gfloat* __restrict__ o1 = (gfloat*)(cur ? qb[0][0] : qb[1][0]);
-  gfloat* __restrict__ o2 = (gfloat*)(cur ? qb[0][1] : qb[1][1]);
-  gfloat* __restrict__ o3 = (gfloat*)(cur ? qb[0][2] : qb[1][2]);
-  kfloat* __restrict__ edge_c = (kfloat*)ec_p;
-  kfloat* __restrict__ edge_s = (kfloat*)es_p;
+  cgR* __restrict__ in1 = (cgR*)(cur ? qb[1][0] : qb[0][0]);
+  cgR* __restrict__ in2 = (cgR*)(cur ? qb[1][1] : qb[0][1]);
+  cgR* __restrict__ in3 = (cgR*)(cur ? qb[1][2] : qb[0][2]);
+  gR* __restrict__ o1 = (gR*)(cur ? qb[0][0] : qb[1][0]);
+  gR* __restrict__ o2 = (gR*)(cur ? qb[0][1] : qb[1][1]);
+  gR* __restrict__ o3 = (gR*)(cur ? qb[0][2] : qb[1][2]);
+  kR* __restrict__ edge_c = (kR*)ec_p;
+  kR* __restrict__ edge_s = (kR*)es_p;
 
   const int i = it.x0 - kHalo + lane;
   const bool col_in = (i >= 0) && (i < nx);
@@ -347,16 +450,17 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   const bool left = !taylor && (ic <= 1);
   const int colA = taylor ? ic : (left ? 2 : (ic >= nx - 2 ? nx - 3 : ic));
   const int colB = taylor ? ic : (left ? 2 : ic);
-  const uint32_t bA = 4u * (uint32_t)colA, bB = 4u * (uint32_t)colB;
-  const uint32_t row_bytes = 4u * (uint32_t)nx;
+  constexpr uint32_t kSz = sizeof(R);
+  const uint32_t bA = kSz * (uint32_t)colA, bB = kSz * (uint32_t)colB;
+  const uint32_t row_bytes = kSz * (uint32_t)nx;
   const bool hifu = kExt && bc == kBcHifu;
-  kfloat* __restrict__ htab = (kfloat*)D.hifu_tab;
-  struct HRow { float ph, nux, nuy, on; };   // aperture row data (hifu)
+  kR* __restrict__ htab = (kR*)D.hifu_tab;
+  struct HRow { R ph, nux, nuy, on; };       // aperture row data (hifu)
 
   // raw (unpatched) load of row r, with the row's drive coefficients; the
   // boundary patch is applied when the row is consumed one iteration later,
   // so the load latency overlaps a full step
-  auto load_raw = [&](int r, float& a, float& b, float& c, float& ec, float& es, HRow& h) {
+  auto load_raw = [&](int r, R& a, R& b, R& c, R& ec, R& es, HRow& h) {
     const int rc = min(max(r, 0), ny - 1);
     const bool grow = (rc <= 1) || (rc >= ny - 2);
     uint32_t o;
@@ -384,12 +488,12 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
       h.on = htab[4 * rc + 3];
     }
   };
-  float n1, n2, n3, nec = 0.0f, nes = 0.0f;
-  HRow nh{0.0f, 0.0f, 0.0f, 0.0f};
+  R n1, n2, n3, nec = R(0), nes = R(0);
+  HRow nh{R(0), R(0), R(0), R(0)};
   load_raw(it.y0 - kHalo, n1, n2, n3, nec, nes, nh);   // in flight during advance()
 
   // ---- control block of this attempt ----
-  const Ctl P = advance(D, F, local == 0);
+  const Ctl P = advance<R>(D, F, local == 0);
   const bool runs = P.err == 0 && P.n_acc < P.target_acc && P.T < P.T_stop;
   if (local == 0 && lane == 0) {
     // one wave per instance: publish the control block for the next launch,
@@ -397,16 +501,24 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     D.ctl[(phase + 1) & 1] = P;
     SyncSlots* nx_sl = D.sync + (phase + 1) % 3;
 #pragma unroll
-    for (int k = 0; k < kGroups; ++k) nx_sl->grp[k].cfl_bits = 0u;
+    for (int k = 0; k < kGroups; ++k) CflSlot<R>::clear(nx_sl->grp[k]);
     nx_sl->err_flag = 0u;
     nx_sl->ran = 0u;
     if (runs) D.sync[phase % 3].ran = 1u;
   }
   if (!runs) return;
-  const float dtx = unif(P.dtx), dty = unif(P.dty);
-  const float ddx2 = unif(P.ddx2), ddy2 = unif(P.ddy2);
-  const float sin_t = unif((float)P.drive_s), cos_t = unif((float)P.drive_c);
-  const float hbeta = 0.5f * D.beta2m2;     // BETA - 1
+  R dtx, dty, ddx2, ddy2, sin_t, cos_t, hbeta;
+  if (sizeof(R) == 4) {     // the control block's FP32 forms
+    dtx = unif(P.dtx); dty = unif(P.dty);
+    ddx2 = unif(P.ddx2); ddy2 = unif(P.ddy2);
+    sin_t = unif((float)P.drive_s); cos_t = unif((float)P.drive_c);
+    hbeta = R(0.5f * D.beta2m2);            // BETA - 1
+  } else {                  // FP64: the same quantities in double
+    dtx = unif(R(P.dt * D.inv_dx)); dty = unif(R(P.dt * D.inv_dy));
+    ddx2 = unif(R(P.dt * D.dif_x)); ddy2 = unif(R(P.dt * D.dif_y));
+    sin_t = unif(R(P.drive_s)); cos_t = unif(R(P.drive_c));
+    hbeta = R(D.hbeta_d);
+  }
   // the edge source runs while the window has not moved in x
   // (apps/hifu-beam/boundary.h:82); after that the left ghost columns are
   // plain zero-gradient copies
@@ -419,16 +531,16 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     sx = uni(md_shift(Tn, D.md_t, D.md_vx, D.dx, P.mdx));
     sy = uni(md_shift(Tn, D.md_t, D.md_vy, D.dy, P.mdy));
   }
-  gfloat* __restrict__ stage = (gfloat*)D.stage;
+  typename Mem<R>::g* __restrict__ stage = (typename Mem<R>::g*)D.stage;
   const bool stage_on = kExt && stage != nullptr;
   const int jcut = D.jcut;
   // HIFU: the packet runs while the window has not moved (boundary.h:82)
   const bool hdrive = hifu && P.mdx == 0 && it.x0 == 0;
   const double Td = P.T;
-  const float hk = D.hifu_k;
-  auto patch = [&](Row& q, float ec, float es, const HRow& h, int r) {
+  const R hk = sizeof(R) == 4 ? R(D.hifu_k) : R(D.hifu_k_d);
+  auto patch = [&](Row<R>& q, R ec, R es, const HRow& h, int r) {
     if (drive) {
-      float a, b, c;
+      R a, b, c;
       fvbc::drive(ec, es, sin_t, cos_t, a, b, c);
       q.q1 = drive_lane ? a : q.q1;
       q.q2 = drive_lane ? b : q.q2;
@@ -437,65 +549,65 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     if (hifu) {
       const int rc = min(max(r, 0), ny - 1);
       q.q3 = rc <= 1 ? -q.q3 : q.q3;          // reflective beam axis
-      if (hdrive && h.on != 0.0f && rc >= 2 && rc <= ny - 3) {
-        float a, b, c;
+      if (hdrive && h.on != R(0) && rc >= 2 && rc <= ny - 3) {
+        R a, b, c;
         // T + phi in FP64: the envelope's 10th power magnifies its rounding
-        fvbc::hifu_packet(ec, es, sin_t, cos_t, hk * (float)(Td + (double)h.ph), h.nux, h.nuy,
+        fvbc::hifu_packet(ec, es, sin_t, cos_t, hk * R(Td + (double)h.ph), h.nux, h.nuy,
                           a, b, c);
         q.q1 = left ? a : q.q1;
         q.q2 = left ? b : q.q2;
         q.q3 = left ? c : q.q3;
       }
       const bool corner = (rc <= 1 || rc >= ny - 2) && (ic <= 1 || ic >= nx - 2);
-      q.q1 = corner ? 0.0f : q.q1;
-      q.q2 = corner ? 0.0f : q.q2;
-      q.q3 = corner ? 0.0f : q.q3;
+      q.q1 = corner ? R(0) : q.q1;
+      q.q2 = corner ? R(0) : q.q2;
+      q.q3 = corner ? R(0) : q.q3;
     }
-    const float ir = fvna::rcp(1.0f + q.q1);
+    const R ir = fvna::rcp(R(1) + q.q1);
     q.un = q.q3 * ir;
     q.ut = q.q2 * ir;
   };
   // source on row s (apps/na/source.h:86-104) from q_xy rows s-1, s, s+1,
   // then the store of row s (lanes 3..60)
-  auto source_store = [&](int s, const XY& lo, const XY& mid, const XY& hi) {
+  auto source_store = [&](int s, const XY<R>& lo, const XY<R>& mid, const XY<R>& hi) {
     const bool s_int = col_int && (s >= 2) && (s <= ny - 3);
     // u(i-1) + u(i+1): one cross-lane move, the other folds into the add
-    const float uwe = shr1(mid.u) + shl1(mid.u);
-    const float vwe = shr1(mid.v) + shl1(mid.v);
-    float r2 = (mid.q2 + ddx2 * fmaf(-2.0f, mid.u, uwe))
-               + ddy2 * (fmaf(-2.0f, mid.u, lo.u) + hi.u);
-    float r3 = (mid.q3 + ddx2 * fmaf(-2.0f, mid.v, vwe))
-               + ddy2 * (fmaf(-2.0f, mid.v, lo.v) + hi.v);
+    const R uwe = shr1(mid.u) + shl1(mid.u);
+    const R vwe = shr1(mid.v) + shl1(mid.v);
+    R r2 = (mid.q2 + ddx2 * fvna::fma_(R(-2), mid.u, uwe))
+           + ddy2 * (fvna::fma_(R(-2), mid.u, lo.u) + hi.u);
+    R r3 = (mid.q3 + ddx2 * fvna::fma_(R(-2), mid.v, vwe))
+           + ddy2 * (fvna::fma_(R(-2), mid.v, lo.v) + hi.v);
     r2 = s_int ? r2 : mid.q2;
     r3 = s_int ? r3 : mid.q3;
     pin(r2);
     pin(r3);
     if (!kExt) {
       if (out_lane) {
-        const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
+        const uint32_t o = (uint32_t)s * row_bytes + kSz * (uint32_t)i;
         stg(o1, o, mid.q1);            // q1 passes the source unchanged
         stg(o2, o, r2);
         stg(o3, o, r3);
       }
     } else {
       if (out_lane && i >= sx && s >= sy) {
-        const uint32_t o = (uint32_t)(s - sy) * row_bytes + 4u * (uint32_t)(i - sx);
+        const uint32_t o = (uint32_t)(s - sy) * row_bytes + kSz * (uint32_t)(i - sx);
         stg(o1, o, mid.q1);
         stg(o2, o, r2);
         stg(o3, o, r3);
       }
       if (out_lane && (i + sx > nx - 1 || s + sy > ny - 1)) {
-        const uint32_t o = (uint32_t)s * row_bytes + 4u * (uint32_t)i;
+        const uint32_t o = (uint32_t)s * row_bytes + kSz * (uint32_t)i;
         stg(o1, o, mid.q1);
         stg(o2, o, r2);
         stg(o3, o, r3);
       }
       // data collection reads row jcut as it is before the shift
-      if (stage_on && s == jcut && out_lane) stg(stage, 4u * (uint32_t)i, mid.q1);
+      if (stage_on && s == jcut && out_lane) stg(stage, kSz * (uint32_t)i, mid.q1);
     }
   };
 
-  float Sx = 0.0f, Sy = 0.0f;
+  R Sx = R(0), Sy = R(0);
   uint64_t err = 0;                          // lanes that met tmp3 < 0 (or NaN)
   const uint64_t in_mask = __builtin_amdgcn_ballot_w64(col_in);
   const uint64_t x_mask = __builtin_amdgcn_ballot_w64(col_x);
@@ -507,8 +619,8 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // prefetch, store row r-4, issue the next prefetch -- so each store has a
   // whole step to drain before the next step waits for its prefetch (stores
   // and loads share one in-order counter).
-  auto body = [&](int r, Row& bn, Row& bm, Row& bo, YI& fn, YI& fc, YI& fl, XY& xn, XY& xm,
-                  XY& xo) {
+  auto body = [&](int r, Row<R>& bn, Row<R>& bm, Row<R>& bo, YI<R>& fn, YI<R>& fc, YI<R>& fl,
+                  XY<R>& xn, XY<R>& xm, XY<R>& xo) {
     // every load of the prefetch (and every store before it) is done: an
     // explicit vmcnt(0) here keeps the compiler's own wait for the prefetched
     // registers from landing after this step's stores
@@ -541,7 +653,7 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     if (r < y0) return;
 
     // ---- y limiter at interface r-1 ----
-    float M[3];
+    R M[3];
     fvna::limited_flux(fc.f, fc.dl[0], fn.dl[0], fc.dl[1], fn.dl[1], fc.dl[2], fn.dl[2], dty,
                        fc.P, M);
     if (r < y0 + 1) return;
@@ -551,43 +663,43 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     const bool upd = col_int && row_int;
     // ghost cells keep their value: dt/d is 0 there (the fluxes are finite
     // wherever the attempt is valid, so q - 0*flux == q exactly)
-    const float dtyu = upd ? dty : 0.0f, dtxu = upd ? dtx : 0.0f;
-    const float y1q = bo.q1 - dtyu * (fl.P[0] + M[0]);
-    const float y3q = bo.q3 - dtyu * (fl.P[1] + M[1]);
-    const float y2q = bo.q2 - dtyu * (fl.P[2] + M[2]);
+    const R dtyu = upd ? dty : R(0), dtxu = upd ? dtx : R(0);
+    const R y1q = bo.q1 - dtyu * (fl.P[0] + M[0]);
+    const R y3q = bo.q3 - dtyu * (fl.P[1] + M[1]);
+    const R y2q = bo.q2 - dtyu * (fl.P[2] + M[2]);
 
     // ---- x sweep on row m: normal = q2, tangential = q3 ----
-    const float ir = fvna::rcp(1.0f + y1q);
-    const float un = y2q * ir, ut = y3q * ir;
-    const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
-    const float lun = shr1(un), lut = shr1(ut);
-    fvna::Iface fx;
+    const R ir = fvna::rcp(R(1) + y1q);
+    const R un = y2q * ir, ut = y3q * ir;
+    const R l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
+    const R lun = shr1(un), lut = shr1(ut);
+    fvna::IfaceT<R> fx;
     fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, hbeta, fx);
     err |= __builtin_amdgcn_ballot_w64(fx.bad) & ((m >= 0 && m < ny) ? x_mask : 0);
     Sx = fvna::fmax2(Sx, lane_sel(mX, fx.smax));
     // waves of the interface to the left (k-1)
-    fvna::Waves wl;
+    fvna::WavesT<R> wl;
     wl.a1 = shr1(fx.w.a1); wl.w21 = shr1(fx.w.w21); wl.w31 = shr1(fx.w.w31);
     wl.a2 = shr1(fx.w.a2); wl.w22 = shr1(fx.w.w22); wl.w32 = shr1(fx.w.w32);
     wl.a3 = shr1(fx.w.a3);
-    const float dl1 = fvna::dot1(fx.w, wl), dl2 = fvna::dot2(fx.w, wl);
-    const float dl3 = fx.w.a3 * wl.a3;
+    const R dl1 = fvna::dot1(fx.w, wl), dl2 = fvna::dot2(fx.w, wl);
+    const R dl3 = fx.w.a3 * wl.a3;
     // W(k).W(k+1) is the right neighbour's W(k+1).W(k)
-    const float dr1 = shl1(dl1), dr2 = shl1(dl2), dr3 = shl1(dl3);
-    float PX[3], MX[3];
+    const R dr1 = shl1(dl1), dr2 = shl1(dl2), dr3 = shl1(dl3);
+    R PX[3], MX[3];
     fvna::limited_flux(fx, dl1, dr1, dl2, dr2, dl3, dr3, dtx, PX, MX);
-    const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
+    const R mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
     xn.q1 = y1q - dtxu * (mr0 + PX[0]);
     xn.q2 = y2q - dtxu * (mr1 + PX[1]);
     xn.q3 = y3q - dtxu * (mr2 + PX[2]);
-    const float ir2 = fvna::rcp(1.0f + xn.q1);
+    const R ir2 = fvna::rcp(R(1) + xn.q1);
     xn.u = xn.q2 * ir2;
     xn.v = xn.q3 * ir2;
   };
 
-  Row B0{}, B1{}, B2{};
-  YI F0{}, F1{}, F2{};
-  XY X0{}, X1{}, X2{};
+  Row<R> B0{}, B1{}, B2{};
+  YI<R> F0{}, F1{}, F2{};
+  XY<R> X0{}, X1{}, X2{};
   for (int r = y0 - kHalo; r <= y1 + 2; r += 3) {
     body(r, B0, B2, B1, F0, F2, F1, X0, X2, X1);
     if (r + 1 > y1 + 2) break;
@@ -605,11 +717,11 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // ---- CFL / error reduction ----
   // per-cell cfl = |s| * dt/d (fv.h:148-180); max over the wave's cells,
   // folded into the group's slot of this attempt (read by the next launch)
-  const float cflw = wave_max(fmaxf(Sy * dty, Sx * dtx));
+  const R cflw = wave_max(fmax_(Sy * dty, Sx * dtx));
   if (lane != 0) return;
   SyncSlots* sl = D.sync + phase % 3;
   if (err != 0) aadd_relaxed(&sl->err_flag, 1u);
-  amax_relaxed(&sl->grp[local % kGroups].cfl_bits, __float_as_uint(cflw));
+  CflSlot<R>::fold_in(sl->grp[local % kGroups], cflw);
 }
 
 // Control block after the last launched attempt (phase = index of the next
@@ -622,60 +734,132 @@ __global__ __launch_bounds__(64) void fv_settle_kernel(const InstDesc* __restric
   if (threadIdx.x == 0) D.ctl[(phase + 1) & 1] = P;
 }
 
-// Data collection of attempt a (apps/hifu-beam/data_collect.h:7-107), run
-// after its step launch: if the attempt was accepted, record its new T, its
-// cfl and row jcut of its new state (staged by the step before the window
-// shift, with the pre-shift MDX) into the instance's measurement grid.
+// Data collection of the hifu app (apps/hifu-beam/data_collect.h:51-106) for
+// one accepted attempt: row jcut of its new state, staged by the step before
+// the window shift, read with the pre-shift MDX.  Thread i = column i.
+template <typename R>
+__device__ void collect_hifu(const InstDesc& D, const Ctl& P, double T, int64_t nf,
+                             typename Mem<R>::g* __restrict__ m, int i) {
+  const int nx = D.nx, ny = D.ny;
+  if (i >= nx) return;
+  const R u1 = ((const R*)D.stage)[i];
+  const int mdx = P.mdx;
+  for (int k = 0; k < 4; ++k) {                     // data_collect.h:51-76
+    if (i + mdx == D.mic[k]) {
+      typename Mem<R>::g* row = m + (size_t)(2 + k) * nx;
+      if (nf == 0)
+        for (int ii = 0; ii < nx; ++ii) row[ii] = R(0);
+      row[nf] = u1;
+      break;
+    }
+  }
+  if (i == 0 && nf == 0)                            // microphones outside the window
+    for (int k = 0; k < 4; ++k)
+      if (D.mic[k] - mdx < 0 || D.mic[k] - mdx >= nx)
+        for (int ii = 0; ii < nx; ++ii) m[(size_t)(2 + k) * nx + ii] = R(0);
+  // central peak only (data_collect.h:100-106)
+  if (fabs((double)(i + mdx) / D.eta - T) > 0.5) return;
+  const int64_t o = (int64_t)i + mdx + 6 * (int64_t)nx;
+  if (o < (int64_t)nx * ny) m[o] = fmax_(u1, m[o]);
+}
+
+// Data collection of the taylor-angle-xy app (apps/taylor-angle-xy/
+// data_collect.h:43-96) for one accepted attempt: rho' of the new state along
+// the propagation direction through the front's centre, interpolated in y
+// between the two rows nearest to the line (entry iwrite of row 2), beside the
+// closed form at the nearer row (row 3).  The arithmetic follows the
+// reference's types: T, dx, dy and the constants in R, the line position ja
+// and the interpolation weights in float.  The state is read from the output
+// buffer, where the step has already applied the window shift (sx, sy) of this
+// attempt: the cell (i, j) of the unshifted frame is (i - sx, j - sy) there.
+// Thread tid = entry iwrite of the cut, 0 <= tid <= 2 cut_half.
+template <typename R>
+__device__ void collect_taylor(const InstDesc& D, const Ctl& P, double Td,
+                               typename Mem<R>::g* __restrict__ m, int tid) {
+  const int L = D.cut_half;
+  if (tid > 2 * L) return;
+  const int nx = D.nx, ny = D.ny;
+  const R T = R(Td), dx = R(D.dx), dy = R(D.dy);
+  const int icenter = (int)(T * R(D.md_vx) / dx) - P.mdx;                  // :43
+  const int jcenter = ny / 2 + (int)((T - R(D.md_t)) * R(D.md_vy) / dy) - P.mdy;  // :44-46
+  const int i = icenter - L + tid;                  // iwrite = tid (:87-89)
+  if (i < 0 || i > nx - 1) return;
+  const float ja = (float)fvna::fma_(R((float)(i - icenter)) * (dx / dy), R(D.tan_theta),
+                               R((float)jcenter));  // :52-55
+  int sx = 0, sy = 0;
+  if (D.md_on) {
+    sx = md_shift(Td, D.md_t, D.md_vx, D.dx, P.mdx);
+    sy = md_shift(Td, D.md_t, D.md_vy, D.dy, P.mdy);
+  }
+  const R* __restrict__ u = (const R*)D.q[P.cur ^ 1][0];
+  // tex2D reads clamp to the grid; the shifted frame starts at (sx, sy)
+  auto at = [&](int ii, int jj) -> R {
+    jj = min(max(jj, 0), ny - 1);
+    const int pi = max(ii - sx, 0), pj = max(jj - sy, 0);
+    return u[(size_t)pj * nx + pi];
+  };
+  // the thread (i, j) of the reference whose row is nearest to ja: j - ja in
+  // [0, 1/2] takes rows j, j-1 (:58-68), j - ja in (-1/2, 0) rows j+1, j (:69-79)
+  const int jf = (int)floorf(ja);
+  for (int j = max(jf - 1, 0); j <= min(jf + 2, ny - 1); ++j) {
+    const float dj = (float)j - ja;
+    R val;
+    if (dj >= 0.0f && dj <= 0.5f) {
+      const float djm = (float)(j - 1) - ja;
+      val = fvna::fma_(R(-djm), at(i, j), R(dj) * at(i, j - 1));
+    } else if (dj < 0.0f && -dj < 0.5f) {
+      const float djp = (float)(j + 1) - ja;
+      val = fvna::fma_(R(-dj), at(i, j + 1), R(djp) * at(i, j));
+    } else {
+      continue;
+    }
+    // the closed form at (i + MDX, j + MDY) (:90-95)
+    const R arg = R((float)(i + P.mdx)) * dx * R(D.tay_cos) +
+                  (R(j + P.mdy - ny / 2) * dy + R(D.md_t) * R(D.tay_sin)) * R(D.tay_sin) - T;
+    m[(size_t)tid + 2 * (size_t)nx] = val;
+    m[(size_t)tid + 3 * (size_t)nx] = R(D.tay_rhoa) * tanh(R(D.tay_c) * arg);
+    return;
+  }
+}
+
+// Data collection of attempt a, run after its step launch: if the attempt
+// was accepted, record its new T and its cfl (row 0 and row 1, entry
+// n - 1 - n0frame) and the app's measurement (collect_hifu, collect_taylor)
+// into the instance's measurement grid (data_collect.h of either app).
 // blockIdx.y = instance; threads cover the columns (and, while T <= 0, the
 // whole grid, which the reference zeroes then).
+template <typename R>
 __global__ __launch_bounds__(256) void fv_collect_kernel(const InstDesc* __restrict__ insts,
                                                          int phase) {
   const InstDesc& D = insts[blockIdx.y];
-  float* __restrict__ m = D.measure;
+  typename Mem<R>::g* __restrict__ m = (typename Mem<R>::g*)D.measure;
   if (m == nullptr) return;
   // attempt a's control block (its step launch published it) and slots
-  const Fold F = fold_decide(fold_load(D.ctl + ((phase + 1) & 1), D.sync + phase % 3),
-                             D.cfl_max, D.dt_seq != nullptr);
+  const Fold<R> F = fold_decide(fold_load<R>(D.ctl + ((phase + 1) & 1), D.sync + phase % 3),
+                                D.cfl_max, D.dt_seq != nullptr);
   if (F.state != kFoldAccept) return;
   const Ctl& P = F.p;
   const double T = P.T + P.dt;                       // update_T
   const int nx = D.nx, ny = D.ny;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (T <= 0.0) {                                    // data_collect.h:27-30
+  if (T <= 0.0) {                                    // data_collect.h:22-30 (both apps)
     for (int64_t k = tid; k < (int64_t)nx * ny; k += (int64_t)gridDim.x * blockDim.x)
-      m[k] = 0.0f;
+      m[k] = R(0);
     return;
   }
-  const int i = tid;
-  if (i >= nx) return;
-  const float u1 = D.stage[i];
-  const int mdx = P.mdx;
   const int64_t nf = P.n_acc - D.n0frame;          // n - 1 - n0frame, n = n_acc + 1
-  if (nf >= nx) {                                   // data_collect.h:96
-    if (i == 0) aadd_relaxed(D.collect_err, 4u);
+  if (nf >= nx) {                                   // simError 4
+    if (tid == 0) aadd_relaxed(D.collect_err, 4u);
     return;
   }
-  for (int k = 0; k < 4; ++k) {                     // data_collect.h:51-76
-    if (i + mdx == D.mic[k]) {
-      float* row = m + (size_t)(2 + k) * nx;
-      if (nf == 0)
-        for (int ii = 0; ii < nx; ++ii) row[ii] = 0.0f;
-      row[nf] = u1;
-      break;
-    }
-  }
-  if (i == 0) {                                     // data_collect.h:78-94
-    m[nf] = (float)T;
+  if (tid == 0) {                                   // T and cfl of the step
+    m[nf] = R(T);
     m[nf + nx] = F.cfl;
-    if (nf == 0)
-      for (int k = 0; k < 4; ++k)
-        if (D.mic[k] - mdx < 0 || D.mic[k] - mdx >= nx)
-          for (int ii = 0; ii < nx; ++ii) m[(size_t)(2 + k) * nx + ii] = 0.0f;
   }
-  // central peak only (data_collect.h:100-106)
-  if (fabs((double)(i + mdx) / D.eta - T) > 0.5) return;
-  const int64_t o = (int64_t)i + mdx + 6 * (int64_t)nx;
-  if (o < (int64_t)nx * ny) m[o] = fmaxf(u1, m[o]);
+  if (D.bc == kBcTaylor)
+    collect_taylor<R>(D, P, T, m, tid);
+  else
+    collect_hifu<R>(D, P, T, nf, m, tid);
 }
 
 // Closed-form ghost ring of the Taylor-shock instances (apps/taylor-angle-xy/
@@ -705,15 +889,24 @@ __global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __
       j = 2 + (kk - cc * (ny - 4));
       i = cc < 2 ? cc : nx - 4 + cc;
     }
-    float a, b, c;
     // the closed form is taken at the cell's position in the unmoved frame
     // (apps/taylor-angle-xy/boundary.h:45-49: i + MDX, j + MDY)
-    fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx, D.dy,
-                 T, i + P.mdx, j + P.mdy, a, b, c);
     const size_t o = (size_t)j * (size_t)nx + (size_t)i;
-    D.q[cur][0][o] = a;
-    D.q[cur][1][o] = b;
-    D.q[cur][2][o] = c;
+    if (D.fp64) {
+      double a, b, c;
+      fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx, D.dy,
+                   T, i + P.mdx, j + P.mdy, a, b, c);
+      ((double*)D.q[cur][0])[o] = a;
+      ((double*)D.q[cur][1])[o] = b;
+      ((double*)D.q[cur][2])[o] = c;
+    } else {
+      float a, b, c;
+      fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx, D.dy,
+                   T, i + P.mdx, j + P.mdy, a, b, c);
+      D.q[cur][0][o] = a;
+      D.q[cur][1][o] = b;
+      D.q[cur][2][o] = c;
+    }
   }
 }
 
@@ -729,10 +922,14 @@ extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_i
 }
 
 extern "C" hipError_t fv_launch_collect(const fv::InstDesc* insts, int n_inst, int max_nx,
-                                        int phase, hipStream_t stream) {
+                                        int phase, int fp64, hipStream_t stream) {
   if (n_inst <= 0) return hipSuccess;
-  hipLaunchKernelGGL(fv::fv_collect_kernel, dim3((max_nx + 255) / 256, n_inst), dim3(256), 0,
-                     stream, insts, phase);
+  if (fp64)
+    hipLaunchKernelGGL(fv::fv_collect_kernel<double>, dim3((max_nx + 255) / 256, n_inst),
+                       dim3(256), 0, stream, insts, phase);
+  else
+    hipLaunchKernelGGL(fv::fv_collect_kernel<float>, dim3((max_nx + 255) / 256, n_inst),
+                       dim3(256), 0, stream, insts, phase);
   return hipGetLastError();
 }
 
@@ -743,34 +940,64 @@ extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, in
   return hipGetLastError();
 }
 
-// Resident step waves per CU (occupancy of the step kernel on this device).
-extern "C" hipError_t fv_step_waves_per_cu(int* waves) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      waves, fv::fv_step_kernel<false, false>, fv::kLanes, 0);
+// Resident step waves per CU (occupancy of the step kernel on this device),
+// of the FP64 kernel if fp64.  The lower of the plain and extended-boundary
+// variants, so the split fits whichever of them a context launches.
+extern "C" hipError_t fv_step_waves_per_cu(int* waves, int fp64) {
+  int a = 0, b = 0;
+  hipError_t e = fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &a, fv::fv_step_kernel<double, false, false>, fv::kLanes, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &a, fv::fv_step_kernel<float, false, false>, fv::kLanes, 0);
+  if (e != hipSuccess) return e;
+  e = fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &b, fv::fv_step_kernel<double, false, true>, fv::kLanes, 0)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &b, fv::fv_step_kernel<float, false, true>, fv::kLanes, 0);
+  *waves = a < b ? a : b;
+  return e;
 }
 
+namespace {
+template <typename R, bool kOne, bool kExt>
+void launch_step_t(const fv::InstDesc* insts, const fv::WorkItem* items, const uint64_t* rowmask,
+                   int n_items, int phase, const fv::Ctl* c_in, const fv::SyncSlots* s_in,
+                   hipStream_t stream) {
+  hipLaunchKernelGGL((fv::fv_step_kernel<R, kOne, kExt>), dim3(n_items), dim3(fv::kLanes), 0,
+                     stream, insts, items, rowmask, n_items, phase, c_in, s_in);
+}
+template <typename R>
+void launch_step_r(bool one, bool ext, const fv::InstDesc* insts, const fv::WorkItem* items,
+                   const uint64_t* rowmask, int n_items, int phase, const fv::Ctl* c_in,
+                   const fv::SyncSlots* s_in, hipStream_t stream) {
+  if (one && !ext)
+    launch_step_t<R, true, false>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+  else if (one)
+    launch_step_t<R, true, true>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+  else if (!ext)
+    launch_step_t<R, false, false>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+  else
+    launch_step_t<R, false, true>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+}
+}  // namespace
+
 // One step attempt of every instance.  ctl0/sync0: the control blocks and
-// slot sets of the only instance when n_inst == 1 (null otherwise); md: some
-// instance has a moving window.
+// slot sets of the only instance when n_inst == 1 (null otherwise); ext: some
+// instance has a moving window, the HIFU boundary or data collection; fp64:
+// the instances are FP64.
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
-                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int md,
-                                     hipStream_t stream) {
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int ext,
+                                     int fp64, hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
   const bool one = n_inst == 1 && ctl0 && sync0;
   const fv::Ctl* c_in = one ? ctl0 + (phase & 1) : nullptr;
   const fv::SyncSlots* s_in = one ? sync0 + (phase + 2) % 3 : nullptr;
-  if (one && !md)
-    hipLaunchKernelGGL((fv::fv_step_kernel<true, false>), dim3(n_items), dim3(fv::kLanes), 0,
-                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
-  else if (one)
-    hipLaunchKernelGGL((fv::fv_step_kernel<true, true>), dim3(n_items), dim3(fv::kLanes), 0,
-                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
-  else if (!md)
-    hipLaunchKernelGGL((fv::fv_step_kernel<false, false>), dim3(n_items), dim3(fv::kLanes), 0,
-                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
+  if (fp64)
+    launch_step_r<double>(one, ext != 0, insts, items, rowmask, n_items, phase, c_in, s_in,
+                          stream);
   else
-    hipLaunchKernelGGL((fv::fv_step_kernel<false, true>), dim3(n_items), dim3(fv::kLanes), 0,
-                       stream, insts, items, rowmask, n_items, phase, c_in, s_in);
+    launch_step_r<float>(one, ext != 0, insts, items, rowmask, n_items, phase, c_in, s_in,
+                         stream);
   return hipGetLastError();
 }

@@ -45,7 +45,9 @@ struct Ctl {
 constexpr int kGroups = 8;
 struct alignas(128) GroupSlot {
   uint32_t cfl_bits;    // max cfl of the group's cells (float bits, >= 0)
-  uint32_t pad[31];
+  uint32_t pad0;
+  uint64_t cfl_bits64;  // the same for FP64 instances (double bits, >= 0)
+  uint32_t pad[28];
 };
 struct alignas(128) SyncSlots {
   GroupSlot grp[kGroups];
@@ -58,8 +60,8 @@ struct alignas(128) SyncSlots {
 struct InstDesc {
   float* q[2][3];       // ping-pong SoA state buffers, offset i + j*nx
   Ctl* ctl;             // 2 control blocks (attempt index mod 2)
-  const float* edge_c;  // amp_j * cos(phase_j), length ny (kBcEdgeDrive)
-  const float* edge_s;  // amp_j * sin(phase_j), length ny
+  const void* edge_c;   // amp_j * cos(phase_j), length ny (kBcEdgeDrive); float or
+  const void* edge_s;   // amp_j * sin(phase_j), length ny         double (fp64)
   const double* dt_seq; // prescribed dt per attempt, or null
   int64_t dt_seq_len;
   double* dt_log;       // when set: dt and T of accepted step n_acc - log_base
@@ -82,17 +84,26 @@ struct InstDesc {
   double md_vx, md_vy, md_t;   // MDVX, MDVY, MDT
   // kBcHifu: edge_c/edge_s hold AMPL cos/sin(ISPW phi_j); per row j,
   // hifu_tab[4j..4j+3] = phi_j, nu_x, nu_y, 1 on aperture rows (0 elsewhere)
-  const float* hifu_tab;
+  const void* hifu_tab; // float or double (fp64)
   float hifu_k;         // FREC/4: envelope argument per unit of T + phi
   int32_t jcut;         // data collection: row (JCUT)
-  // data collection (apps/hifu-beam/data_collect.h): measure = nx*ny grid
-  // (null: off), stage = row jcut of the last attempt before the window shift
-  float* measure;
-  float* stage;
+  // data collection (apps/hifu-beam/data_collect.h, apps/taylor-angle-xy/
+  // data_collect.h): measure = nx*ny grid (null: off), stage = row jcut of
+  // the last attempt before the window shift (hifu); float or double (fp64)
+  void* measure;
+  void* stage;
   uint32_t* collect_err;  // 4 once a frame holds more than nx steps
   int64_t n0frame;      // accepted steps when the current frame started
   double eta;           // cells per length unit
   int32_t mic[4];
+  // taylor-angle-xy cut: i in [icenter - cut_half, icenter + cut_half]
+  int32_t cut_half, pad6;
+  double tan_theta;
+  // FP64 instances (the reference's PRECISION=2): state, tables and the cfl
+  // reduction in double
+  int32_t fp64, pad5;
+  double hbeta_d;       // BETA - 1
+  double hifu_k_d;      // FREC/4
 };
 
 // One wave's work: a 58-column strip of one instance, rows [y0, y1).

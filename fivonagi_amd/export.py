@@ -1,16 +1,23 @@
-"""On-disk frame exports of the hifu-beam app, in the reference's formats
-(apps/hifu-beam/data_export.h:14-179), so that its analysis scripts read our
-runs unchanged.
+"""On-disk frame exports of the hifu-beam and taylor-angle-xy apps, in the
+reference's formats (apps/hifu-beam/data_export.h:14-179,
+apps/taylor-angle-xy/data_export.h:14-176), so that their analysis scripts
+read our runs unchanged.
 
-Per frame (after the frame's ``Context.step_until``):
+hifu-beam, per frame (after the frame's ``Context.step_until``):
   u1-NNNN.dat      the whole rho' grid with its x, y axes (in focal lengths)
   u1-cut-NNNN.dat  rho' on the cut row JCUT
   u1max.dat        the running maximum of rho' along the cut (rewritten)
   microphone_N.dat the four microphones' samples of the frame (appended)
   u1-cfl.dat       every step's cfl and the running deviation from CFLWHISH
+taylor-angle-xy, per frame:
+  u1-NNNN.dat      the whole rho' grid (axes in units of 1/ETA)
+  u1-cut-NNNN.dat  the numeric and analytic rho' along the propagation cut
+  u1-error.dat     the cut's relative L1, L2 and Linf errors (appended)
+  u1-cfl.dat       as above
 
 Numbers are written as C++ ostreams write them: 6 significant digits (%g),
-T with 5 (setprecision(5)); floats are the device's FP32 values.
+T with 5 (setprecision(5)); the values are the device's, in the run's
+precision (float for FP32 contexts, double for FP64 ones).
 """
 import os
 
@@ -20,6 +27,29 @@ import numpy as np
 def _g(v, prec=6):
     """operator<< of an ostream with the default float format."""
     return "%.*g" % (prec, float(v))
+
+
+def _export_cfl(ex, frame, flat, nx, steps):
+    """u1-cfl.dat, the same in both apps (apps/hifu-beam/data_export.h:143-175,
+    apps/taylor-angle-xy/data_export.h:141-172): for the frame's steps with
+    T > 0 but the very first, n, T, frame, cfl and sqrt(sum (cfl - CFLWHISH)^2
+    / (frame + 1)).  The difference is taken in the run's precision, squared
+    in double and summed into a float (the static cfl_std_tmp); the mean is a
+    float division and its square root a double."""
+    R = flat.dtype.type
+    with open(ex._file("u1-cfl.dat"), "w" if frame == 0 else "a") as f:
+        if frame == 0:
+            f.write("# n T frame cfl std\n")
+            ex.cfl_std_tmp = np.float32(0.0)
+        for k in range(steps):
+            if flat[k] > 0:
+                if ex.first == 0:
+                    d = float(R(flat[k + nx]) - R(ex.cfl_wish))
+                    ex.cfl_std_tmp = np.float32(float(ex.cfl_std_tmp) + d * d)
+                    std = float(ex.cfl_std_tmp / np.float32(frame + 1)) ** 0.5
+                    f.write("%d %s %d %s %s\n" % (k + ex.previous_n + 1, _g(flat[k]),
+                                                  frame, _g(flat[k + nx]), _g(std)))
+                ex.first = 0
 
 
 class HifuExport:
@@ -46,12 +76,12 @@ class HifuExport:
         return os.path.join(self.path, name)
 
     def __call__(self, frame, u1, measure, T, n, mdx, mdy):
-        """Write frame `frame` (0, 1, ...): u1 (ny, nx) float32 state, measure
-        (ny, nx) float32 measurement grid, T after the frame, n accepted steps
-        so far, mdx/mdy the moving-window offset."""
+        """Write frame `frame` (0, 1, ...): u1 (ny, nx) state, measure (ny, nx)
+        measurement grid (float32, or float64 from an FP64 run), T after the
+        frame, n accepted steps so far, mdx/mdy the moving-window offset."""
         ny, nx = u1.shape
-        flat = np.ascontiguousarray(measure, np.float32).reshape(-1)
-        Tstr = _g(T, 5)
+        flat = np.ascontiguousarray(measure).reshape(-1)
+        Tstr = _g(flat.dtype.type(T), 5)
         fstr = "%04d" % frame
         # u1 for the whole domain (data_export.h:35-75)
         with open(self._file("u1-" + fstr + ".dat"), "w") as f:
@@ -91,21 +121,7 @@ class HifuExport:
                     f.write("%d %s %s\n" % (k + self.previous_n + 1, _g(flat[k]),
                                             _g(flat[k + (m + 1) * nx])))
         # cfl values (data_export.h:143-175)
-        with open(self._file("u1-cfl.dat"), "w" if frame == 0 else "a") as f:
-            if frame == 0:
-                f.write("# n T frame cfl std\n")
-                self.cfl_std_tmp = np.float32(0.0)
-            for k in range(steps):
-                if flat[k] > 0:
-                    if self.first == 0:
-                        # float cfl - float CFLWHISH, squared in double, summed
-                        # into a float; the mean is a float division
-                        d = float(np.float32(flat[k + nx]) - np.float32(self.cfl_wish))
-                        self.cfl_std_tmp = np.float32(float(self.cfl_std_tmp) + d * d)
-                        std = float(self.cfl_std_tmp / np.float32(frame + 1)) ** 0.5
-                        f.write("%d %s %d %s %s\n" % (k + self.previous_n + 1, _g(flat[k]),
-                                                      frame, _g(flat[k + nx]), _g(std)))
-                    self.first = 0
+        _export_cfl(self, frame, flat, nx, steps)
         self.previous_n = n
 
 
@@ -130,3 +146,123 @@ def hifu_frames(ctx, prob, deploy_path, frames, dt_print=1.0, inst=0, collect=No
            info["md_y"])
         Tprint += dt_print
     return ex
+
+
+class TaylorExport:
+    """dataExport of apps/taylor-angle-xy/data_export.h:14-176 for one instance
+    of a Context with Taylor data collection on (``Context.set_collect`` with
+    ``workloads.taylor_collect``).
+
+    nx, ny: the grid; eta: ETA = NX/XMAX; theta: ISPTHETA; cut_half: the cut's
+    half-length (the cut file and the errors take entries 0..2 cut_half - 1);
+    cfl_wish: CFLWHISH; fp64: the run's precision (the reference's
+    PRECISION=2), which sets the type of ETA, cos(ISPTHETA) and T in the
+    formulas; grids: also write u1-NNNN.dat (the whole grid, the largest
+    file by far).  The static counters of the reference (previous_n, first,
+    cfl_std_tmp) are members; errors holds every frame's (L1, L2, Linf)."""
+
+    def __init__(self, deploy_path, nx, ny, eta, theta, cut_half, cfl_wish, fp64=False,
+                 grids=True):
+        self.path = deploy_path
+        self.nx, self.ny = int(nx), int(ny)
+        self.R = np.float64 if fp64 else np.float32
+        self.eta = self.R(eta)
+        self.cos = self.R(np.cos(self.R(theta)))
+        self.length = 2 * int(cut_half)
+        self.cfl_wish = cfl_wish
+        self.grids = grids
+        self.previous_n = 0
+        self.first = 1
+        self.cfl_std_tmp = np.float32(0.0)
+        self.errors = []
+        # the cut's abscissa: -5 + i/(ETA cos(ISPTHETA)), stored in a float
+        # (data_export.h:92-93)
+        i = np.arange(self.length, dtype=np.float32)
+        den = self.R(np.float32(self.eta)) * self.cos
+        self.xi = (-5.0 + (i.astype(self.R) / den).astype(np.float64)).astype(np.float32)
+        os.makedirs(deploy_path, exist_ok=True)
+
+    def _file(self, name):
+        return os.path.join(self.path, name)
+
+    def __call__(self, frame, measure, T, n, mdx=0, mdy=0, u1=None):
+        """Write frame `frame`: measure = the first 4 (or more) rows of the
+        measurement grid, T after the frame, n accepted steps so far, mdx/mdy
+        the window offset; u1 (ny, nx) the state's rho' (needed with grids)."""
+        nx, ny = self.nx, self.ny
+        flat = np.ascontiguousarray(measure).reshape(-1)
+        Tstr = _g(self.R(T), 5)
+        fstr = "%04d" % frame
+        if self.grids:                                   # data_export.h:35-75
+            with open(self._file("u1-" + fstr + ".dat"), "w") as f:
+                f.write("# T=" + Tstr + "\n")
+                f.write("# Lines are parallel to y-axis, an columns parallel to x-axis.\n")
+                f.write("# First line are y values, and first column x values.\n")
+                f.write("# Left-top corner is the number of columns\n")
+                e32 = np.float32(self.eta)
+                ys = (np.arange(ny) + mdy - ny // 2).astype(np.float32) / e32
+                f.write(str(ny) + " " + "".join(_g(y) + " " for y in ys) + " \n")
+                xs = (np.arange(nx) + mdx).astype(np.float32) / e32
+                for i in range(nx):
+                    f.write(_g(xs[i]) + " " + "".join(_g(v) + " " for v in u1[:, i]) + " \n")
+        num = flat[2 * nx:2 * nx + self.length]
+        ana = flat[3 * nx:3 * nx + self.length]
+        with open(self._file("u1-cut-" + fstr + ".dat"), "w") as f:   # :77-103
+            f.write("# T=" + Tstr + "\n")
+            f.write("# i xi numeric analytic \n")
+            f.write("".join("%d %s %s %s  \n" % (i, _g(self.xi[i]), _g(num[i]), _g(ana[i]))
+                            for i in range(self.length)))
+        from . import cut_errors
+        err = cut_errors(num, ana)                       # :96-122
+        self.errors.append(tuple(float(v) for v in err))
+        with open(self._file("u1-error.dat"), "w" if frame == 0 else "a") as f:   # :124-139
+            if frame == 0:
+                f.write("# frame T error_L1 error_L2 error_Li\n")
+            f.write("%d %s %s %s %s\n" % (frame, Tstr, _g(err[0]), _g(err[1]), _g(err[2])))
+        _export_cfl(self, frame, flat, nx, n - self.previous_n)   # :141-172
+        self.previous_n = n
+
+
+def taylor_frames(ctx, deploy_paths, final_time=100.0, dt_print=1.0, grids=False,
+                  collect=None, on_frame=None):
+    """calcFrame's frame loop (common/main.cu:248-756) for every instance of a
+    Context of taylor-angle-xy problems: frames of dt_print from T0 (all
+    instances share the frame times; each keeps its own dt), data collection
+    on, each frame exported into that instance's deploy path, until T >
+    final_time (main.cu:741).  collect: per instance set_collect options
+    (default: workloads.taylor_collect in the context's precision).
+    on_frame(frame): called after each frame's steps (timing hooks).
+    Returns the TaylorExport objects."""
+    from .workloads import taylor_collect
+    probs = ctx.problems
+    n = len(probs)
+    exps = []
+    for k, p in enumerate(probs):
+        opts = taylor_collect(p, ctx.fp64) if collect is None else collect[k]
+        ctx.set_collect(k, **opts)
+        exps.append(TaylorExport(deploy_paths[k], p.nx, p.ny, opts["eta"], p.taylor["theta"],
+                                 opts["cut_half"], p.cfl_wish, fp64=ctx.fp64, grids=grids))
+    # every instance's frames end at T0 + dt_print, T0 + 2 dt_print, ...
+    # (main.cu:263, :652); with different T0 the instances share the frame
+    # ends where they overlap, and an instance joins once its first one comes
+    done = [False] * n
+    frames = [0] * n
+    Tprint = min(p.T0 for p in probs) + dt_print
+    step = 0
+    while not all(done):
+        ctx.step_until(Tprint)
+        if on_frame is not None:
+            on_frame(step)
+        for k in range(n):
+            if done[k] or Tprint < probs[k].T0 + dt_print * (1 - 1e-9):
+                continue
+            info = ctx.info(k)
+            u1 = ctx.collect(k)[0][0] if grids else None
+            exps[k](frames[k], ctx.measure(k, rows=4), info["T"], info["n_accepted"],
+                    info["md_x"], info["md_y"], u1=u1)
+            frames[k] += 1
+            if final_time > 0 and info["T"] > final_time:
+                done[k] = True
+        Tprint += dt_print
+        step += 1
+    return exps

@@ -105,11 +105,13 @@ TAYLOR_C = 2.1972245773362196   # apps/taylor-angle-xy/parameters.h:105
 
 
 def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_REFERENCE_FOLD,
-           moving=False):
+           moving=False, cflw=0.9):
     """apps/taylor-angle-xy: NX = NY = 64*ETAC, XMAX = YMAX = 12.5; the exact
     Taylor profile is both the initial state and the Dirichlet ghost data.
-    moving: the app's moving window, MDVX = cos(theta), MDVY = sin(theta),
-    MDT = XMAX/2/cos(theta) (apps/taylor-angle-xy/parameters.h:92-94)."""
+    cflw: CFLWHISH (CFLW/100).  The app's window velocity and start,
+    MDVX = cos(theta), MDVY = sin(theta), MDT = XMAX/2/cos(theta)
+    (apps/taylor-angle-xy/parameters.h:92-94), locate the measured cut;
+    moving: the window also moves (moveDomain)."""
     nx = ny = 64 * eta_mult
     dx = dy = xmax / (nx - 1)
     rhoa = -delta / beta
@@ -123,10 +125,26 @@ def taylor(eta_mult=1, theta=0.0, beta=1.0, delta=1e-7, xmax=12.5, cfl_mode=CFL_
     r = rhoa * np.tanh(TAYLOR_C * (X * dx * math.cos(theta) + ((Y - jc) * dy + yoff)
                                    * math.sin(theta) - T0))
     q = np.stack([r, r * math.cos(theta), r * math.sin(theta)])
-    t = dict(rhoa=rhoa, c=TAYLOR_C, theta=theta, yoff=yoff, jc=jc)
+    t = dict(rhoa=rhoa, c=TAYLOR_C, theta=theta, yoff=yoff, jc=jc, xmax=xmax,
+             mdvx=math.cos(theta), mdvy=math.sin(theta), mdt=mdt)
     md = (math.cos(theta), math.sin(theta), mdt) if moving else None
-    return Problem(nx, ny, dx, dy, beta, delta, q, cfl_max=1.0, cfl_wish=0.9, dt0=0.005,
+    return Problem(nx, ny, dx, dy, beta, delta, q, cfl_max=1.0, cfl_wish=cflw, dt0=0.005,
                    T0=T0, bc=BC_TAYLOR, cfl_mode=cfl_mode, taylor=t, moving=md)
+
+
+def taylor_collect(prob, fp64=False):
+    """fivonagi_set_collect options of the taylor-angle-xy app: ETA = NX/XMAX,
+    the cut half-length static_cast<int>(5.0*ETA*cos(ISPTHETA)) and
+    tan(ISPTHETA), with ETA and the angle's cosine in the build's precision
+    (float for PRECISION=1, apps/taylor-angle-xy/data_collect.h:49-55)."""
+    t = prob.taylor
+    if fp64:
+        eta = prob.nx / t["xmax"]
+        cos = math.cos(t["theta"])
+    else:
+        eta = float(np.float32(prob.nx) / np.float32(t["xmax"]))
+        cos = float(np.cos(np.float32(t["theta"])))
+    return dict(eta=eta, cut_half=int(5.0 * eta * cos), tan_theta=math.tan(t["theta"]))
 
 
 def hifu(etac=1, ispam=30.0, cflw=0.9, nx=None, ny=None, moving=True,

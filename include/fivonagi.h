@@ -89,12 +89,18 @@ typedef struct {
   double skew;             /* height skew of the work items in use            */
   int32_t rotate, pad;     /* 1: skew levels rotated per strip                */
   int32_t md_x, md_y;      /* moving-window offset MDX, MDY                   */
+  int32_t fp64, pad2;      /* 1: FP64 context (fivonagi_init_f64)             */
 } fivonagi_info_t;
 
 /* gpu_init (common/data_definitions.h:523-871) + init() upload
    (apps/APP/init.h): n_inst instances, p[k] and q_host[3k..3k+2] (q1,q2,q3). */
 int fivonagi_init(fivonagi_ctx **ctx, const fivonagi_params *p, int n_inst,
                   const float *const *q_host);
+/* The same with the reference's PRECISION=2 (common/main.cu:11-24): state,
+   tables, arithmetic and the cfl reduction in FP64.  The adjoint and the data
+   collection are FP32 only. */
+int fivonagi_init_f64(fivonagi_ctx **ctx, const fivonagi_params *p, int n_inst,
+                      const double *const *q_host);
 
 /* Run on the caller's HIP stream (hipStream_t) instead of the context's own. */
 int fivonagi_set_stream(fivonagi_ctx *ctx, void *hip_stream);
@@ -138,8 +144,12 @@ int fivonagi_set_skew(fivonagi_ctx *ctx, double skew, int rotate);
 
 /* dataCollect/dataExport stand-in (apps/hifu-beam/data_export.h:29-75):
    copy instance inst's state and counters to the host.  Any pointer may be NULL. */
+/* (An FP64 context's state is rounded to float here; fivonagi_collect_f64
+   returns it in full, and an FP32 context's widened.) */
 int fivonagi_collect(fivonagi_ctx *ctx, int inst, float *q1, float *q2, float *q3,
                      double *T, double *dt, int64_t *n_accepted, int64_t *n_rejected);
+int fivonagi_collect_f64(fivonagi_ctx *ctx, int inst, double *q1, double *q2, double *q3,
+                         double *T, double *dt, int64_t *n_accepted, int64_t *n_rejected);
 int fivonagi_info(fivonagi_ctx *ctx, int inst, fivonagi_info_t *info);
 
 /* ---- discrete adjoint (no counterpart in the reference) -------------------
@@ -170,25 +180,54 @@ int fivonagi_step_adjoint(fivonagi_ctx *ctx, int64_t nsteps, const fivonagi_obje
 /* dJ/d(state at the start of the last fivonagi_step_adjoint call), 3 grids. */
 int fivonagi_adjoint_state(fivonagi_ctx *ctx, int inst, float *l1, float *l2, float *l3);
 
-/* ---- data collection (apps/hifu-beam/data_collect.h:7-107) ---------------
-   After every accepted step, on row jcut of the new state (before the
-   moving-window shift), into a measurement grid of nx*ny floats laid out as
-   the reference's measure1: row 0 = T and row 1 = cfl of the frame's steps,
-   rows 2..5 = u1 at the four microphones (x index mic[k] of the unmoved
-   grid, i.e. i + MDX == mic[k]), from row 6 on = max of u1 over the steps
-   where |(i + MDX)/eta - T| <= 0.5, at flat index i + MDX + 6 nx.  Step n of a
-   frame is entry n - 1 - n0, n0 = the step count when the frame started
-   (fivonagi_step_until starts a frame).  While T <= 0 the grid is zeroed.
-   More than nx steps in one frame stops the call with code 4 (simError 4). */
+/* ---- data collection (apps/hifu-beam/data_collect.h:7-107,
+   apps/taylor-angle-xy/data_collect.h:7-98) ----------------------------------
+   After every accepted step, into a measurement grid of nx*ny values of the
+   context's precision, laid out as the reference's measure1: row 0 = T and
+   row 1 = cfl of the frame's steps (step n of a frame is entry n - 1 - n0,
+   n0 = the step count when the frame started; fivonagi_step_until starts a
+   frame).  While T <= 0 the grid is zeroed.  More than nx steps in one frame
+   stops the call with code 4 (simError 4).
+   hifu app (and every boundary but the Taylor one), on row jcut of the new
+   state before the moving-window shift: rows 2..5 = u1 at the four
+   microphones (x index mic[k] of the unmoved grid, i.e. i + MDX == mic[k]),
+   from row 6 on = max of u1 over the steps where |(i + MDX)/eta - T| <= 0.5,
+   at flat index i + MDX + 6 nx.
+   taylor-angle-xy app (bc = FIVONAGI_BC_TAYLOR), along the propagation
+   direction through the front's centre icenter = int(T MDVX/dx) - MDX,
+   jcenter = NY/2 + int((T - MDT) MDVY/dy) - MDY, for i in
+   [icenter - cut_half, icenter + cut_half]: row 2 = rho' interpolated in y at
+   ja = jcenter + (i - icenter)(dx/dy) tan(theta), row 3 = the closed form at
+   the nearer row, entry i - icenter + cut_half.  MDVX, MDVY, MDT are the
+   params' md_vx, md_vy, md_t (also when md_on = 0). */
 typedef struct {
   int32_t jcut;            /* JCUT (2 in the hifu app)                        */
   int32_t mic[4];          /* microphone x indices: round(ISPF*{.5,.9}), ISPF, round(ISPF*1.1) */
   double eta;              /* ETA = NX/XMAX cells per length unit             */
+  int32_t cut_half;        /* Taylor: static_cast<int>(5.0*ETA*cos(ISPTHETA)) */
+  int32_t pad;
+  double tan_theta;        /* Taylor: tan(ISPTHETA)                           */
 } fivonagi_collect_opts;
 /* opts = NULL switches collection off for the instance. */
 int fivonagi_set_collect(fivonagi_ctx *ctx, int inst, const fivonagi_collect_opts *opts);
-/* Copy the measurement grid (nx*ny floats) of instance inst to the host. */
+/* Copy the measurement grid (nx*ny values) of instance inst to the host:
+   floats from an FP32 context, doubles (the _f64 form) from an FP64 one. */
 int fivonagi_measure(fivonagi_ctx *ctx, int inst, float *measure);
+int fivonagi_measure_f64(fivonagi_ctx *ctx, int inst, double *measure);
+/* The first `rows` rows only (rows*nx values), e.g. rows 0..3 of a Taylor
+   instance each frame (apps/taylor-angle-xy/data_export.h:77-176 reads no more). */
+int fivonagi_measure_rows(fivonagi_ctx *ctx, int inst, int rows, float *measure);
+int fivonagi_measure_rows_f64(fivonagi_ctx *ctx, int inst, int rows, double *measure);
+
+/* Host-side: the relative L1, L2 and Linf errors of a Taylor cut
+   (apps/taylor-angle-xy/data_export.h:96-122: sum|a-n|/sum|a|,
+   sqrt(sum (a-n)^2/sum a^2), max|a-n|/max|a|) with the reference's float
+   accumulators, over entries 0..length-1 of the numeric and analytic rows
+   (rows 2 and 3 of the measurement grid; length = 2 cut_half). */
+int fivonagi_cut_errors(const float *numeric, const float *analytic, int length,
+                        float errors[3]);
+int fivonagi_cut_errors_f64(const double *numeric, const double *analytic, int length,
+                            float errors[3]);
 
 /* clean_gpu (common/data_definitions.h:415-517) */
 void fivonagi_destroy(fivonagi_ctx *ctx);

@@ -116,3 +116,13 @@ int orc_sweep_f64(const orc_params *p, int axis, double *q1, double *q2,
   ctx_free_f64(&c);
   return rc;
 }
+
+int orc_collect_f64(const orc_params *p, const double *q1, const orc_state *st) {
+  if (check_grid(p) || !p->measure || !st) return ORC_ERR_ARG;
+  ctx_t_f64 c;
+  if (ctx_alloc_f64(&c, p->nx, p->ny)) { ctx_free_f64(&c); return ORC_ERR_ARG; }
+  memcpy(c.C[0], q1, c.n * sizeof(double));
+  int rc = collect_f64(&c, p, st);
+  ctx_free_f64(&c);
+  return rc;
+}

@@ -91,6 +91,17 @@ typedef struct {
   int32_t jcut, mic[4], pad1;
   double eta;
   int64_t n0frame;
+  /* ORC_BC_TAYLOR instead (apps/taylor-angle-xy/data_collect.h:7-98): rows
+     0, 1 as above; for |i - icenter| <= cut_half, icenter =
+     int(T md_vx/dx) - MDX, jcenter = ny/2 + int((T - md_t) md_vy/dy) - MDY,
+     row 2 entry i - icenter + cut_half = rho' interpolated in y at ja =
+     jcenter + (i - icenter)(dx/dy) tan_theta (float), between the rows of the
+     thread whose j - ja lies in (-1/2, 1/2], row 3 = the closed form at that
+     row.  md_vx, md_vy, md_t are read also when md_on = 0.  collect_f32: T,
+     dx, dy, the constants and the closed form in float (the reference built
+     with PRECISION=1), else double. */
+  int32_t cut_half, collect_f32;
+  double tan_theta;
 } orc_params;
 
 typedef struct {
@@ -129,6 +140,9 @@ int orc_source_f64(const orc_params *p, const double *in1, const double *in2,
                    double dt);
 /* One y-sweep (fv.h:195-499, 722-930, 994-1054) or x-sweep on a state whose
    ghost cells are already set; returns the per-cell cfl in cfl (may be NULL). */
+/* dataCollect on the state whose rho' is q1, at st's T, counts and MDX, MDY
+   (st->last_cfl = the step's cfl), into p->measure. */
+int orc_collect_f64(const orc_params *p, const double *q1, const orc_state *st);
 int orc_sweep_f64(const orc_params *p, int axis, double *q1, double *q2,
                   double *q3, double dt, double *cfl);
 

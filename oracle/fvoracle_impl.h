@@ -425,10 +425,78 @@ static void SFX(sweep)(SFX(ctx_t) *c, const orc_params *p, int axis, REAL dt) {
   SFX(write_limiters)(c, p, axis, c->B, c->A, dt);
 }
 
-/* dataCollect, apps/hifu-beam/data_collect.h:7-107, on the accepted state in
-   c->C at the updated T and n, before the window shift (main.cu:563-573).
-   Within one launch the threads touch disjoint entries, so the column loop
-   below is the same computation.  Returns 0 or ORC_ERR_COLLECT. */
+/* u1 of the state as tex2D reads it: coordinates clamped to the grid */
+static double SFX(tex_u1)(const SFX(ctx_t) *c, int i, int j) {
+  i = i < 0 ? 0 : (i > c->nx - 1 ? c->nx - 1 : i);
+  j = j < 0 ? 0 : (j > c->ny - 1 ? c->ny - 1 : j);
+  return (double)c->C[0][(size_t)i + (size_t)j * c->nx];
+}
+
+/* The cut of apps/taylor-angle-xy/data_collect.h:41-96, every thread (i, j)
+   of the reference's grid visited in turn (at most one j per column
+   passes the test on dj).  ja and the weights are float as declared there; T,
+   dx, dy, the constants and the closed form are float when the reference is
+   built with PRECISION=1 (collect_f32), else double.  The interpolation
+   keeps the state's own precision. */
+static void SFX(collect_taylor)(SFX(ctx_t) *c, const orc_params *p, const orc_state *st,
+                                double *m) {
+  const int NX = c->nx, NY = c->ny, L = p->cut_half;
+  const int f32 = p->collect_f32;
+  int icenter, jcenter;
+  if (f32) {                                           /* data_collect.h:43-46 */
+    const float T = (float)st->T, dx = (float)p->dx, dy = (float)p->dy;
+    icenter = (int)(T * (float)p->md_vx / dx) - st->mdx;
+    jcenter = NY / 2 + (int)((T - (float)p->md_t) * (float)p->md_vy / dy) - st->mdy;
+  } else {
+    icenter = (int)(st->T * p->md_vx / p->dx) - st->mdx;
+    jcenter = NY / 2 + (int)((st->T - p->md_t) * p->md_vy / p->dy) - st->mdy;
+  }
+  const double cs = cos(p->tay_theta), sn = sin(p->tay_theta);
+  for (int i = 0; i < NX; ++i) {
+    if (abs(i - icenter) > L) continue;                /* :48-50 */
+    float ja;                                          /* :52-55 */
+    if (f32)
+      ja = fmaf((float)(i - icenter) * ((float)p->dx / (float)p->dy), (float)p->tan_theta,
+                (float)jcenter);
+    else
+      ja = (float)fma((double)(float)(i - icenter) * (p->dx / p->dy), p->tan_theta,
+                      (double)(float)jcenter);
+    for (int j = 0; j < NY; ++j) {
+      const float dj = (float)j - ja;
+      double val;
+      if (dj >= 0.0f && dj <= 0.5f) {                  /* :58-68 */
+        const float djm = (float)(j - 1) - ja;
+        val = -(double)djm * SFX(tex_u1)(c, i, j) + (double)dj * SFX(tex_u1)(c, i, j - 1);
+      } else if (dj < 0.0f && -dj < 0.5f) {            /* :69-79 */
+        const float djp = (float)(j + 1) - ja;
+        val = -(double)dj * SFX(tex_u1)(c, i, j + 1) + (double)djp * SFX(tex_u1)(c, i, j);
+      } else {
+        continue;
+      }
+      const int iw = i - (icenter - L);                /* :87-89 */
+      double an;                                       /* :90-95 */
+      if (f32) {
+        const float T = (float)st->T, dx = (float)p->dx, dy = (float)p->dy;
+        const float arg = (float)(i + st->mdx) * dx * (float)cs +
+                          ((float)(j + st->mdy - NY / 2) * dy + (float)p->md_t * (float)sn) *
+                              (float)sn - T;
+        an = (double)((float)p->tay_rhoa * tanhf((float)p->tay_c * arg));
+      } else {
+        const double arg = (double)(i + st->mdx) * p->dx * cs +
+                           ((double)(j + st->mdy - NY / 2) * p->dy + p->md_t * sn) * sn - st->T;
+        an = p->tay_rhoa * tanh(p->tay_c * arg);
+      }
+      m[(size_t)iw + 2 * (size_t)NX] = val;
+      m[(size_t)iw + 3 * (size_t)NX] = an;
+    }
+  }
+}
+
+/* dataCollect, apps/hifu-beam/data_collect.h:7-107 (apps/taylor-angle-xy/
+   data_collect.h:7-98 for a Taylor instance), on the accepted state in c->C at
+   the updated T and n, before the window shift (main.cu:563-573).  Within one
+   launch the threads touch disjoint entries, so the column loop below is the
+   same computation.  Returns 0 or ORC_ERR_COLLECT. */
 static int SFX(collect)(SFX(ctx_t) *c, const orc_params *p, const orc_state *st) {
   double *m = p->measure;
   const int NX = c->nx, NY = c->ny;
@@ -438,6 +506,12 @@ static int SFX(collect)(SFX(ctx_t) *c, const orc_params *p, const orc_state *st)
   }
   const int64_t nf = st->n_accepted - 1 - p->n0frame;
   if (nf >= NX) return ORC_ERR_COLLECT;               /* data_collect.h:96 */
+  if (p->bc == ORC_BC_TAYLOR) {                        /* taylor data_collect.h:34-40 */
+    m[nf] = p->collect_f32 ? (double)(float)st->T : st->T;
+    m[nf + NX] = st->last_cfl;
+    SFX(collect_taylor)(c, p, st, m);
+    return 0;
+  }
   const REAL *row = c->C[0] + (size_t)p->jcut * NX;
   for (int i = 0; i < NX; ++i) {
     const double u1 = (double)row[i];

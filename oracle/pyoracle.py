@@ -40,6 +40,7 @@ class Params(C.Structure):
         ("measure", C.POINTER(C.c_double)),
         ("jcut", C.c_int32), ("mic", C.c_int32 * 4), ("pad1", C.c_int32),
         ("eta", C.c_double), ("n0frame", C.c_int64),
+        ("cut_half", C.c_int32), ("collect_f32", C.c_int32), ("tan_theta", C.c_double),
     ]
 
 
@@ -82,6 +83,8 @@ def _load(name):
     lib.orc_source_f64.restype = C.c_int
     lib.orc_sweep_f64.argtypes = [P, C.c_int, dp, dp, dp, C.c_double, dp]
     lib.orc_sweep_f64.restype = C.c_int
+    lib.orc_collect_f64.argtypes = [P, dp, C.POINTER(State)]
+    lib.orc_collect_f64.restype = C.c_int
     B = C.POINTER(Box)
     lib.orc_line_step_f64.argtypes = [P, dp, dp, dp, C.c_double, C.c_double]
     lib.orc_line_step_f64.restype = C.c_int
@@ -128,7 +131,8 @@ class Problem:
         self.moving = None if moving is None else tuple(float(v) for v in moving)
         self.T_stop = None if T_stop is None else float(T_stop)
         self.hifu = hifu  # dict(ampl, frec, a, f)
-        # data collection: dict(jcut, mic, eta); measure (ny, nx) float64 is
+        # data collection: dict(jcut, mic, eta) (hifu) or dict(eta, cut_half,
+        # tan_theta, f32) (Taylor); measure (ny, nx) float64 is
         # the measurement grid the runs write into, n0frame the frame start
         self.collect = None
         self.measure = None
@@ -158,6 +162,9 @@ class Problem:
         if self.moving is not None:
             p.md_on = 1
             p.md_vx, p.md_vy, p.md_t = self.moving
+        elif self.taylor and "mdt" in self.taylor:
+            t = self.taylor
+            p.md_vx, p.md_vy, p.md_t = t["mdvx"], t["mdvy"], t["mdt"]
         if self.T_stop is not None:
             p.stop_at_T = 1
             p.T_stop = self.T_stop
@@ -166,9 +173,12 @@ class Problem:
                 self.measure = np.zeros((self.ny, self.nx))
             c = self.collect
             p.measure = _ptr(self.measure, C.c_double)
-            p.jcut, p.eta, p.n0frame = int(c["jcut"]), float(c["eta"]), int(self.n0frame)
+            p.jcut, p.eta, p.n0frame = int(c.get("jcut", 0)), float(c["eta"]), int(self.n0frame)
             for k in range(4):
-                p.mic[k] = int(c["mic"][k])
+                p.mic[k] = int(c.get("mic", (0, 0, 0, 0))[k])
+            p.cut_half = int(c.get("cut_half", 0))
+            p.tan_theta = float(c.get("tan_theta", 0.0))
+            p.collect_f32 = int(bool(c.get("f32", False)))
         return p
 
 
@@ -272,3 +282,14 @@ def adjoint(prob, q0, nsteps, box):
                                C.byref(J), _ptr(ga, C.c_double), _ptr(gp, C.c_double),
                                _ptr(gq, C.c_double))
     return J.value, ga, gp, gq, rc
+
+
+def collect(prob, q1, T, n_accepted, mdx=0, mdy=0, cfl=0.0):
+    """dataCollect (either app) on a state whose rho' is q1 (ny, nx), after
+    accepted step n_accepted ending at T, into prob.measure.  Returns rc."""
+    p = prob.cparams()
+    st = State()
+    st.T, st.n_accepted, st.mdx, st.mdy, st.last_cfl = float(T), int(n_accepted), int(mdx), \
+        int(mdy), float(cfl)
+    q1 = np.ascontiguousarray(q1, np.float64)
+    return lib().orc_collect_f64(C.byref(p), _ptr(q1, C.c_double), C.byref(st))

@@ -79,3 +79,138 @@ def test_hifu_export_round_trip(tmp_path):
     assert sorted(os.listdir(tmp_path)) == sorted(
         ["u1-0000.dat", "u1-0001.dat", "u1-cut-0000.dat", "u1-cut-0001.dat", "u1max.dat",
          "u1-cfl.dat"] + [f"microphone_{k}.dat" for k in range(1, 5)])
+
+
+# ----------------------------------------------------------- taylor-angle-xy --
+# apps/taylor-angle-xy/data_export.h:14-176, plots/error.py,
+# plots/error_table_{1,2,3,4}.py
+
+def _float_accumulate(num, ana):
+    """data_export.h:96-122 written out with numpy scalars: float
+    accumulators, the squares in double."""
+    f32 = np.float32
+    l1a = l1b = l2a = l2b = lia = lib = f32(0)
+    for a, n in zip(ana, num):
+        d, aa = abs(a - n), abs(a)
+        l1a = f32(float(l1a) + float(d)) if d.dtype == np.float64 else f32(l1a + d)
+        l1b = f32(float(l1b) + float(aa)) if d.dtype == np.float64 else f32(l1b + aa)
+        l2a = f32(float(l2a) + float(d) ** 2)
+        l2b = f32(float(l2b) + float(aa) ** 2)
+        lia = f32(max(float(d), float(lia)))
+        lib = f32(max(float(aa), float(lib)))
+    return np.array([l1a / l1b, f32(float(l2a / l2b) ** 0.5), lia / lib], np.float32)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_cut_errors_match_the_float_accumulation(dtype):
+    import fivonagi_amd as fa
+    rng = np.random.default_rng(3)
+    ana = (1e-7 * np.tanh(np.linspace(-4, 4, 300))).astype(dtype)
+    num = (ana + 1e-9 * rng.standard_normal(300)).astype(dtype)
+    np.testing.assert_array_equal(fa.cut_errors(num, ana), _float_accumulate(num, ana))
+
+
+def test_cut_errors_closed_form():
+    # numeric = analytic (1 + eps): every relative error is eps
+    import fivonagi_amd as fa
+    ana = -1e-7 * np.tanh(np.linspace(-3, 5, 101))
+    e = fa.cut_errors(ana * (1 + 1e-3), ana)
+    np.testing.assert_allclose(e, 1e-3, rtol=1e-4)
+    assert np.isnan(fa.cut_errors(np.zeros(4), np.zeros(4))).all()   # T < 0: 0/0
+
+
+def test_taylor_export_round_trip(tmp_path):
+    from fivonagi_amd.export import TaylorExport
+    import fivonagi_amd as fa
+    nx, ny, eta, theta, L = 64, 64, 5.12, np.pi / 8, 23
+    ex = TaylorExport(str(tmp_path), nx, ny, eta, theta, L, 0.9, fp64=False, grids=True)
+    rng = np.random.default_rng(2)
+    n = 0
+    for frame, steps in enumerate((4, 6)):
+        m = np.zeros((4, nx), np.float32)
+        m[0, :steps] = np.linspace(-0.5, 0.5, steps) + frame
+        m[1, :steps] = 0.9 + 0.01 * rng.standard_normal(steps)
+        m[3, :2 * L + 1] = -1e-7 * np.tanh(np.linspace(-3, 3, 2 * L + 1))
+        m[2] = m[3] * (1 + 1e-3 * rng.standard_normal(nx))
+        u1 = rng.standard_normal((ny, nx)).astype(np.float32)
+        n += steps
+        ex(frame, m, T=float(m[0, steps - 1]), n=n, mdx=5, mdy=2, u1=u1)
+    cut = np.loadtxt(tmp_path / "u1-cut-0001.dat")
+    assert cut.shape == (2 * L, 4)                     # entries 0 .. 2L-1
+    np.testing.assert_array_equal(cut[:, 0], np.arange(2 * L))
+    np.testing.assert_allclose(cut[:, 1], -5.0 + np.arange(2 * L) / (eta * np.cos(theta)),
+                               rtol=1e-5)
+    np.testing.assert_allclose(cut[:, 2], m[2, :2 * L], rtol=1e-5)
+    np.testing.assert_allclose(cut[:, 3], m[3, :2 * L], rtol=1e-5)
+    err = np.loadtxt(tmp_path / "u1-error.dat")
+    assert err.shape == (2, 5) and list(err[:, 0]) == [0, 1]
+    np.testing.assert_allclose(err[1, 2:], fa.cut_errors(m[2, :2 * L], m[3, :2 * L]),
+                               rtol=1e-5)
+    with open(tmp_path / "u1-0001.dat") as f:
+        lines = f.read().splitlines()
+    assert lines[0].startswith("# T=") and len(lines) == 4 + 1 + nx
+    ys = [float(t) for t in lines[4].split()]
+    assert ys[0] == ny and ys[1] == pytest.approx((0 + 2 - ny // 2) / eta, rel=1e-5)
+    row = [float(t) for t in lines[5 + 7].split()]     # column i = 7
+    assert row[0] == pytest.approx((7 + 5) / eta, rel=1e-5)
+    np.testing.assert_allclose(row[1:], u1[:, 7], rtol=1e-5)
+    cfl = np.loadtxt(tmp_path / "u1-cfl.dat")
+    # T > 0 steps but the very first: frame 0 has 2 (one skipped), frame 1 all 6
+    assert cfl.shape[0] == 1 + 6
+    assert list(cfl[:, 0]) == [4] + [5, 6, 7, 8, 9, 10]
+
+
+@pytest.mark.filterwarnings("ignore:Input line 1 contained no data")
+def test_convergence_tables_of_power_law_errors(tmp_path):
+    # errors = C eta^-r fit the line exactly: every rate is r, R^2 = 1, and the
+    # last halving (41 -> 82) is r too
+    from fivonagi_amd import taylor_study as ts
+    etas = np.array(ts.ETAS, float)
+    r = {1: 1.7, 2: 2.0}
+    err = np.zeros((5, 5, 2, 5, 3))
+    for pi, p in enumerate(ts.PRECISIONS):
+        for ei in range(3):
+            err[:, :, pi, :, ei] = (0.3 * (ei + 1) * etas ** -r[p])[:, None, None]
+    ts.write_tables(str(tmp_path), err)
+    t1 = np.loadtxt(tmp_path / "table_error_1" / "conv_rates_stats1.dat", dtype=str)
+    assert list(t1[:, 0]) == ["L1", "L2", "Li"]
+    np.testing.assert_allclose(t1[:, 1:7].astype(float), 1.7, atol=1e-6)
+    np.testing.assert_allclose(t1[:, 7].astype(float), 1.0, atol=1e-6)
+    t2 = np.loadtxt(tmp_path / "table_error_2" / "conv_rates_all.dat", dtype=str)
+    assert t2.shape == (2 * 3 * 5 * 5, 6) and t2[0, 1] == "cfl60-p1-angle00000"
+    np.testing.assert_allclose(t2[-1, 2:4].astype(float), 2.0, atol=1e-6)
+    t3 = np.loadtxt(tmp_path / "table_error_3" / "conv_rates_stats_angles-p2.dat", dtype=str)
+    assert t3.shape == (15, 11) and t3[0, 1] == "00000"
+    # %f: six decimals, as the reference's table prints them
+    np.testing.assert_allclose(t3[:, 9].astype(float), 0.3 * np.tile([1, 2, 3], 5) * 82.0 ** -2,
+                               rtol=0, atol=5e-7)
+    t4 = (tmp_path / "table_error_4" / "conv_rates_stats_angles-p1.dat").read_text()
+    assert t4.count("\\midrule") == 5 and "$\\theta_T=\\pi/32$" in t4
+    assert "1.7000 & 1.7000 & 1.7000 & 1.0000" in t4 and "$E_\\infty$" in t4
+
+
+def test_error_extraction_reads_the_last_cut(tmp_path):
+    # plots/error.py: the last u1-cut file of every run, at T = final time
+    from fivonagi_amd import taylor_study as ts
+    from fivonagi_amd.export import TaylorExport
+    etas, cfls, ps, angles = [5, 10], [90], [1], ["00000"]
+    expect = {}
+    for eta in etas:
+        d = tmp_path / "res" / ("d-" + ts.exec_name(eta, 90, 1, "00000"))
+        ex = TaylorExport(str(d), 64, 64, 5.12, 0.0, 10, 0.9, grids=False)
+        for frame, T in enumerate((9.0, 10.004)):
+            m = np.zeros((4, 64), np.float32)
+            m[3, :20] = np.linspace(1, 2, 20)
+            m[2, :20] = m[3, :20] * (1 + 0.01 * eta * (frame + 1))
+            ex(frame, m, T=T, n=frame + 1)
+        expect[eta] = 0.01 * eta * 2
+    err = ts.extract_errors(str(tmp_path / "res"), str(tmp_path / "dep"), etas, cfls, ps,
+                            angles, final_time=10)
+    assert err.shape == (2, 1, 1, 1, 3)
+    for ei, eta in enumerate(etas):
+        np.testing.assert_allclose(err[ei, 0, 0, 0], expect[eta], rtol=1e-4)
+    raw = np.fromfile(tmp_path / "dep" / "dr_error_etas_CPU" / "error.npy")
+    np.testing.assert_array_equal(raw, err.reshape(-1))
+    e = np.loadtxt(tmp_path / "dep" / "dr_error_etas_CPU" / ("d-" + ts.exec_name(5, 90, 1, "00000"))
+                   / "u1-error.dat")
+    assert e[0] == 1 and e[1] == pytest.approx(10.004)

@@ -7,6 +7,7 @@ GPU path is held to 1e-5 on runs of up to ~300 steps; longer runs are checked
 against the FP32 oracle's own error (the GPU may be at most 5x worse).
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -42,7 +43,10 @@ def run_oracle(oracle, prob, nsteps, dt_seq=None, dtype=np.float64, form=None):
 
 
 def report(name, e):
-    print(f"[parity] {name}: rel Linf q1={e[0]:.3e} q2={e[1]:.3e} q3={e[2]:.3e}", flush=True)
+    if len(e) == 3:
+        print(f"[parity] {name}: rel Linf q1={e[0]:.3e} q2={e[1]:.3e} q3={e[2]:.3e}", flush=True)
+    else:
+        print(f"[parity] {name}: " + " ".join(f"{v:.3e}" for v in e), flush=True)
 
 
 def small_pulse(nx=64, ny=64, **kw):
@@ -278,6 +282,137 @@ def test_hifu_frames_export(tmp_path):
     mic = np.loadtxt(tmp_path / "microphone_1.dat", comments="#")
     assert mic.shape[0] == st["n_accepted"] and mic[-1, 0] == st["n_accepted"]
     assert np.abs(mic[:, 2]).max() > 0
+
+
+TOL64 = 1e-10
+
+
+@pytest.mark.parametrize("case", ["pulse_dt", "pulse_400", "edge_drive", "rejections",
+                                  "taylor_window", "hifu_window", "batch"])
+def test_fp64_matches_fp64_oracle(oracle, case):
+    """FP64 contexts (the reference's PRECISION=2): state, tables, arithmetic and
+    the cfl reduction in double, against the FP64 oracle in the rho'-direct
+    form from the same unrounded initial state.  Counts exact, fields to 1e-10."""
+    dts = None
+    if case == "pulse_dt":
+        probs = [small_pulse(100, 72)]
+        # one entry past the last step: the dt of the attempt after it
+        n, dts = 50, np.full(55, 0.0045)
+    elif case == "pulse_400":
+        probs = [wl.gauss_pulse(400, 400)]
+        n = 200
+    elif case == "edge_drive":
+        probs = [wl.edge_drive(256, 128, halfwidth_cells=20.0)]
+        n = 300
+    elif case == "rejections":
+        probs = [small_pulse(64, 64, dt0=0.5)]
+        n = 20
+    elif case == "taylor_window":
+        probs = [wl.taylor(1, theta=math.pi / 6, moving=True)]
+        n = 110
+    elif case == "hifu_window":
+        probs = [wl.hifu(nx=128, ny=256, ispam=10.0)]
+        n = 150
+    else:
+        probs = [small_pulse(64, 64), wl.edge_drive(200, 96, halfwidth_cells=20.0)]
+        n = 40
+    with fa.Context(probs, fp64=True) as ctx:
+        if dts is not None:
+            ctx.set_dt_sequence(0, dts)
+        ctx.step(n)
+        outs = [(ctx.collect(k), ctx.info(k)) for k in range(len(probs))]
+    for k, p in enumerate(probs):
+        (q, st), info = outs[k]
+        assert q.dtype == np.float64 and info["fp64"] == 1
+        op = to_oracle(p, form=oracle.FORM_RHO1, dt_seq=dts if k == 0 else None)
+        qo, sto, rc = oracle.run(op, p.q0_f64.copy(), n)
+        assert rc == 0
+        assert st["n_accepted"] == sto.n_accepted and st["n_rejected"] == sto.n_rejected
+        assert (info["md_x"], info["md_y"]) == (sto.mdx, sto.mdy)
+        assert st["T"] == pytest.approx(sto.T, rel=1e-13)
+        e = rel_linf(q, qo)
+        report(f"fp64 {case} instance {k}", e)
+        assert max(e) < TOL64
+    if case == "rejections":
+        assert outs[0][0][1]["n_rejected"] > 0
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_taylor_collection_matches_oracle(oracle, fp64):
+    """dataCollect of the taylor-angle-xy app (apps/taylor-angle-xy/
+    data_collect.h:7-98) over frames with its moving window: T and cfl of
+    every step and the cut's numeric and closed-form rows.  FP32 runs a
+    prescribed dt so that T, and with it the cut's cells, is the oracle's
+    exactly; FP64 runs the CFL feedback."""
+    theta = math.pi / 8
+    prob = wl.taylor(1, theta=theta, moving=True)
+    opts = wl.taylor_collect(prob, fp64)
+    dts = None if fp64 else np.full(400, 0.15)
+    T_ends = [prob.T0 + k for k in range(1, 16)]
+    with fa.Context(prob, fp64=fp64) as ctx:
+        if dts is not None:
+            ctx.set_dt_sequence(0, dts)
+        ctx.set_collect(0, **opts)
+        for Te in T_ends:
+            ctx.step_until(Te)
+        m = ctx.measure(0, rows=4)
+        info = ctx.info(0)
+    op = to_oracle(prob, form=oracle.FORM_RHO1 if fp64 else oracle.FORM_LITERAL, dt_seq=dts)
+    op.collect = dict(opts, f32=not fp64)
+    q = prob.q0_f64.copy() if fp64 else prob.q0.astype(np.float64)
+    st = None
+    for Te in T_ends:
+        op.T_stop = Te
+        op.n0frame = 0 if st is None else st.n_accepted
+        q, st, rc = oracle.run(op, q, 10 ** 6, state=st)
+        assert rc == 0
+    mo = op.measure
+    assert info["n_accepted"] == st.n_accepted and (info["md_x"], info["md_y"]) == (st.mdx, st.mdy)
+    assert st.mdx > 0 and st.mdy > 0 and st.T > 8
+    nf = st.n_accepted - op.n0frame
+    L = opts["cut_half"]
+    tol = TOL64 if fp64 else 1e-5
+    if fp64:
+        np.testing.assert_allclose(m[0, :nf], mo[0, :nf], rtol=1e-13)
+    else:
+        np.testing.assert_array_equal(m[0, :nf], mo[0, :nf])
+    np.testing.assert_allclose(m[1, :nf], mo[1, :nf], rtol=tol)
+    amp = abs(prob.taylor["rhoa"])
+    assert np.count_nonzero(mo[2, :2 * L + 1]) == 2 * L + 1
+    e = [np.abs(m[r, :2 * L + 1] - mo[r, :2 * L + 1]).max() / amp for r in (2, 3)]
+    report(f"taylor collection {'fp64' if fp64 else 'fp32'} numeric analytic", e)
+    assert max(e) < tol
+    assert not m[2:, 2 * L + 1:].any()
+
+
+def test_taylor_study_end_to_end(tmp_path):
+    """The convergence study (run.sh + plots/error.py) on a small sweep:
+    three resolutions, two angles, both precisions, each precision one batch;
+    the error at T = 20 falls with the resolution in every run, and an
+    instance run alone writes the same cut as in the batch."""
+    from fivonagi_amd import taylor_study as ts
+    from fivonagi_amd.export import taylor_frames
+    res, dep = tmp_path / "res", tmp_path / "dep"
+    etas, cfls, ps, angles = [5, 10, 20], [90], [1, 2], ["00000", "PIo08"]
+    stats = ts.run_sweep(str(res), etas, cfls, ps, angles, final_time=20.0, log=lambda *a: None)
+    assert stats[1]["instances"] == 6 and stats[2]["instances"] == 6
+    err = ts.extract_errors(str(res), str(dep), etas, cfls, ps, angles, final_time=20)
+    report("taylor L1 error at eta 5, 10, 20 (p1, angle 0)", list(err[:, 0, 0, 0, 0]))
+    assert np.all(np.isfinite(err))
+    for pi in range(2):
+        for ai in range(2):
+            for ei in range(3):
+                e = err[:, 0, pi, ai, ei]
+                assert e[0] > e[1] > e[2], (pi, ai, ei, e)
+    # the batch does not change an instance's results
+    name = "d-" + ts.exec_name(10, 90, 1, "PIo08")
+    prob = wl.taylor(2, theta=math.pi / 8, moving=True, cflw=0.9)
+    with fa.Context(prob) as ctx:
+        taylor_frames(ctx, [str(tmp_path / "alone")], final_time=20.0)
+    last = sorted(f for f in os.listdir(res / name) if f.startswith("u1-cut-"))[-1]
+    assert (tmp_path / "alone" / last).read_text() == (res / name / last).read_text()
+    assert (tmp_path / "alone" / "u1-error.dat").read_text() == \
+        (res / name / "u1-error.dat").read_text()
 
 
 def test_moving_window_in_a_batch(oracle):

@@ -488,3 +488,91 @@ def test_run_stops_at_T_stop(oracle):
     assert st2.T < 0.3
     q3, st3, rc = oracle.run(to_oracle(p), q2, 1, state=st2)
     assert st3.T == st.T and np.array_equal(q3, q)
+
+
+# ------------------------------------------- taylor-angle-xy data collection --
+# apps/taylor-angle-xy/data_collect.h:7-98: rows 0/1 = T and cfl of the step,
+# row 2 = rho' interpolated in y on the line through the front's centre along
+# the propagation direction, row 3 = the closed form at the nearer row.
+
+def _taylor_collect_problem(oracle, theta, fp64=True):
+    p = wl.taylor(1, theta=theta)
+    op = to_oracle(p)
+    op.collect = dict(wl.taylor_collect(p, fp64=fp64), f32=not fp64)
+    op.measure = np.zeros((p.ny, p.nx))
+    return p, op
+
+
+def test_taylor_cut_on_the_exact_profile_at_zero_angle(oracle):
+    # theta = 0: ja = jcenter exactly, so the cut is row jcenter of the state,
+    # columns icenter - L .. icenter + L, and the state equal to the closed
+    # form makes both rows the same numbers
+    p, op = _taylor_collect_problem(oracle, 0.0)
+    t, T = p.taylor, 3.3
+    X = np.arange(p.nx) * p.dx
+    prof = t["rhoa"] * np.tanh(t["c"] * (X - T))
+    q1 = np.tile(prof, (p.ny, 1))
+    assert oracle.collect(op, q1, T, n_accepted=5, cfl=0.8) == 0
+    m = op.measure
+    L = op.collect["cut_half"]
+    ic = int(T / p.dx)                                  # 16
+    assert L == 25 and ic == 16
+    i = np.arange(max(ic - L, 0), ic + L + 1)
+    iw = i - (ic - L)
+    np.testing.assert_array_equal(m[2, iw], q1[p.ny // 2, i])
+    np.testing.assert_allclose(m[3, iw], prof[i], rtol=1e-14, atol=1e-22)
+    assert np.all(m[2, :iw[0]] == 0)                    # i < 0 has no thread
+    assert m[0, 4] == T and m[1, 4] == 0.8              # entry n - 1 - n0frame
+
+
+def test_taylor_cut_interpolates_on_the_line(oracle):
+    # a state linear in i and j is interpolated exactly: row 2 returns
+    # 1000 i + ja, which fixes the weights, the line and the offsets
+    theta = math.pi / 8
+    p, op = _taylor_collect_problem(oracle, theta)
+    t, T, mdx, mdy = p.taylor, 8.0, 3, 1
+    I, J = np.meshgrid(np.arange(p.nx), np.arange(p.ny))
+    q1 = 1000.0 * I + J
+    assert oracle.collect(op, q1, T, n_accepted=1, mdx=mdx, mdy=mdy) == 0
+    m = op.measure
+    L = op.collect["cut_half"]
+    ic = int(T * math.cos(theta) / p.dx) - mdx
+    jc = p.ny // 2 + int((T - t["mdt"]) * math.sin(theta) / p.dy) - mdy
+    i = np.arange(ic - L, ic + L + 1)
+    ja = jc + (i - ic) * math.tan(theta)
+    np.testing.assert_allclose(m[2, i - (ic - L)], 1000.0 * i + ja, rtol=0, atol=1e-4)
+    jn = np.floor(ja + 0.5)                             # j - ja in (-1/2, 1/2]
+    an = t["rhoa"] * np.tanh(t["c"] * ((i + mdx) * p.dx * math.cos(theta) +
+                                       ((jn + mdy - p.ny // 2) * p.dy + t["mdt"] *
+                                        math.sin(theta)) * math.sin(theta) - T))
+    np.testing.assert_allclose(m[3, i - (ic - L)], an, rtol=1e-12, atol=0)
+
+
+def test_taylor_cut_float_build_matches_double(oracle):
+    # PRECISION=1 arithmetic (float T, constants and closed form) picks the
+    # same cells and agrees to float rounding
+    theta = math.pi / 16
+    T = 7.25
+    p, o64 = _taylor_collect_problem(oracle, theta, fp64=True)
+    _, o32 = _taylor_collect_problem(oracle, theta, fp64=False)
+    t = p.taylor
+    X, Y = np.meshgrid(np.arange(p.nx) * p.dx, (np.arange(p.ny) - p.ny // 2) * p.dy)
+    q1 = t["rhoa"] * np.tanh(t["c"] * (X * math.cos(theta) + (Y + t["yoff"]) *
+                                       math.sin(theta) - T))
+    assert oracle.collect(o64, q1, T, 1) == 0 and oracle.collect(o32, q1, T, 1) == 0
+    a, b = o64.measure[2:4], o32.measure[2:4]
+    assert np.array_equal(a != 0, b != 0)
+    amp = abs(t["rhoa"])
+    assert np.abs(a - b).max() < 1e-5 * amp
+    # the interpolated profile and the closed form differ by the half-cell
+    # offset of the closed form's row and the interpolation error only
+    assert np.abs(a[0] - a[1]).max() < 0.2 * amp
+
+
+def test_taylor_collection_zeroes_before_T0_and_overflows(oracle):
+    p, op = _taylor_collect_problem(oracle, 0.0)
+    op.measure[:] = 1.0
+    q1 = np.zeros((p.ny, p.nx))
+    assert oracle.collect(op, q1, -0.5, 3) == 0 and not op.measure.any()
+    op.n0frame = 0
+    assert oracle.collect(op, q1, 1.0, p.nx + 1) == oracle.ERR_COLLECT
