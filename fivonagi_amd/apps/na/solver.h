@@ -29,12 +29,28 @@ namespace fvna {
 
 // The bodies below are templates on the real type R: float is the
 // production path (FP32 with the v_rcp/v_rsq/v_med3 forms), double the FP64
-// path (PRECISION=2 of the reference), where the same formulas run with IEEE
-// division and square root.
+// path (PRECISION=2 of the reference).  In double, 1/x and 1/sqrt(x) are the
+// hardware estimates refined by two Newton steps each (within an ulp of the
+// IEEE quotient the reference takes, at less than half the instructions of
+// the correctly rounded sequences).  Their arguments here are O(1) away from
+// zero (2 Z1, (L1 S1 - s3)(L2 S1 - s3), 1 + rho', the discriminant) except
+// the limiter's |W|^2, where 0 gives NaN as the IEEE form's 0 * inf does.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ double rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);          // e = 1 - x y
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double rsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double r = fma(-x * y, y, 1.0);      // r = 1 - x y^2
+  y = fma(0.5 * y, r, y);
+  r = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, r, y);
+}
 __device__ __forceinline__ float fma_(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double fma_(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ float min_(float a, float b) { return fminf(a, b); }
