@@ -475,7 +475,10 @@ static int init_impl(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   for (int k = 0; k < n_inst; ++k) {
     const auto& q = p[k];
     if (q.nx < 8 || q.ny < 8) return fail(FIVONAGI_ERR_ARG, "nx, ny must be >= 8");
-    if ((long)q.nx * q.ny > (1L << 30)) return fail(FIVONAGI_ERR_ARG, "grid too large");
+    // the kernels address a field with 32-bit byte offsets: at most 4 GiB
+    // per field (2^30 cells in FP32, 2^29 in FP64)
+    if ((long)q.nx * q.ny > (f64 ? (1L << 29) : (1L << 30)))
+      return fail(FIVONAGI_ERR_ARG, "grid too large (a field must fit 4 GiB)");
     if (!(q.dx > 0) || !(q.dy > 0) || !(q.dt0 > 0) || !(q.cfl_wish > 0))
       return fail(FIVONAGI_ERR_ARG, "dx, dy, dt0, cfl_wish must be > 0");
     if (q.bc < 0 || q.bc > 3) return fail(FIVONAGI_ERR_ARG, "unknown boundary kind");

@@ -93,7 +93,9 @@ typedef struct {
 } fivonagi_info_t;
 
 /* gpu_init (common/data_definitions.h:523-871) + init() upload
-   (apps/APP/init.h): n_inst instances, p[k] and q_host[3k..3k+2] (q1,q2,q3). */
+   (apps/APP/init.h): n_inst instances, p[k] and q_host[3k..3k+2] (q1,q2,q3).
+   Each instance: nx, ny >= 8 and one field within 4 GiB (nx*ny <= 2^30 cells;
+   2^29 for fivonagi_init_f64); larger grids return FIVONAGI_ERR_ARG. */
 int fivonagi_init(fivonagi_ctx **ctx, const fivonagi_params *p, int n_inst,
                   const float *const *q_host);
 /* The same with the reference's PRECISION=2 (common/main.cu:11-24): state,

@@ -72,3 +72,21 @@ def test_argument_errors_do_not_touch_the_gpu():
     assert lib.fivonagi_init(ctypes.byref(h), ctypes.byref(p), 1, arr) == fa.ERR_ARG
     assert lib.fivonagi_step(None, 1) == fa.ERR_ARG
     assert lib.fivonagi_sync(None) == fa.ERR_ARG
+
+
+@pytest.mark.parametrize("fp64,nx,ny,ok", [(False, 32768, 32768, True), (False, 32768, 32769, False),
+                                           (True, 32768, 16384, True), (True, 32768, 16385, False)])
+def test_grid_size_limit_follows_the_byte_offsets(fp64, nx, ny, ok):
+    # one field must fit 32-bit byte offsets: 2^30 cells in FP32, 2^29 in FP64.
+    # A grid at the limit fails later (on the null state pointers), never at
+    # the size check; the check itself touches no GPU
+    lib = fa.load()
+    h = ctypes.c_void_p()
+    p = fa.Params()
+    p.nx, p.ny, p.dx, p.dy, p.dt0, p.cfl_wish = nx, ny, 1.0, 1.0, 0.1, 0.9
+    T = ctypes.c_double if fp64 else ctypes.c_float
+    arr = (ctypes.POINTER(T) * 3)()
+    init = lib.fivonagi_init_f64 if fp64 else lib.fivonagi_init
+    assert init(ctypes.byref(h), ctypes.byref(p), 1, arr) == fa.ERR_ARG
+    msg = lib.fivonagi_last_error()
+    assert (b"too large" in msg) != ok, msg
