@@ -104,9 +104,22 @@ def cpu_baseline(prob, seconds=15.0):
     if rc != 0:
         raise RuntimeError(f"CPU oracle failed: {rc}")
     value = prob.nx * prob.ny * steps / el / 1e9
+    # the same on one core, over a shorter sample
+    op.nthreads = 1
+    t0 = time.perf_counter()
+    q, st, rc = po.run(op, q, 2, dtype=np.float32, state=st, fast=True)
+    per_step1 = max((time.perf_counter() - t0) / 2, 1e-4)
+    steps1 = max(2, int(seconds * 0.4 / per_step1))
+    t0 = time.perf_counter()
+    q, st, rc1 = po.run(op, q, steps1, dtype=np.float32, state=st, fast=True)
+    el1 = time.perf_counter() - t0
+    if rc1 != 0:
+        raise RuntimeError(f"CPU oracle failed: {rc1}")
     return {"value": value, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} accepted steps of the same {prob.nx}x{prob.ny} grid from the "
-                      f"quiescent start, FP32 CPU oracle (-O3, OpenMP over rows), {el:.1f} s"}
+                      f"quiescent start, FP32 CPU oracle (-O3, OpenMP over rows), {el:.1f} s; "
+                      f"1 core: {steps1} further steps, {el1:.1f} s",
+            "value_1core": prob.nx * prob.ny * steps1 / el1 / 1e9}
 
 
 def copy_probe(nbytes=256 << 20, reps=5):

@@ -697,9 +697,14 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     xn.v = xn.q3 * ir2;
   };
 
-  Row<R> B0{}, B1{}, B2{};
-  YI<R> F0{}, F1{}, F2{};
-  XY<R> X0{}, X1{}, X2{};
+  // every window is written before it is read, except the waves of the
+  // interface before the first one (their products with the first
+  // interface's waves are formed but never used): only those are zeroed, so
+  // no unset value is read and the other windows cost no initialisation
+  Row<R> B0, B1, B2;
+  YI<R> F0, F1, F2;
+  XY<R> X0, X1, X2;
+  F0.f.w = fvna::WavesT<R>{};
   for (int r = y0 - kHalo; r <= y1 + 2; r += 3) {
     body(r, B0, B2, B1, F0, F2, F1, X0, X2, X1);
     if (r + 1 > y1 + 2) break;
