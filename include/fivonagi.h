@@ -99,8 +99,8 @@ typedef struct {
 int fivonagi_init(fivonagi_ctx **ctx, const fivonagi_params *p, int n_inst,
                   const float *const *q_host);
 /* The same with the reference's PRECISION=2 (common/main.cu:11-24): state,
-   tables, arithmetic and the cfl reduction in FP64.  The adjoint and the data
-   collection are FP32 only. */
+   tables, arithmetic, the cfl reduction and the data collection in FP64
+   (read back with the _f64 calls).  The adjoint is FP32 only. */
 int fivonagi_init_f64(fivonagi_ctx **ctx, const fivonagi_params *p, int n_inst,
                       const double *const *q_host);
 

@@ -210,22 +210,25 @@ def hifu_collect_opts(p):
     return wl.hifu_collect(p)
 
 
-def test_data_collection_matches_oracle(oracle):
+@pytest.mark.parametrize("fp64", [False, True])
+def test_data_collection_matches_oracle(oracle, fp64):
     """dataCollect (apps/hifu-beam/data_collect.h:7-107) over frames of the
     hifu app with its moving window: T and cfl of every step, the
-    microphones and the running u1 maximum of the central peak."""
+    microphones and the running u1 maximum of the central peak (FP64: the
+    PRECISION=2 build, against the oracle's rho'-direct form)."""
     prob = wl.hifu(nx=128, ny=256, ispam=10.0)
     opts = hifu_collect_opts(prob)
     T_ends = [prob.T0 + k for k in range(1, 30)]
-    with fa.Context(prob) as ctx:
+    with fa.Context(prob, fp64=fp64) as ctx:
         ctx.set_collect(0, **opts)
         for Te in T_ends:
             ctx.step_until(Te)
         m = ctx.measure(0)
         info = ctx.info(0)
-    op = to_oracle(prob)
+    assert m.dtype == (np.float64 if fp64 else np.float32)
+    op = to_oracle(prob, form=oracle.FORM_RHO1 if fp64 else oracle.FORM_LITERAL)
     op.collect = opts
-    q, st = prob.q0.astype(np.float64), None
+    q, st = (prob.q0_f64.copy() if fp64 else prob.q0.astype(np.float64)), None
     for Te in T_ends:
         op.T_stop = Te
         op.n0frame = 0 if st is None else st.n_accepted
@@ -234,16 +237,18 @@ def test_data_collection_matches_oracle(oracle):
     mo = op.measure
     assert info["n_accepted"] == st.n_accepted and info["md_x"] == st.mdx > 0
     nf = st.n_accepted - 1 - op.n0frame + 1            # steps in the last frame
-    np.testing.assert_allclose(m[0, :nf], mo[0, :nf], rtol=1e-6)
-    np.testing.assert_allclose(m[1, :nf], mo[1, :nf], rtol=1e-5)
+    tol = TOL64 if fp64 else 1e-5
+    np.testing.assert_allclose(m[0, :nf], mo[0, :nf], rtol=1e-13 if fp64 else 1e-6)
+    np.testing.assert_allclose(m[1, :nf], mo[1, :nf], rtol=tol)
     # microphones and u1max sample the rho' field: relative to its peak on the
     # cut (the u1max maximum), as the field parity is relative to the field max
     den = np.abs(mo[6:]).max()
     assert den > 0 and np.abs(mo[2]).max() > 0     # the first microphone heard the beam
     e_mic = [np.abs(m[2 + k] - mo[2 + k]).max() / den for k in range(4)]
     e_max = np.abs(m[6:] - mo[6:]).max() / den
-    report("data collection mic1 mic2 u1max", [e_mic[0], e_mic[1], e_max])
-    assert max(e_mic) < 1e-5 and e_max < 1e-5
+    report(f"data collection {'fp64' if fp64 else 'fp32'} mic1 mic2 u1max",
+           [e_mic[0], e_mic[1], e_max])
+    assert max(e_mic) < tol and e_max < tol
 
 
 def test_data_collection_frame_overflow(oracle):
