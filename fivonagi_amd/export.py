@@ -224,16 +224,23 @@ class TaylorExport:
 
 
 def taylor_frames(ctx, deploy_paths, final_time=100.0, dt_print=1.0, grids=False,
-                  collect=None, on_frame=None):
+                  collect=None, stats=None):
     """calcFrame's frame loop (common/main.cu:248-756) for every instance of a
     Context of taylor-angle-xy problems: frames of dt_print from T0 (all
     instances share the frame times; each keeps its own dt), data collection
     on, each frame exported into that instance's deploy path, until T >
     final_time (main.cu:741).  collect: per instance set_collect options
     (default: workloads.taylor_collect in the context's precision).
-    on_frame(frame): called after each frame's steps (timing hooks).
-    Returns the TaylorExport objects."""
+    stats: a dict that receives the seconds spent stepping ("step_s", the
+    step_until calls) and in all ("total_s").  Returns the TaylorExport
+    objects."""
+    import time
     from .workloads import taylor_collect
+    if not final_time > 0:
+        # finalTime = 0 runs forever in the reference (main.cu:741)
+        raise ValueError("final_time must be > 0")
+    t_start = time.perf_counter()
+    step_s = 0.0
     probs = ctx.problems
     n = len(probs)
     exps = []
@@ -248,11 +255,10 @@ def taylor_frames(ctx, deploy_paths, final_time=100.0, dt_print=1.0, grids=False
     done = [False] * n
     frames = [0] * n
     Tprint = min(p.T0 for p in probs) + dt_print
-    step = 0
     while not all(done):
+        t0 = time.perf_counter()
         ctx.step_until(Tprint)
-        if on_frame is not None:
-            on_frame(step)
+        step_s += time.perf_counter() - t0
         for k in range(n):
             if done[k] or Tprint < probs[k].T0 + dt_print * (1 - 1e-9):
                 continue
@@ -261,8 +267,10 @@ def taylor_frames(ctx, deploy_paths, final_time=100.0, dt_print=1.0, grids=False
             exps[k](frames[k], ctx.measure(k, rows=4), info["T"], info["n_accepted"],
                     info["md_x"], info["md_y"], u1=u1)
             frames[k] += 1
-            if final_time > 0 and info["T"] > final_time:
+            if info["T"] > final_time:
                 done[k] = True
         Tprint += dt_print
-        step += 1
+    if stats is not None:
+        stats["step_s"] = step_s
+        stats["total_s"] = time.perf_counter() - t_start
     return exps

@@ -72,27 +72,14 @@ def run_sweep(results_path, etas=ETAS, cfls=CFLS, precisions=PRECISIONS, angles=
         if not probs:
             continue
         t_wall = time.perf_counter()
-        gpu = [0.0]
-        last = [time.perf_counter()]
-
-        def on_frame(_):
-            now = time.perf_counter()
-            gpu[0] += now - last[0]
-            last[0] = now
-
+        st = {}
         with fa.Context(probs, fp64=(p == 2)) as ctx:
-            orig = ctx.step_until
-
-            def timed_step_until(T):
-                last[0] = time.perf_counter()
-                orig(T)
-            ctx.step_until = timed_step_until
-            taylor_frames(ctx, paths, final_time=final_time, grids=grids, on_frame=on_frame)
+            taylor_frames(ctx, paths, final_time=final_time, grids=grids, stats=st)
             cells = sum(pr.nx * pr.ny * ctx.info(k)["n_attempts"] for k, pr in enumerate(probs))
-        stats[p] = dict(instances=len(probs), gpu_s=gpu[0], cell_updates=cells,
+        stats[p] = dict(instances=len(probs), gpu_s=st["step_s"], cell_updates=cells,
                         wall_s=time.perf_counter() - t_wall)
         log("p%d: %d runs, %.3g cell updates, %.2f s stepping, %.2f s in all"
-            % (p, len(probs), cells, gpu[0], stats[p]["wall_s"]))
+            % (p, len(probs), cells, st["step_s"], stats[p]["wall_s"]))
     return stats
 
 
