@@ -250,6 +250,7 @@ def bench_taylor_study(args, hip):
     import shutil
     import tempfile
     from fivonagi_amd import taylor_study as ts
+    from fivonagi_amd.study import rates
     root = args.study_dir or tempfile.mkdtemp(prefix="taylor_study_")
     res, dep = os.path.join(root, "results"), os.path.join(root, "deploy")
     os.makedirs(res, exist_ok=True)
@@ -262,7 +263,7 @@ def bench_taylor_study(args, hip):
     rates = {}
     for pi, p in enumerate(ts.PRECISIONS):
         for ei, et in enumerate(ts.ERROR_TYPES):
-            rr = [ts._rates(err, ts.ETAS, ci, pi, ai, ei)[1]
+            rr = [rates(err, ts.ETAS, ci, pi, ai, ei)[1]
                   for ai in range(len(ts.ANGLES)) for ci in range(len(ts.CFLS))]
             rates[f"p{p}_{et}_rate_reg_mean"] = round(float(np.mean(rr)), 4)
     cells = sum(v["cell_updates"] for v in stats.values())

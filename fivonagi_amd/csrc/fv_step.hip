@@ -831,8 +831,7 @@ __device__ void collect_taylor(const InstDesc& D, const Ctl& P, double Td,
 // was accepted, record its new T and its cfl (row 0 and row 1, entry
 // n - 1 - n0frame) and the app's measurement (collect_hifu, collect_taylor)
 // into the instance's measurement grid (data_collect.h of either app).
-// blockIdx.y = instance; threads cover the columns (and, while T <= 0, the
-// whole grid, which the reference zeroes then).
+// blockIdx.y = instance; threads cover the columns.
 template <typename R>
 __global__ __launch_bounds__(256) void fv_collect_kernel(const InstDesc* __restrict__ insts,
                                                          int phase) {
@@ -845,13 +844,13 @@ __global__ __launch_bounds__(256) void fv_collect_kernel(const InstDesc* __restr
   if (F.state != kFoldAccept) return;
   const Ctl& P = F.p;
   const double T = P.T + P.dt;                       // update_T
-  const int nx = D.nx, ny = D.ny;
+  const int nx = D.nx;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (T <= 0.0) {                                    // data_collect.h:22-30 (both apps)
-    for (int64_t k = tid; k < (int64_t)nx * ny; k += (int64_t)gridDim.x * blockDim.x)
-      m[k] = R(0);
-    return;
-  }
+  // While T <= 0 the reference zeroes the whole grid every step and records
+  // nothing (data_collect.h:22-30, both apps).  Here the grid is zeroed when
+  // collection is switched on, and nothing writes it while T <= 0, so it is
+  // already zero: the pass over the grid is skipped.
+  if (T <= 0.0) return;
   const int64_t nf = P.n_acc - D.n0frame;          // n - 1 - n0frame, n = n_acc + 1
   if (nf >= nx) {                                   // simError 4
     if (tid == 0) aadd_relaxed(D.collect_err, 4u);

@@ -52,6 +52,19 @@ def _export_cfl(ex, frame, flat, nx, steps):
                 ex.first = 0
 
 
+def write_u1max(path, flat, nx, mdx, ispf, ampl):
+    """u1max.dat of the hifu app (apps/hifu-beam/data_export.h:95-116): the
+    running maximum of rho' on the cut at the unmoved grid's columns
+    0 .. nx + MDX - 1 (flat index i + 6 nx of the measurement grid), x in
+    focal lengths (i / ISPF, in float)."""
+    ispf = np.float32(ispf)
+    with open(path, "w") as f:
+        f.write("# AMPL=" + _g(ampl) + "\n")
+        f.write("# x u1max(x,y_cut)\n")
+        f.write("".join(_g(np.float32(i) / ispf) + " " + _g(flat[i + 6 * nx]) + "\n"
+                        for i in range(nx + mdx)))
+
+
 class HifuExport:
     """dataExport of apps/hifu-beam/data_export.h:14-179 for one instance of a
     Context with data collection on (``Context.set_collect``).
@@ -76,13 +89,24 @@ class HifuExport:
         return os.path.join(self.path, name)
 
     def __call__(self, frame, u1, measure, T, n, mdx, mdy):
-        """Write frame `frame` (0, 1, ...): u1 (ny, nx) state, measure (ny, nx)
-        measurement grid (float32, or float64 from an FP64 run), T after the
-        frame, n accepted steps so far, mdx/mdy the moving-window offset."""
-        ny, nx = u1.shape
+        """Write frame `frame` (0, 1, ...): u1 (ny, nx) state, measure the
+        measurement grid (its first rows suffice: up to the u1max entries,
+        7 + MDX/nx of them; float32, or float64 from an FP64 run), T after the
+        frame, n accepted steps so far, mdx/mdy the moving-window offset.
+        u1 = None leaves out the two files made from the state (u1-NNNN.dat,
+        u1-cut-NNNN.dat)."""
+        nx = measure.shape[-1]
         flat = np.ascontiguousarray(measure).reshape(-1)
         Tstr = _g(flat.dtype.type(T), 5)
         fstr = "%04d" % frame
+        if u1 is not None:
+            self._state_files(fstr, Tstr, u1, mdx, mdy)
+        # u1max on the cut (data_export.h:95-116)
+        write_u1max(self._file("u1max.dat"), flat, nx, mdx, self.ispf, self.ampl)
+        self._collected_files(frame, flat, nx, Tstr, n)
+
+    def _state_files(self, fstr, Tstr, u1, mdx, mdy):
+        ny, nx = u1.shape
         # u1 for the whole domain (data_export.h:35-75)
         with open(self._file("u1-" + fstr + ".dat"), "w") as f:
             f.write("# T=" + Tstr + "\n")
@@ -101,13 +125,8 @@ class HifuExport:
             f.write("# x u1(x,y_cut)\n")
             for i in range(nx):
                 f.write(_g(xs[i]) + " " + _g(u1[self.jcut, i]) + " \n")
-        # u1max on the cut (data_export.h:95-116)
-        with open(self._file("u1max.dat"), "w") as f:
-            f.write("# AMPL=" + _g(self.ampl) + "\n")
-            f.write("# x u1max(x,y_cut)\n")
-            for i in range(nx + mdx):
-                x = np.float32(i) / self.ispf
-                f.write(_g(x) + " " + _g(flat[i + 6 * nx]) + "\n")
+
+    def _collected_files(self, frame, flat, nx, Tstr, n):
         # microphones (data_export.h:118-141)
         steps = n - self.previous_n
         for m in range(1, 5):
@@ -223,6 +242,41 @@ class TaylorExport:
         self.previous_n = n
 
 
+def frame_loop(ctx, final_time, dt_print, on_frame):
+    """calcFrame's frame loop (common/main.cu:248-756, :263, :652, :741) over
+    every instance of a context: frames end at T0 + dt_print, T0 + 2 dt_print,
+    ... (fivonagi_step_until, which starts a frame for the collection), each
+    instance after its own T0 (with different T0 the instances share the
+    frame ends where they overlap, an instance joining once its first one
+    comes), until its T > final_time.  on_frame(k, frame, info) runs after
+    instance k's frame `frame` (info = Context.info(k)).  Returns the seconds
+    spent stepping."""
+    import time
+    if not final_time > 0:
+        # finalTime = 0 runs forever in the reference (main.cu:741)
+        raise ValueError("final_time must be > 0")
+    probs = ctx.problems
+    n = len(probs)
+    done = [False] * n
+    frames = [0] * n
+    Tprint = min(p.T0 for p in probs) + dt_print
+    step_s = 0.0
+    while not all(done):
+        t0 = time.perf_counter()
+        ctx.step_until(Tprint)
+        step_s += time.perf_counter() - t0
+        for k in range(n):
+            if done[k] or Tprint < probs[k].T0 + dt_print * (1 - 1e-9):
+                continue
+            info = ctx.info(k)
+            on_frame(k, frames[k], info)
+            frames[k] += 1
+            if info["T"] > final_time:
+                done[k] = True
+        Tprint += dt_print
+    return step_s
+
+
 def taylor_frames(ctx, deploy_paths, final_time=100.0, dt_print=1.0, grids=False,
                   collect=None, stats=None):
     """calcFrame's frame loop (common/main.cu:248-756) for every instance of a
@@ -236,40 +290,21 @@ def taylor_frames(ctx, deploy_paths, final_time=100.0, dt_print=1.0, grids=False
     objects."""
     import time
     from .workloads import taylor_collect
-    if not final_time > 0:
-        # finalTime = 0 runs forever in the reference (main.cu:741)
-        raise ValueError("final_time must be > 0")
     t_start = time.perf_counter()
-    step_s = 0.0
     probs = ctx.problems
-    n = len(probs)
     exps = []
     for k, p in enumerate(probs):
         opts = taylor_collect(p, ctx.fp64) if collect is None else collect[k]
         ctx.set_collect(k, **opts)
         exps.append(TaylorExport(deploy_paths[k], p.nx, p.ny, opts["eta"], p.taylor["theta"],
                                  opts["cut_half"], p.cfl_wish, fp64=ctx.fp64, grids=grids))
-    # every instance's frames end at T0 + dt_print, T0 + 2 dt_print, ...
-    # (main.cu:263, :652); with different T0 the instances share the frame
-    # ends where they overlap, and an instance joins once its first one comes
-    done = [False] * n
-    frames = [0] * n
-    Tprint = min(p.T0 for p in probs) + dt_print
-    while not all(done):
-        t0 = time.perf_counter()
-        ctx.step_until(Tprint)
-        step_s += time.perf_counter() - t0
-        for k in range(n):
-            if done[k] or Tprint < probs[k].T0 + dt_print * (1 - 1e-9):
-                continue
-            info = ctx.info(k)
-            u1 = ctx.collect(k)[0][0] if grids else None
-            exps[k](frames[k], ctx.measure(k, rows=4), info["T"], info["n_accepted"],
-                    info["md_x"], info["md_y"], u1=u1)
-            frames[k] += 1
-            if info["T"] > final_time:
-                done[k] = True
-        Tprint += dt_print
+
+    def on_frame(k, frame, info):
+        u1 = ctx.collect(k)[0][0] if grids else None
+        exps[k](frame, ctx.measure(k, rows=4), info["T"], info["n_accepted"],
+                info["md_x"], info["md_y"], u1=u1)
+
+    step_s = frame_loop(ctx, final_time, dt_print, on_frame)
     if stats is not None:
         stats["step_s"] = step_s
         stats["total_s"] = time.perf_counter() - t_start

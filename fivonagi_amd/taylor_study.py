@@ -28,6 +28,9 @@ import time
 
 import numpy as np
 
+from .study import ERROR_TYPES, Layout
+from .study import write_tables as _write_tables
+
 # run.sh:4-8 and plots/error_table_*.py:9-14
 ANGLES = {"00000": 0.0, "PIo32": math.pi / 32, "PIo16": math.pi / 16, "PIo08": math.pi / 8,
           "PIo04": math.pi / 4}
@@ -37,7 +40,6 @@ ETA_MULT = {5: 1, 10: 2, 20: 4, 41: 8, 82: 16}           # run.sh:28-43: eta -> 
 ETAS = [5, 10, 20, 41, 82]
 CFLS = [60, 70, 80, 90, 99]
 PRECISIONS = [1, 2]
-ERROR_TYPES = ["L1", "L2", "Li"]
 
 
 def exec_name(eta, cfl, p, angle):
@@ -145,100 +147,22 @@ def extract_errors(results_path, deploy_path, etas=ETAS, cfls=CFLS, precisions=P
     return error
 
 
-def _rates(error, etas, ci, pi, ai, ei):
-    """rate_last (the last halving) and the least-squares slope of log error
-    against -log eta (plots/error_table_1.py:62-74)."""
-    from scipy import stats
-    e = error[:, ci, pi, ai, ei]
-    rate_last = (1 / np.log(2)) * np.log(e[-2] / e[-1])
-    rate_reg, intercept, R, _, _ = stats.linregress(-np.log(etas), np.log(e))
-    return rate_last, rate_reg, intercept, R ** 2
+LAYOUT = Layout("angle", list(ANGLES), [ANGLE_TEX[a] for a in ANGLES],
+                "cfl{0}-p{1}-angle{2}", "conv_rates_stats_angles-p{0}.dat", "\\theta_T=",
+                "{:.2e}")
 
 
 def write_tables(deploy_path, error=None, etas=ETAS, cfls=CFLS, precisions=PRECISIONS,
                  angles=tuple(ANGLES)):
-    """plots/error_table_1.py .. error_table_4.py from error.npy: table 1 the
-    rates over all angles and CFLs per precision and error type, table 2
-    every run's rates, table 3 per angle with the finest errors, table 4 the
-    LaTeX rows of table 3.  A table whose directory exists is skipped."""
+    """plots/error_table_1.py .. error_table_4.py of the Taylor app from
+    error.npy (study.write_tables with the angle as the grouping column)."""
     if error is None:
         error = np.fromfile(os.path.join(deploy_path, "dr_error_etas_CPU", "error.npy"))
         error = error.reshape((len(etas), len(cfls), len(precisions), len(angles),
                                len(ERROR_TYPES)))
-
-    def table_dir(k):
-        d = os.path.join(deploy_path, "table_error_%d" % k)
-        if os.path.isdir(d):
-            return None
-        os.mkdir(d)
-        return d
-
-    def stats_over(pi, ei, angle_list):
-        s = dict(last=[], reg=[], R2=[], fin=[])
-        for ai in angle_list:
-            for ci in range(len(cfls)):
-                rl, rr, _, r2 = _rates(error, etas, ci, pi, ai, ei)
-                s["last"].append(rl)
-                s["reg"].append(rr)
-                s["R2"].append(r2)
-                s["fin"].append(error[-1, ci, pi, ai, ei])
-        return s
-
-    d = table_dir(1)                                     # error_table_1.py
-    if d:
-        for pi, p in enumerate(precisions):
-            with open(os.path.join(d, "conv_rates_stats%d.dat" % p), "w") as f:
-                f.write("# error_type rate_last_best rate_last_worst rate_last_mean "
-                        "rate_reg_best rate_reg_worst rate_reg_mean R2_reg_worst\n")
-                for ei, et in enumerate(ERROR_TYPES):
-                    s = stats_over(pi, ei, range(len(angles)))
-                    f.write("%s %f %f %f %f %f %f %f\n" % (
-                        et, max(max(s["last"]), 0), min(s["last"]), np.mean(s["last"]),
-                        max(max(s["reg"]), 0), min(s["reg"]), np.mean(s["reg"]), min(s["R2"])))
-    d = table_dir(2)                                     # error_table_2.py
-    if d:
-        with open(os.path.join(d, "conv_rates_all.dat"), "w") as f:
-            f.write("# error_type exec_name rate_last rate_reg intercept_reg R2_reg\n")
-            for pi, p in enumerate(precisions):
-                for ei, et in enumerate(ERROR_TYPES):
-                    for ai, a in enumerate(angles):
-                        for ci, cfl in enumerate(cfls):
-                            rl, rr, ic, r2 = _rates(error, etas, ci, pi, ai, ei)
-                            f.write("%s %s %f %f %f %f\n" % (
-                                et, "cfl{0}-p{1}-angle{2}".format(cfl, p, a), rl, rr, ic, r2))
-    d = table_dir(3)                                     # error_table_3.py
-    if d:
-        for pi, p in enumerate(precisions):
-            with open(os.path.join(d, "conv_rates_stats_angles-p%d.dat" % p), "w") as f:
-                f.write("# error_type angle rate_last_best rate_last_worst rate_last_mean "
-                        "rate_reg_best rate_reg_worst rate_reg_mean R2_reg_worst finest_best "
-                        "finest_worst\n")
-                for ai, a in enumerate(angles):
-                    f.write("# ------------------------ \n")
-                    for ei, et in enumerate(ERROR_TYPES):
-                        s = stats_over(pi, ei, [ai])
-                        f.write("%s %s %f %f %f %f %f %f %f %f %f \n" % (
-                            et, a, max(max(s["last"]), 0), min(s["last"]), np.mean(s["last"]),
-                            max(max(s["reg"]), 0), min(s["reg"]), np.mean(s["reg"]),
-                            min(s["R2"]), min(s["fin"]), max(max(s["fin"]), 0)))
-    d = table_dir(4)                                     # error_table_4.py
-    if d:
-        for pi, p in enumerate(precisions):
-            with open(os.path.join(d, "conv_rates_stats_angles-p%d.dat" % p), "w") as f:
-                f.write("% rate_reg_best & rate_reg_worst & rate_reg_mean & R2_reg_worst & "
-                        "finest_best & finest_worst \\\\\n")
-                for ai, a in enumerate(angles):
-                    f.write("\\midrule\n")
-                    f.write("\\multirow{2}{*}{$\\theta_T=" + ANGLE_TEX[a] + "$} &\n")
-                    for ei, et in enumerate(ERROR_TYPES):
-                        if et == "L2":
-                            continue
-                        f.write("$E_1$ &\n" if et == "L1" else "& $E_\\infty$ &\n")
-                        s = stats_over(pi, ei, [ai])
-                        f.write("{:.4f} & {:.4f} & {:.4f} & {:.4f} & {:.2e} & {:.2e} \\\\\n".format(
-                            max(max(s["reg"]), 0), min(s["reg"]), np.mean(s["reg"]),
-                            min(s["R2"]), min(s["fin"]), max(max(s["fin"]), 0)))
-    return error
+    layout = Layout("angle", list(angles), [ANGLE_TEX[a] for a in angles], LAYOUT.exec_fmt,
+                    LAYOUT.file34, LAYOUT.tex_prefix, LAYOUT.finest_fmt)
+    return _write_tables(deploy_path, error, etas, cfls, precisions, layout)
 
 
 def main(argv=None):

@@ -214,3 +214,67 @@ def test_error_extraction_reads_the_last_cut(tmp_path):
     e = np.loadtxt(tmp_path / "dep" / "dr_error_etas_CPU" / ("d-" + ts.exec_name(5, 90, 1, "00000"))
                    / "u1-error.dat")
     assert e[0] == 1 and e[1] == pytest.approx(10.004)
+
+
+# --------------------------------------------------------------- hifu study --
+# apps/hifu-beam/run.sh (<exec_name>.dat = the last u1max.dat),
+# plots/error_collect.py, plots/error_table_{1,2,3,4}.py
+
+def test_hifu_error_collect_against_a_reference_curve(tmp_path):
+    # the reference curve u/AMPL = 1 + x is linear, so its interpolation is
+    # exact: a run 2 eps above it has relative L1, L2, Linf errors of 2 eps
+    # over 0.1 <= x <= 1.8; points past the curve's end are left out
+    from fivonagi_amd import hifu_study as hs
+    from fivonagi_amd.export import write_u1max
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    xr = np.linspace(0.0, 1.9, 951)
+    np.savetxt(ref / "data-2-a10-d.dat", np.stack([xr, 1.0 + xr], 1))
+    nx, mdx, ispf = 64, 40, 50.0
+    eps = {20: 1e-3, 41: 2.5e-4}
+    for eta in (20, 41):
+        name = hs.exec_name(eta, 90, 1, 10)
+        d = tmp_path / "res" / ("d-" + name)
+        d.mkdir(parents=True)
+        flat = np.zeros(8 * nx + mdx, np.float32)
+        x = np.arange(nx + mdx) / np.float32(ispf)
+        flat[6 * nx:7 * nx + mdx] = hs.AMPL * (1.0 + x) * (1.0 + 2 * eps[eta])
+        write_u1max(str(d / (name + ".dat")), flat, nx, mdx, ispf, hs.AMPL)
+    err = hs.error_collect(str(tmp_path / "res"), str(tmp_path / "dep"), str(ref),
+                           etas=[20, 41], cfls=[90], precisions=[1], apertures=[10])
+    for ei, eta in enumerate((20, 41)):
+        np.testing.assert_allclose(err[ei, 0, 0, 0], 2 * eps[eta], rtol=2e-3)
+    row = np.loadtxt(tmp_path / "dep" / "dr_error_collect" / "u1-error-eta41-cfl90-p1-a10.dat")
+    np.testing.assert_allclose(row, err[1, 0, 0, 0], atol=1e-6)
+    raw = np.fromfile(tmp_path / "dep" / "dr_error_collect" / "error.npy")
+    np.testing.assert_array_equal(raw, err.reshape(-1))
+
+
+def test_hifu_curve_truncation_follows_error_collect():
+    # error_collect.py:69-77: the abscissae up to the curve's last x are kept,
+    # then one more is dropped; the error window is 0.1 <= x <= 1.8
+    from fivonagi_amd import hifu_study as hs
+    xr = np.array([0.0, 1.0, 2.0])
+    ur = np.array([1.0, 1.0, 1.0])
+    num_x = np.array([0.5, 1.0, 1.5, 1.7, 1.9, 2.5])
+    num_u = np.array([1.0, 1.0, 1.0, 2.0, 5.0, 9.0])
+    # kept: 0.5, 1.0, 1.5, 1.7 (1.9 is dropped with the first point past 2.0)
+    e = hs._errors_against(xr, ur, num_x, num_u)
+    assert e[0] == pytest.approx(1.0 / 4) and e[2] == pytest.approx(1.0)
+    assert e[1] == pytest.approx((1.0 / 4) ** 0.5)
+
+
+@pytest.mark.filterwarnings("ignore:Input line 1 contained no data")
+def test_hifu_tables_layout(tmp_path):
+    from fivonagi_amd import hifu_study as hs
+    etas = np.array(hs.ETAS, float)
+    err = np.zeros((3, 5, 2, 3, 3))
+    err[:] = (0.5 * etas ** -1.5)[:, None, None, None, None]
+    hs.write_tables(str(tmp_path), err)
+    t2 = np.loadtxt(tmp_path / "table_error_2" / "conv_rates_all.dat", dtype=str)
+    assert t2[0, 1] == "cfl60-p1-a10" and t2.shape == (2 * 3 * 3 * 5, 6)
+    np.testing.assert_allclose(t2[:, 3].astype(float), 1.5, atol=1e-6)
+    t3 = np.loadtxt(tmp_path / "table_error_3" / "conv_rates_stats_as-p1.dat", dtype=str)
+    assert list(t3[::3, 1]) == ["10", "20", "30"]
+    t4 = (tmp_path / "table_error_4" / "conv_rates_stats_as-p2.dat").read_text()
+    assert "$a=20$" in t4 and "& %.4f \\\\" % (0.5 * 82 ** -1.5) in t4

@@ -420,6 +420,46 @@ def test_taylor_study_end_to_end(tmp_path):
         (res / name / "u1-error.dat").read_text()
 
 
+def test_hifu_study_end_to_end(tmp_path):
+    """The hifu study (run.sh + plots/error_collect.py) on a small sweep: one
+    resolution, two apertures, both precisions to T = 20, each precision one
+    batch.  Every run leaves <exec_name>.dat (its last u1max.dat, nx + MDX
+    lines); an instance run alone writes the same file; the comparison with a
+    reference curve runs on them."""
+    from fivonagi_amd import hifu_study as hs
+    from fivonagi_amd.export import frame_loop, write_u1max
+    res, dep = tmp_path / "res", tmp_path / "dep"
+    stats = hs.run_sweep(str(res), [20], [90], [1, 2], [30, 10], final_time=20.0,
+                         log=lambda *a: None)
+    assert stats[1]["instances"] == 2 and stats[2]["instances"] == 2
+    for p in (1, 2):
+        for a in (10, 30):
+            name = hs.exec_name(20, 90, p, a)
+            u = np.loadtxt(res / ("d-" + name) / (name + ".dat"))
+            assert u.shape[0] > 512 and u[:, 1].max() > 0.5 * hs.AMPL
+    # the batch does not change an instance's results
+    prob = wl.hifu(1, ispam=10.0, cflw=0.9)
+    with fa.Context(prob) as ctx:
+        ctx.set_collect(0, **wl.hifu_collect(prob))
+        frame_loop(ctx, 20.0, 1.0, lambda k, f, info: None)
+        info = ctx.info(0)
+        m = ctx.measure(0, rows=8 + info["md_x"] // prob.nx)
+    write_u1max(str(tmp_path / "alone.dat"), m.reshape(-1), prob.nx, info["md_x"],
+                prob.hifu["f"], prob.hifu["ampl"])
+    name = hs.exec_name(20, 90, 1, 10)
+    assert (tmp_path / "alone.dat").read_text() == (res / ("d-" + name) / (name + ".dat")).read_text()
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    for a in (10, 30):
+        x = np.linspace(0.0, 2.0, 1001)
+        np.savetxt(ref / ("data-2-a%d-d.dat" % a), np.stack([x, 1.0 + 0 * x], 1))
+    err = hs.error_collect(str(res), str(dep), str(ref), [20], [90], [1, 2], [10, 30])
+    assert err.shape == (1, 1, 2, 2, 3) and np.all(np.isfinite(err))
+    # FP32 and FP64 runs agree on the measured curve far better than either
+    # agrees with a flat line
+    assert np.abs(err[0, 0, 0] - err[0, 0, 1]).max() < 1e-3 * np.abs(err).max()
+
+
 def test_moving_window_in_a_batch(oracle):
     """A batch where one instance moves and one does not (the multi-instance
     form of the moving-window kernel)."""
