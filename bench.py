@@ -238,6 +238,20 @@ def bench_hifu(args, hip):
     }
 
 
+def taylor_rate_summary(err):
+    """Mean regression rate per precision and error type over the Taylor
+    study's 25 CFL x angle runs (table 1's rate_reg_mean)."""
+    from fivonagi_amd import taylor_study as ts
+    from fivonagi_amd.study import rates as run_rates
+    out = {}
+    for pi, p in enumerate(ts.PRECISIONS):
+        for ei, et in enumerate(ts.ERROR_TYPES):
+            rr = [run_rates(err, ts.ETAS, ci, pi, ai, ei)[1]
+                  for ai in range(len(ts.ANGLES)) for ci in range(len(ts.CFLS))]
+            out[f"p{p}_{et}_rate_reg_mean"] = round(float(np.mean(rr)), 4)
+    return out
+
+
 def bench_taylor_study(args, hip):
     """The taylor-angle-xy convergence study as the reference runs it
     (apps/taylor-angle-xy/run.sh): 5 resolutions (NX = 64..1024) x 5 CFLs x 5
@@ -250,7 +264,6 @@ def bench_taylor_study(args, hip):
     import shutil
     import tempfile
     from fivonagi_amd import taylor_study as ts
-    from fivonagi_amd.study import rates
     root = args.study_dir or tempfile.mkdtemp(prefix="taylor_study_")
     res, dep = os.path.join(root, "results"), os.path.join(root, "deploy")
     os.makedirs(res, exist_ok=True)
@@ -260,12 +273,7 @@ def bench_taylor_study(args, hip):
     err = ts.extract_errors(res, dep)
     ts.write_tables(dep, err)
     wall = time.perf_counter() - t0
-    rates = {}
-    for pi, p in enumerate(ts.PRECISIONS):
-        for ei, et in enumerate(ts.ERROR_TYPES):
-            rr = [rates(err, ts.ETAS, ci, pi, ai, ei)[1]
-                  for ai in range(len(ts.ANGLES)) for ci in range(len(ts.CFLS))]
-            rates[f"p{p}_{et}_rate_reg_mean"] = round(float(np.mean(rr)), 4)
+    rates = taylor_rate_summary(err)
     cells = sum(v["cell_updates"] for v in stats.values())
     gpu_s = sum(v["gpu_s"] for v in stats.values())
     if not args.study_dir:

@@ -278,3 +278,14 @@ def test_hifu_tables_layout(tmp_path):
     assert list(t3[::3, 1]) == ["10", "20", "30"]
     t4 = (tmp_path / "table_error_4" / "conv_rates_stats_as-p2.dat").read_text()
     assert "$a=20$" in t4 and "& %.4f \\\\" % (0.5 * 82 ** -1.5) in t4
+
+
+def test_bench_taylor_rate_summary():
+    # bench.py's summary of the Taylor study: table 1's mean regression rate
+    import bench
+    etas = np.array([5, 10, 20, 41, 82], float)
+    err = np.zeros((5, 5, 2, 5, 3))
+    err[:, :, 0] = (etas ** -1.5)[:, None, None, None]
+    err[:, :, 1] = (etas ** -2.0)[:, None, None, None]
+    s = bench.taylor_rate_summary(err)
+    assert s["p1_L1_rate_reg_mean"] == pytest.approx(1.5) and s["p2_Li_rate_reg_mean"] == 2.0
