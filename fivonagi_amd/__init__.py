@@ -23,6 +23,7 @@ EXPORTS = (
     "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_set_skew",
     "fivonagi_set_collect", "fivonagi_measure", "fivonagi_measure_f64", "fivonagi_measure_rows",
     "fivonagi_measure_rows_f64", "fivonagi_cut_errors", "fivonagi_cut_errors_f64",
+    "fivonagi_cfl_subset",
     "fivonagi_destroy", "fivonagi_last_error",
 )
 
@@ -119,6 +120,7 @@ def load():
     lib.fivonagi_measure_rows.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
     lib.fivonagi_measure_rows_f64.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     lib.fivonagi_cut_errors.argtypes = [fp, fp, C.c_int, fp]
+    lib.fivonagi_cfl_subset.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_uint8)]
     lib.fivonagi_cut_errors_f64.argtypes = [dp, dp, C.c_int, fp]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
@@ -131,7 +133,8 @@ def load():
                  "fivonagi_info", "fivonagi_step_adjoint", "fivonagi_adjoint_state",
                  "fivonagi_autotune", "fivonagi_set_skew", "fivonagi_set_collect",
                  "fivonagi_measure", "fivonagi_measure_f64", "fivonagi_measure_rows",
-                 "fivonagi_measure_rows_f64", "fivonagi_cut_errors", "fivonagi_cut_errors_f64"):
+                 "fivonagi_measure_rows_f64", "fivonagi_cut_errors", "fivonagi_cut_errors_f64",
+                 "fivonagi_cfl_subset"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -386,3 +389,11 @@ def cut_errors(numeric, analytic):
         _check(load().fivonagi_cut_errors(numeric.ctypes.data_as(fp),
                                           analytic.ctypes.data_as(fp), len(numeric), ef))
     return e
+
+
+def cfl_subset(nx, ny):
+    """(ny, nx) uint8 mask of the cells the reference's CFL fold carries into
+    cfl[0] (fivonagi_cfl_subset)."""
+    m = np.zeros((ny, nx), np.uint8)
+    _check(load().fivonagi_cfl_subset(int(nx), int(ny), m.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return m

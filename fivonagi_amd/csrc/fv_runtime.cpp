@@ -68,15 +68,16 @@ std::vector<uint8_t> fold_subset(int nx, int ny) {
   long sizered = N;
   while (sizered > 32) { sizered /= 2; strides.push_back((sizered / 16) * 16); }
   while (sizered > 2) { sizered /= 2; strides.push_back(sizered); }
+  // Walk the folds backwards from cfl[0].  A fold of stride S sets slot
+  // i < S to max(slot i, slot i + S) and leaves the slots >= S as they are,
+  // so a later fold can read such a slot again (when its stride is more than
+  // S/2, as where the two loops meet).  Slots past the grid hold no cell.
   std::vector<uint8_t> m(N, 0);
-  for (long c = 0; c < N; ++c) {
-    long p = c;
-    bool alive = true;
-    for (long S : strides) {
-      if (p >= S && p < 2 * S) p -= S;
-      else if (p >= 2 * S) { alive = false; break; }
-    }
-    m[c] = (alive && p == 0) ? 1 : 0;
+  m[0] = 1;
+  for (auto it = strides.rbegin(); it != strides.rend(); ++it) {
+    const long S = *it;
+    for (long i = 0; i < S && i + S < N; ++i)
+      if (m[i]) m[i + S] = 1;
   }
   return m;
 }
@@ -1019,6 +1020,14 @@ int fivonagi_measure_rows(fivonagi_ctx* c, int inst, int rows, float* out) {
 
 int fivonagi_measure_rows_f64(fivonagi_ctx* c, int inst, int rows, double* out) {
   return measure_impl<double>(c, inst, rows, out);
+}
+
+int fivonagi_cfl_subset(int nx, int ny, uint8_t* mask) {
+  if (nx < 1 || ny < 1 || !mask || (long)nx * ny > (1L << 30))
+    return fail(FIVONAGI_ERR_ARG, "cfl subset: bad grid or null mask");
+  const std::vector<uint8_t> m = fold_subset(nx, ny);
+  std::memcpy(mask, m.data(), m.size());
+  return 0;
 }
 
 int fivonagi_cut_errors(const float* numeric, const float* analytic, int length,

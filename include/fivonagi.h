@@ -221,6 +221,12 @@ int fivonagi_measure_f64(fivonagi_ctx *ctx, int inst, double *measure);
 int fivonagi_measure_rows(fivonagi_ctx *ctx, int inst, int rows, float *measure);
 int fivonagi_measure_rows_f64(fivonagi_ctx *ctx, int inst, int rows, double *measure);
 
+/* Host-side: the cells whose cfl reaches cfl[0] in the reference's fold
+   (reduceCFL and its schedule, common/main.cu:179-212, :503-524), which
+   FIVONAGI_CFL_REFERENCE_FOLD reduces over: mask[i + j*nx] = 1 for those
+   cells (nx*ny bytes). */
+int fivonagi_cfl_subset(int nx, int ny, uint8_t *mask);
+
 /* Host-side: the relative L1, L2 and Linf errors of a Taylor cut
    (apps/taylor-angle-xy/data_export.h:96-122: sum|a-n|/sum|a|,
    sqrt(sum (a-n)^2/sum a^2), max|a-n|/max|a|) with the reference's float

@@ -66,6 +66,22 @@ def test_prescribed_dt_pulse(oracle, nx, ny):
     assert abs(st["T"] - sto.T) < 1e-12
 
 
+@pytest.mark.parametrize("nx,ny", [(8, 8), (9, 13), (59, 8), (8, 200), (117, 11)])
+def test_smallest_and_ragged_grids(oracle, nx, ny):
+    """The smallest grid the ABI takes (8x8: a 4x4 interior in one strip,
+    most lanes out of the grid) and ragged shapes: a strip with one column
+    (59 = 58 + 1), one row chunk, a tall narrow grid, odd sizes; CFL feedback,
+    so the fold subsets of these shapes decide dt as in the oracle."""
+    prob = wl.gauss_pulse(nx, ny, sigma_cells=2.0)
+    q, st = run_gpu(prob, 30)
+    qo, sto = run_oracle(oracle, prob, 30)
+    e = rel_linf(q, qo)
+    report(f"pulse {nx}x{ny} feedback 30 steps", e)
+    assert max(e) < TOL
+    assert st["n_accepted"] == sto.n_accepted and st["n_rejected"] == sto.n_rejected
+    assert st["T"] == pytest.approx(sto.T, rel=1e-6)
+
+
 @pytest.mark.parametrize("mode", [fa.CFL_REFERENCE_FOLD, fa.CFL_ALL])
 def test_feedback_pulse_400(oracle, mode):
     prob = wl.gauss_pulse(400, 400, cfl_mode=mode)

@@ -322,6 +322,41 @@ def test_fold_subset_400(oracle):
     assert 0.40 < frac < 0.42 and m[0, 0] == 1
 
 
+def _fold_brute_force(nx, ny):
+    """common/main.cu:503-524 run on sets: slot c starts as {c}; every
+    reduceCFL launch of stride S (:179-197) merges slot i + S into slot i for
+    i < S (slots past the grid hold no cell); cfl[0] ends as the subset."""
+    N = nx * ny
+    slots = [{c} for c in range(N)]
+
+    def fold(S):
+        for i in range(S):
+            if i + S < N:
+                slots[i] = slots[i] | slots[i + S]
+    size = N
+    while size > 32:
+        size //= 2
+        fold((size // 16) * 16)
+    while size > 2:
+        size //= 2
+        fold(size)
+    m = np.zeros(N, np.uint8)
+    m[sorted(slots[0])] = 1
+    return m.reshape(ny, nx)
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 8), (117, 11), (59, 8), (9, 13), (8, 200), (33, 17),
+                                   (100, 72), (130, 40), (64, 64)])
+def test_fold_subset_against_brute_force(oracle, nx, ny):
+    # the oracle's backward walk and the library's host table both equal a
+    # direct simulation of the reference's folds (the library's first table
+    # followed each cell on one path and lost the slots a later fold re-reads)
+    import fivonagi_amd as fa
+    ref = _fold_brute_force(nx, ny)
+    np.testing.assert_array_equal(oracle.cfl_subset(nx, ny), ref)
+    np.testing.assert_array_equal(fa.cfl_subset(nx, ny), ref)
+
+
 # ------------------------------------------------------------- whole step --
 
 def run(oracle, prob, n, form=None, dtype=np.float64):
