@@ -613,9 +613,10 @@ def test_imaginary_eigenvalue_reports_2(oracle):
     assert orc == oracle.ORC_ERR_EIGEN if hasattr(oracle, "ORC_ERR_EIGEN") else orc == 2
 
 
-def test_autotune_leaves_state_and_results_unchanged():
+@pytest.mark.parametrize("fp64", [False, True])
+def test_autotune_leaves_state_and_results_unchanged(fp64):
     probs = [small_pulse(64, 64), wl.edge_drive(200, 96, halfwidth_cells=20.0)]
-    with fa.Context(probs) as ctx:
+    with fa.Context(probs, fp64=fp64) as ctx:
         ctx.step(5)
         before = [ctx.collect(k) for k in range(2)]
         h = ctx.autotune(4)
@@ -625,11 +626,41 @@ def test_autotune_leaves_state_and_results_unchanged():
         ctx.step(20)
         tuned = [ctx.collect(k)[0] for k in range(2)]
     assert h > 0
-    with fa.Context(probs) as ctx:
+    with fa.Context(probs, fp64=fp64) as ctx:
         ctx.step(25)
         plain = [ctx.collect(k)[0] for k in range(2)]
     for a, b in zip(tuned, plain):
         assert np.array_equal(a, b)
+
+
+def test_collection_switched_off_and_on_again():
+    """set_collect(off) stops the recording (the grid stays as it was);
+    switching it on again starts from a zero grid, with the frame counted
+    from the step_until that follows, like the first time."""
+    prob = wl.hifu(nx=128, ny=256, ispam=10.0)
+    opts = hifu_collect_opts(prob)
+    with fa.Context(prob) as ctx:
+        ctx.set_collect(0, **opts)
+        T = prob.T0
+        for _ in range(25):
+            T += 1.0
+            ctx.step_until(T)
+        m1 = ctx.measure(0)
+        assert np.abs(m1[6:]).max() > 0
+        ctx.set_collect(0, on=False)
+        T += 1.0
+        ctx.step_until(T)
+        with pytest.raises(fa.FivonagiError):
+            ctx.measure(0)                      # off: nothing to read
+        ctx.set_collect(0, **opts)
+        m0 = ctx.measure(0)
+        assert not m0.any()                     # on again: a zero grid
+        T += 1.0
+        ctx.step_until(T)
+        m2 = ctx.measure(0)
+        n_frame = int(np.count_nonzero(m2[0]))
+        assert n_frame > 0 and np.all(np.diff(m2[0, :n_frame]) > 0)
+        assert m2[0, 0] > T - 1.0 and m2[0, n_frame - 1] >= T
 
 
 def test_call_pattern_does_not_change_results():
