@@ -71,8 +71,9 @@ class Hip:
 
 
 def traffic_from_profiles(workload_tag):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    recorded under profiles/ (see DESIGN.md); None if not measured."""
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE,
+    each scaled by the counter's reading for a copy of known size), recorded
+    under profiles/; None if not measured."""
     path = os.path.join(ROOT, "profiles", f"{KERNEL}_{workload_tag}_traffic.json")
     if not os.path.exists(path):
         return None
@@ -399,7 +400,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step "
-              "shards by instances or by y-slabs (DESIGN.md) but no multi-GPU path is built.",
+              "would shard by instances or by y-slabs, but no multi-GPU path is built.",
               file=sys.stderr)
         sys.exit(2)
 
