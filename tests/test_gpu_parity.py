@@ -685,3 +685,49 @@ def test_call_pattern_does_not_change_results():
         assert np.array_equal(q, ref_q), pattern
         assert s == ref_s, (pattern, s, ref_s)
     assert ref_s["n_rejected"] > 0 and ref_s["n_accepted"] == 67
+
+
+def test_long_run_edge_drive_with_shocks(oracle):
+    """512x256 edge drive over 3000 accepted steps: the drive steepens into a
+    sawtooth with a shock per wavelength, and the dt feedback (next dt =
+    cfl_wish dx / max wave speed) integrates the round-off of the speed at the
+    shocks into T.  Held to the FP32 oracle's own distance from the FP64 oracle
+    (measured: q1 1.2e-4 against 5.6e-5, T +4.4e-8 relative).  With the FP64
+    oracle's dt sequence prescribed, the feedback is taken out and the GPU's
+    fields are closer to the FP64 oracle than the FP32 oracle's (measured: q1
+    6.6e-6, q2 1.2e-5, q3 1.3e-4 against 5.6e-5, 5.1e-5, 2.7e-4)."""
+    prob = wl.edge_drive(512, 256)
+    n = 3000
+    op = to_oracle(prob, form=oracle.FORM_RHO1, nthreads=min(os.cpu_count() or 1, 16))
+    op.collect = dict(jcut=2, mic=(0, 0, 0, 0), eta=20.0)
+    q = prob.q0_f64.copy()
+    st = None
+    T_seq = []
+    for done in range(0, n, 500):                 # one data-collection frame per 500 steps
+        op.n0frame = done
+        q, st, rc = oracle.run(op, q, 500, state=st, fast=True)
+        assert rc == 0
+        T_seq.append(op.measure[0, :500].copy())
+    qo, T_o = q, np.concatenate(T_seq)
+    op32 = to_oracle(prob, form=oracle.FORM_RHO1, nthreads=min(os.cpu_count() or 1, 16))
+    q32, st32, rc = oracle.run(op32, prob.q0_f64.copy(), n, dtype=np.float32, fast=True)
+    assert rc == 0
+    e32 = rel_linf(q32, qo)
+
+    qg, sg = run_gpu(prob, n)
+    e = rel_linf(qg, qo)
+    report("long-run edge drive 512x256, 3000 steps (oracle FP32: "
+           + " ".join(f"{v:.2e}" for v in e32) + ")", e)
+    assert sg["n_accepted"] == st.n_accepted and sg["n_rejected"] == st.n_rejected
+    drift = (sg["T"] - st.T) / st.T
+    print(f"[parity] long-run T drift {drift:+.2e} (oracle FP32 {(st32.T - st.T) / st.T:+.2e})",
+          flush=True)
+    assert abs(drift) < 2e-7
+    assert all(a <= 5 * b for a, b in zip(e, e32)), (e, e32)
+
+    dts = np.diff(np.concatenate([[prob.T0], T_o]))
+    qp, sp = run_gpu(prob, n, np.concatenate([dts, np.full(8, dts[-1])]))
+    ep = rel_linf(qp, qo)
+    report("long-run edge drive, FP64 oracle's dt prescribed", ep)
+    assert sp["T"] == pytest.approx(st.T, rel=1e-12)
+    assert all(a <= b for a, b in zip(ep, e32)), (ep, e32)
