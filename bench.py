@@ -440,8 +440,8 @@ def main():
     # dominant (only) kernel: one launch per attempt
     bpc = BYTES_PER_CELL * (2 if args.fp64 else 1)
     achieved = bpc * cells / (ms_per_step / 1e3) / 1e9
-    tag = f"edge_drive_{prob.nx}x{prob.ny}"
-    traffic = None if args.fp64 else traffic_from_profiles(tag)
+    tag = f"edge_drive_{prob.nx}x{prob.ny}" + ("_fp64" if args.fp64 else "")
+    traffic = traffic_from_profiles(tag)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
