@@ -289,3 +289,20 @@ def test_bench_taylor_rate_summary():
     err[:, :, 1] = (etas ** -2.0)[:, None, None, None]
     s = bench.taylor_rate_summary(err)
     assert s["p1_L1_rate_reg_mean"] == pytest.approx(1.5) and s["p2_Li_rate_reg_mean"] == 2.0
+
+
+def test_bench_traffic_from_profiles():
+    # bench.py's roofline.traffic: the rocprofv3 FETCH_SIZE/WRITE_SIZE figures
+    # recorded for the 4096x1024 step, FP32 and FP64, scaled by the copy probe
+    import json
+    import bench
+    for tag, alg in (("edge_drive_4096x1024", 24), ("edge_drive_4096x1024_fp64", 48)):
+        t = bench.traffic_from_profiles(tag)
+        path = os.path.join(bench.ROOT, "profiles", f"{bench.KERNEL}_{tag}_traffic.json")
+        d = json.load(open(path))
+        assert d["algorithmic_bytes_per_launch"] == alg * 4096 * 1024
+        kib = d["fetch_size_kib_per_dispatch"] * d["fetch_scale"] + \
+            d["write_size_kib_per_dispatch"] * d["write_scale"]
+        assert t == pytest.approx(kib * 1024, rel=1e-9)
+        assert 1.0 < t / d["algorithmic_bytes_per_launch"] < 1.5
+    assert bench.traffic_from_profiles("edge_drive_1x1") is None
