@@ -397,3 +397,4 @@ def cfl_subset(nx, ny):
     m = np.zeros((ny, nx), np.uint8)
     _check(load().fivonagi_cfl_subset(int(nx), int(ny), m.ctypes.data_as(C.POINTER(C.c_uint8))))
     return m
+

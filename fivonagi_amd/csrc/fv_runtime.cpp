@@ -103,7 +103,11 @@ struct fivonagi_ctx {
   long cus = 256;                          // CUs of the device
   int per_cu = 16;                         // resident step waves per CU
   std::vector<std::vector<uint32_t>> fold_bits;  // kCflBitmap: bit i + j*nx = cell counted
-  std::vector<std::vector<double>> dt_seq;
+  // the caller's edge_amp / edge_phase arrays, copied at init (params point here)
+  std::vector<std::vector<double>> edge_amp, edge_phase;
+  // prescribed dt sequence per instance: device buffer and its capacity
+  std::vector<double*> dt_seq_dev;
+  std::vector<int64_t> dt_seq_cap;
   std::vector<void*> allocs;               // every device allocation
   fv::InstDesc* d_desc = nullptr;
   fv::WorkItem* d_items = nullptr;
@@ -142,6 +146,7 @@ struct fivonagi_ctx {
   ~fivonagi_ctx() {
     drop_graph();
     for (float* q : adj_lam) (void)hipFree(q);
+    for (double* d : dt_seq_dev) (void)hipFree(d);
     if (stream) (void)hipStreamSynchronize(stream);
     if (d_items) (void)hipFree(d_items);
     if (d_rowmask) (void)hipFree(d_rowmask);
@@ -462,6 +467,135 @@ int write_targets(fivonagi_ctx* c, const std::vector<int64_t>& tgt,
   return write_ctls(c, ctl);
 }
 
+struct HostInst {
+  fv::InstDesc d;                    // scalar fields; device pointers null
+  std::vector<double> ec, es, tab;   // edge drive (or aperture) and HIFU row tables
+  std::vector<uint32_t> fold_bits;   // kCflBitmap: bit i + j*nx = cell counted
+  bool ext = false;                  // needs the extended step kernel
+  int taylor_ring = 0;               // ghost ring of a Taylor instance (0: none)
+};
+
+// Parameters every entry point that sets up an instance accepts.
+int check_params(const fivonagi_params& q, bool f64) {
+  if (q.nx < 8 || q.ny < 8) return fail(FIVONAGI_ERR_ARG, "nx, ny must be >= 8");
+  // the kernels address a field with 32-bit byte offsets: at most 4 GiB
+  // per field (2^30 cells in FP32, 2^29 in FP64)
+  if ((long)q.nx * q.ny > (f64 ? (1L << 29) : (1L << 30)))
+    return fail(FIVONAGI_ERR_ARG, "grid too large (a field must fit 4 GiB)");
+  if (!(q.dx > 0) || !(q.dy > 0) || !(q.dt0 > 0) || !(q.cfl_wish > 0))
+    return fail(FIVONAGI_ERR_ARG, "dx, dy, dt0, cfl_wish must be > 0");
+  if (q.bc < 0 || q.bc > 3) return fail(FIVONAGI_ERR_ARG, "unknown boundary kind");
+  if (q.bc == FIVONAGI_BC_HIFU &&
+      !(q.hifu_f > 0 && q.hifu_frec > 0 && q.hifu_a >= 0 && std::isfinite(q.hifu_ampl)))
+    return fail(FIVONAGI_ERR_ARG, "hifu: hifu_f, hifu_frec must be > 0, hifu_a >= 0");
+  if (q.cfl_mode < 0 || q.cfl_mode > 1) return fail(FIVONAGI_ERR_ARG, "unknown cfl mode");
+  if (q.md_on != 0 && q.md_on != 1) return fail(FIVONAGI_ERR_ARG, "md_on must be 0 or 1");
+  if (q.md_on && !(q.md_vx >= 0 && q.md_vy >= 0 && std::isfinite(q.md_vx) &&
+                   std::isfinite(q.md_vy) && std::isfinite(q.md_t)))
+    return fail(FIVONAGI_ERR_ARG, "moving window: md_vx, md_vy must be finite and >= 0");
+  return 0;
+}
+
+// The host side of one instance: every scalar field of its descriptor (the
+// device pointers are left null) and the tables its device copies are made
+// from -- edge drive or HIFU aperture, CFL fold subset.  Allocates no device
+// memory.
+void host_inst(const fivonagi_params& q, bool f64, HostInst& h) {
+  fv::InstDesc& d = h.d;
+  std::memset(&d, 0, sizeof(d));
+  const size_t n = (size_t)q.nx * q.ny;
+  d.fp64 = f64 ? 1 : 0;
+  d.hbeta_d = q.beta - 1.0;
+  d.hifu_k_d = q.hifu_frec / 4.0;
+  // edge drive amplitude/phase folded into amp*cos(phase), amp*sin(phase);
+  // the HIFU aperture's AMPL and phase delay ISPW*phi_j the same way
+  h.ec.assign(q.ny, 0.0);
+  h.es.assign(q.ny, 0.0);
+  h.tab.clear();
+  const bool hifu = q.bc == FIVONAGI_BC_HIFU;
+  const double ispw = 2.0 * 3.14159265358979323846 * q.hifu_frec;   // ISPW
+  if (hifu) h.tab.assign(4 * (size_t)q.ny, 0.0);
+  for (int j = 0; j < q.ny; ++j) {
+    if (hifu) {
+      // apps/hifu-beam/boundary.h:79-90
+      const int width = (int)(q.hifu_a / q.dy);
+      const int jp = j - 2;
+      if (jp >= 0 && jp < width) {
+        const double F = q.hifu_f, jp2 = (double)jp * jp;
+        const double nd = std::sqrt(jp2 + F * F);
+        const double phi = (q.dx * jp2) / (2.0 * F);
+        h.ec[j] = q.hifu_ampl * std::cos(ispw * phi);
+        h.es[j] = q.hifu_ampl * std::sin(ispw * phi);
+        h.tab[4 * (size_t)j + 0] = phi;
+        h.tab[4 * (size_t)j + 1] = F / nd;
+        h.tab[4 * (size_t)j + 2] = -jp / nd;
+        h.tab[4 * (size_t)j + 3] = 1.0;
+      }
+      continue;
+    }
+    const double a = q.edge_amp ? q.edge_amp[j] : 0.0;
+    const double ph = q.edge_phase ? q.edge_phase[j] : 0.0;
+    h.ec[j] = a * std::cos(ph);
+    h.es[j] = a * std::sin(ph);
+  }
+  d.hifu_k = (float)(q.hifu_frec / 4.0);
+  // CFL subset
+  d.cfl_mask = fv::kCflAll;
+  h.fold_bits.clear();
+  if (q.cfl_mode == FIVONAGI_CFL_REFERENCE_FOLD) {
+    std::vector<uint8_t> m = fold_subset(q.nx, q.ny);
+    bool even = true;
+    for (size_t o = 0; o < n && even; ++o) even = (m[o] != 0) == ((o & 1) == 0);
+    if (even) {
+      d.cfl_mask = fv::kCflEven;
+    } else {
+      h.fold_bits.assign((n + 31) / 32, 0u);
+      for (size_t o = 0; o < n; ++o)
+        if (m[o]) h.fold_bits[o >> 5] |= 1u << (o & 31);
+      d.cfl_mask = fv::kCflBitmap;
+    }
+  }
+  d.dt_seq = nullptr;
+  d.dt_seq_len = 0;
+  d.dx = q.dx; d.dy = q.dy; d.delta = q.delta;
+  d.cfl_max = q.cfl_max; d.cfl_wish = q.cfl_wish; d.omega = q.omega;
+  d.inv_dx = 1.0 / q.dx; d.inv_dy = 1.0 / q.dy;
+  d.dif_x = q.delta / (q.dx * q.dx); d.dif_y = q.delta / (q.dy * q.dy);
+  d.omega_turns = (hifu ? ispw : q.omega) / 6.283185307179586476925286766559;
+  d.tay_rhoa = q.tay_rhoa; d.tay_c = q.tay_c;
+  d.tay_cos = std::cos(q.tay_theta); d.tay_sin = std::sin(q.tay_theta);
+  d.tay_yoff = q.tay_yoff; d.tay_jc = q.tay_jc;
+  d.nx = q.nx; d.ny = q.ny; d.bc = q.bc;
+  h.taylor_ring = q.bc == FIVONAGI_BC_TAYLOR ? 4 * q.nx + 4 * (q.ny - 4) : 0;
+  d.beta2m2 = (float)(2.0 * (q.beta - 1.0));
+  d.md_on = q.md_on;
+  d.md_vx = q.md_vx; d.md_vy = q.md_vy; d.md_t = q.md_t;
+  h.ext = hifu || q.md_on;
+}
+
+// A table of doubles as the bytes of the context's precision.
+void table_bytes(const std::vector<double>& v, bool f64, char* out) {
+  if (f64) {
+    std::memcpy(out, v.data(), sizeof(double) * v.size());
+  } else {
+    for (size_t k = 0; k < v.size(); ++k) {
+      const float f = (float)v[k];
+      std::memcpy(out + sizeof(float) * k, &f, sizeof(float));
+    }
+  }
+}
+
+// Control block of an instance's first attempt: dt0 at T0, no counts, no
+// stop conditions yet.
+void initial_ctl(const fivonagi_params& q, const fv::InstDesc& d, fv::Ctl& t) {
+  std::memset(&t, 0, sizeof(t));
+  t.dt = q.dt0;
+  t.T = q.T0;
+  t.target_acc = 0;
+  t.T_stop = std::numeric_limits<double>::infinity();
+  set_ctl_derived(t, d);
+}
+
 }  // namespace
 
 extern "C" {
@@ -474,25 +608,9 @@ static int init_impl(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   if (!out || !p || !q_host || n_inst <= 0) return fail(FIVONAGI_ERR_ARG, "null argument");
   *out = nullptr;
   for (int k = 0; k < n_inst; ++k) {
-    const auto& q = p[k];
-    if (q.nx < 8 || q.ny < 8) return fail(FIVONAGI_ERR_ARG, "nx, ny must be >= 8");
-    // the kernels address a field with 32-bit byte offsets: at most 4 GiB
-    // per field (2^30 cells in FP32, 2^29 in FP64)
-    if ((long)q.nx * q.ny > (f64 ? (1L << 29) : (1L << 30)))
-      return fail(FIVONAGI_ERR_ARG, "grid too large (a field must fit 4 GiB)");
-    if (!(q.dx > 0) || !(q.dy > 0) || !(q.dt0 > 0) || !(q.cfl_wish > 0))
-      return fail(FIVONAGI_ERR_ARG, "dx, dy, dt0, cfl_wish must be > 0");
-    if (q.bc < 0 || q.bc > 3) return fail(FIVONAGI_ERR_ARG, "unknown boundary kind");
-    if (q.bc == FIVONAGI_BC_HIFU &&
-        !(q.hifu_f > 0 && q.hifu_frec > 0 && q.hifu_a >= 0 && std::isfinite(q.hifu_ampl)))
-      return fail(FIVONAGI_ERR_ARG, "hifu: hifu_f, hifu_frec must be > 0, hifu_a >= 0");
-    if (q.cfl_mode < 0 || q.cfl_mode > 1) return fail(FIVONAGI_ERR_ARG, "unknown cfl mode");
+    if (int rc = check_params(p[k], f64)) return rc;
     if (!q_host[3 * k] || !q_host[3 * k + 1] || !q_host[3 * k + 2])
       return fail(FIVONAGI_ERR_ARG, "null state pointer");
-    if (q.md_on != 0 && q.md_on != 1) return fail(FIVONAGI_ERR_ARG, "md_on must be 0 or 1");
-    if (q.md_on && !(q.md_vx >= 0 && q.md_vy >= 0 && std::isfinite(q.md_vx) &&
-                     std::isfinite(q.md_vy) && std::isfinite(q.md_t)))
-      return fail(FIVONAGI_ERR_ARG, "moving window: md_vx, md_vy must be finite and >= 0");
   }
   auto* c = new fivonagi_ctx();
   c->n_inst = n_inst;
@@ -500,8 +618,25 @@ static int init_impl(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   const size_t es = f64 ? sizeof(double) : sizeof(float);
   c->params.assign(p, p + n_inst);
   c->desc.resize(n_inst);
-  c->dt_seq.resize(n_inst);
+  c->dt_seq_dev.assign(n_inst, nullptr);
+  c->dt_seq_cap.assign(n_inst, 0);
   c->fold_bits.resize(n_inst);
+  // the edge arrays are copied: the caller may free or reuse its own after
+  // init, and the adjoint reads these copies (the drive the forward runs)
+  c->edge_amp.resize(n_inst);
+  c->edge_phase.resize(n_inst);
+  for (int k = 0; k < n_inst; ++k) {
+    auto& q = c->params[k];
+    if (q.edge_amp) {
+      c->edge_amp[k].assign(q.edge_amp, q.edge_amp + q.ny);
+      q.edge_amp = c->edge_amp[k].data();
+    }
+    if (q.edge_phase) {
+      c->edge_phase[k].assign(q.edge_phase, q.edge_phase + q.ny);
+      q.edge_phase = c->edge_phase[k].data();
+    }
+  }
+  p = c->params.data();
   hipError_t e;
   e = hipGetDevice(&c->device);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipGetDevice"); }
@@ -518,8 +653,10 @@ static int init_impl(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
   std::vector<fv::Ctl> ctl(2 * (size_t)n_inst);
   for (int k = 0; k < n_inst; ++k) {
     const auto& q = p[k];
+    HostInst h;
+    host_inst(q, f64, h);
     auto& d = c->desc[k];
-    std::memset(&d, 0, sizeof(d));
+    d = h.d;
     const size_t n = (size_t)q.nx * q.ny;
     char* buf = (char*)c->dalloc(es * n * 6, &e);
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipMalloc state"); }
@@ -531,104 +668,32 @@ static int init_impl(fivonagi_ctx** out, const fivonagi_params* p, int n_inst,
       e = hipMemcpy(d.q[1][f], q_host[3 * k + f], es * n, hipMemcpyHostToDevice);
       if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy state"); }
     }
-    d.fp64 = f64 ? 1 : 0;
-    d.hbeta_d = q.beta - 1.0;
-    d.hifu_k_d = q.hifu_frec / 4.0;
-    // edge drive amplitude/phase folded into amp*cos(phase), amp*sin(phase);
-    // the HIFU aperture's AMPL and phase delay ISPW*phi_j the same way
-    // (double here, uploaded in the context's precision)
-    std::vector<double> ec(q.ny, 0.0), es_(q.ny, 0.0), tab;
-    const bool hifu = q.bc == FIVONAGI_BC_HIFU;
-    const double ispw = 2.0 * 3.14159265358979323846 * q.hifu_frec;   // ISPW
-    if (hifu) tab.assign(4 * (size_t)q.ny, 0.0);
-    for (int j = 0; j < q.ny; ++j) {
-      if (hifu) {
-        // apps/hifu-beam/boundary.h:79-90
-        const int width = (int)(q.hifu_a / q.dy);
-        const int jp = j - 2;
-        if (jp >= 0 && jp < width) {
-          const double F = q.hifu_f, jp2 = (double)jp * jp;
-          const double nd = std::sqrt(jp2 + F * F);
-          const double phi = (q.dx * jp2) / (2.0 * F);
-          ec[j] = q.hifu_ampl * std::cos(ispw * phi);
-          es_[j] = q.hifu_ampl * std::sin(ispw * phi);
-          tab[4 * (size_t)j + 0] = phi;
-          tab[4 * (size_t)j + 1] = F / nd;
-          tab[4 * (size_t)j + 2] = -jp / nd;
-          tab[4 * (size_t)j + 3] = 1.0;
-        }
-        continue;
-      }
-      const double a = q.edge_amp ? q.edge_amp[j] : 0.0;
-      const double ph = q.edge_phase ? q.edge_phase[j] : 0.0;
-      ec[j] = a * std::cos(ph);
-      es_[j] = a * std::sin(ph);
-    }
     // a table in the context's precision on the device
     auto upload = [&](const std::vector<double>& v, const void** dst) -> hipError_t {
       hipError_t r;
       void* t = c->dalloc(es * v.size(), &r);
       if (r != hipSuccess) return r;
       *dst = t;
-      if (f64) return hipMemcpy(t, v.data(), es * v.size(), hipMemcpyHostToDevice);
-      std::vector<float> vf(v.begin(), v.end());
-      return hipMemcpy(t, vf.data(), es * v.size(), hipMemcpyHostToDevice);
+      std::vector<char> bytes(es * v.size());
+      table_bytes(v, f64, bytes.data());
+      return hipMemcpy(t, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
     };
-    d.hifu_tab = nullptr;
-    d.hifu_k = (float)(q.hifu_frec / 4.0);
-    if (hifu) {
-      e = upload(tab, &d.hifu_tab);
+    if (!h.tab.empty()) {
+      e = upload(h.tab, &d.hifu_tab);
       if (e != hipSuccess) { delete c; return hip_fail(e, "hifu table"); }
-      c->ext_any = true;
     }
-    e = upload(ec, &d.edge_c);
-    if (e == hipSuccess) e = upload(es_, &d.edge_s);
+    e = upload(h.ec, &d.edge_c);
+    if (e == hipSuccess) e = upload(h.es, &d.edge_s);
     if (e != hipSuccess) { delete c; return hip_fail(e, "edge tables"); }
-    // CFL subset
-    d.cfl_mask = fv::kCflAll;
-    if (q.cfl_mode == FIVONAGI_CFL_REFERENCE_FOLD) {
-      std::vector<uint8_t> m = fold_subset(q.nx, q.ny);
-      bool even = true;
-      for (size_t o = 0; o < n && even; ++o) even = (m[o] != 0) == ((o & 1) == 0);
-      if (even) {
-        d.cfl_mask = fv::kCflEven;
-      } else {
-        std::vector<uint32_t>& bits = c->fold_bits[k];
-        bits.assign((n + 31) / 32, 0u);
-        for (size_t o = 0; o < n; ++o)
-          if (m[o]) bits[o >> 5] |= 1u << (o & 31);
-        d.cfl_mask = fv::kCflBitmap;
-      }
-    }
+    c->fold_bits[k] = std::move(h.fold_bits);
+    if (h.ext) c->ext_any = true;
+    if (q.md_on) c->md_any = true;
+    c->taylor_ring = std::max(c->taylor_ring, h.taylor_ring);
     d.ctl = c->d_ctl + 2 * k;
     d.sync = c->d_sync + 3 * k;
-    d.dt_seq = nullptr;
-    d.dt_seq_len = 0;
-    d.dx = q.dx; d.dy = q.dy; d.delta = q.delta;
-    d.cfl_max = q.cfl_max; d.cfl_wish = q.cfl_wish; d.omega = q.omega;
-    d.inv_dx = 1.0 / q.dx; d.inv_dy = 1.0 / q.dy;
-    d.dif_x = q.delta / (q.dx * q.dx); d.dif_y = q.delta / (q.dy * q.dy);
-    d.omega_turns = (q.bc == FIVONAGI_BC_HIFU ? ispw : q.omega) / 6.283185307179586476925286766559;
-    d.tay_rhoa = q.tay_rhoa; d.tay_c = q.tay_c;
-    d.tay_cos = std::cos(q.tay_theta); d.tay_sin = std::sin(q.tay_theta);
-    d.tay_yoff = q.tay_yoff; d.tay_jc = q.tay_jc;
-    d.nx = q.nx; d.ny = q.ny; d.bc = q.bc;
-    if (q.bc == FIVONAGI_BC_TAYLOR)
-      c->taylor_ring = std::max(c->taylor_ring, 4 * q.nx + 4 * (q.ny - 4));
-    d.beta2m2 = (float)(2.0 * (q.beta - 1.0));
-    d.md_on = q.md_on;
-    d.md_vx = q.md_vx; d.md_vy = q.md_vy; d.md_t = q.md_t;
-    if (q.md_on) c->md_any = c->ext_any = true;
-
     // attempt 0 starts from block 0; slot set 2 (attempt -1) never ran
-    auto& t = ctl[2 * (size_t)k];
-    std::memset(&t, 0, sizeof(t));
-    t.dt = q.dt0;
-    t.T = q.T0;
-    t.target_acc = 0;
-    t.T_stop = std::numeric_limits<double>::infinity();
-    set_ctl_derived(t, d);
-    ctl[2 * (size_t)k + 1] = t;
+    initial_ctl(q, d, ctl[2 * (size_t)k]);
+    ctl[2 * (size_t)k + 1] = ctl[2 * (size_t)k];
   }
   e = hipMemcpy(c->d_ctl, ctl.data(), sizeof(fv::Ctl) * 2 * n_inst, hipMemcpyHostToDevice);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipMemcpy ctl"); }
@@ -671,15 +736,21 @@ int fivonagi_set_dt_sequence(fivonagi_ctx* c, int inst, const double* dt, int64_
   if (dt) {
     for (int64_t k = 0; k < n; ++k)
       if (!(dt[k] > 0)) return fail(FIVONAGI_ERR_ARG, "dt sequence values must be > 0");
-    hipError_t e;
-    double* dd = (double*)c->dalloc(sizeof(double) * n, &e);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc dt_seq");
-    HIP_TRY(hipMemcpy(dd, dt, sizeof(double) * n, hipMemcpyHostToDevice));
-    d.dt_seq = dd;
-    d.dt_seq_len = n;
-    // the sequence is indexed by the attempt counter
+    // the sequence is indexed by the attempt counter: it must hold an entry
+    // for the next attempt (checked before anything changes)
     auto& t = ctl[inst];
     if (t.n_att >= n) return fail(FIVONAGI_ERR_ARG, "dt sequence shorter than attempts made");
+    // one device buffer per instance, reused while it is large enough
+    if (c->dt_seq_cap[inst] < n) {
+      double* dd = nullptr;
+      HIP_TRY(hipMalloc(&dd, sizeof(double) * n));
+      if (c->dt_seq_dev[inst]) (void)hipFree(c->dt_seq_dev[inst]);
+      c->dt_seq_dev[inst] = dd;
+      c->dt_seq_cap[inst] = n;
+    }
+    HIP_TRY(hipMemcpy(c->dt_seq_dev[inst], dt, sizeof(double) * n, hipMemcpyHostToDevice));
+    d.dt_seq = c->dt_seq_dev[inst];
+    d.dt_seq_len = n;
     t.dt = dt[t.n_att];
     set_ctl_derived(t, d);
     if (int rc = write_ctls(c, ctl)) return rc;
@@ -710,6 +781,18 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
       (void)hipMemcpyAsync((char*)keep[k] + es * (size_t)f * n, c->desc[k].q[saved[k].cur][f],
                            es * n, hipMemcpyDeviceToDevice, c->stream);
   }
+  // the trials run with data collection and prescribed dt sequences switched
+  // off: they must neither record into the measurement grid (or its overflow
+  // flag) nor use up the sequence; both come back with the saved state
+  const std::vector<fv::InstDesc> desc_saved = c->desc;
+  const int collect_nx_saved = c->collect_nx;
+  for (auto& d : c->desc) {
+    d.dt_seq = nullptr;
+    d.dt_seq_len = 0;
+    d.measure = nullptr;
+    d.stage = nullptr;
+  }
+  c->collect_nx = 0;
   const int auto_h = auto_rows(c->params, c->cus * c->per_cu);
   std::vector<int> cands = {auto_h, 8, 12, 16, 20, 24, 32, 40, 48, 64, 96, 128};
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -752,11 +835,14 @@ int fivonagi_autotune(fivonagi_ctx* c, int trial_steps, int32_t* rows_out) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  // restore: chosen work split, the saved state in the saved buffer, counters
+  // restore: chosen work split, the saved state in the saved buffer, counters,
+  // data collection and dt sequences (build_items sets n_items afresh)
   c->params = orig;
   for (auto& p : c->params) p.rows_per_wave = best_h;
   c->skew = best_skew;
   c->rotate = best_rot;
+  c->desc = desc_saved;
+  c->collect_nx = collect_nx_saved;
   build_items(c, best_h, c->skew, c->rotate);
   int rc2 = upload_items(c);
   for (int k = 0; k < c->n_inst; ++k) {

@@ -297,8 +297,9 @@ __device__ Ctl advance(const InstDesc& D, const Fold<R>& f, bool log) {
   }
   double ndt;
   if (presc) {
+    // attempt att uses dt[att]; past the end the dt is kept, and an attempt
+    // that would run with it is refused (prescribed_exhausted)
     ndt = (att < D.dt_seq_len) ? D.dt_seq[att] : p.dt;
-    if (att >= D.dt_seq_len) o.err = 9;   // ran past the prescribed sequence
   } else {
     ndt = p.dt * D.cfl_wish / cfl;
     if (!acc && ndt > p.dt * 0.9) ndt = p.dt * 0.9;
@@ -325,13 +326,25 @@ __device__ Ctl advance(const InstDesc& D, const Fold<R>& f, bool log) {
   return o;
 }
 
+// An attempt that would run (instance not done, not stopped, no error) while
+// its prescribed dt sequence has no entry for it: error 9 instead of running.
+// The check sits where an attempt is about to run, not where the last entry
+// is used, so a sequence of exactly n entries runs n steps.
+__device__ __forceinline__ void prescribed_exhausted(const InstDesc& D, Ctl& P) {
+  if (D.dt_seq != nullptr && P.err == 0 && P.n_att >= D.dt_seq_len &&
+      P.n_acc < P.target_acc && P.T < P.T_stop)
+    P.err = 9;
+}
+
 // Control block of the attempt a launch runs (phase = a mod 6), read-only.
 template <typename R>
 __device__ __forceinline__ Ctl current_t(const InstDesc& D, int phase, bool log) {
-  return advance(D,
-                 fold_decide(fold_load<R>(D.ctl + (phase & 1), D.sync + (phase + 2) % 3),
-                             D.cfl_max, D.dt_seq != nullptr),
-                 log);
+  Ctl P = advance(D,
+                  fold_decide(fold_load<R>(D.ctl + (phase & 1), D.sync + (phase + 2) % 3),
+                              D.cfl_max, D.dt_seq != nullptr),
+                  log);
+  prescribed_exhausted(D, P);
+  return P;
 }
 __device__ __forceinline__ Ctl current(const InstDesc& D, int phase, bool log) {
   return D.fp64 ? current_t<double>(D, phase, log) : current_t<float>(D, phase, log);
@@ -493,7 +506,8 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   load_raw(it.y0 - kHalo, n1, n2, n3, nec, nes, nh);   // in flight during advance()
 
   // ---- control block of this attempt ----
-  const Ctl P = advance<R>(D, F, local == 0);
+  Ctl P = advance<R>(D, F, local == 0);
+  prescribed_exhausted(D, P);
   const bool runs = P.err == 0 && P.n_acc < P.target_acc && P.T < P.T_stop;
   if (local == 0 && lane == 0) {
     // one wave per instance: publish the control block for the next launch,

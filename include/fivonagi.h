@@ -57,8 +57,8 @@ typedef struct {
   double dt0, T0;          /* DTINI, -TIDE (common/main.cu:788, :792)         */
   int32_t bc, cfl_mode;    /* FIVONAGI_BC_*, FIVONAGI_CFL_*                   */
   double omega;            /* edge drive: p_j = amp_j sin(omega T + phase_j)  */
-  const double *edge_amp;  /* host, ny values (NULL = 0)                      */
-  const double *edge_phase;/* host, ny values (NULL = 0)                      */
+  const double *edge_amp;  /* host, ny values (NULL = 0); copied by init       */
+  const double *edge_phase;/* host, ny values (NULL = 0); copied by init       */
   double tay_rhoa, tay_c, tay_theta, tay_yoff; /* Taylor closed form          */
   int32_t tay_jc;
   int32_t rows_per_wave;   /* 0 = automatic; else rows per work item          */
@@ -108,7 +108,11 @@ int fivonagi_init_f64(fivonagi_ctx **ctx, const fivonagi_params *p, int n_inst,
 int fivonagi_set_stream(fivonagi_ctx *ctx, void *hip_stream);
 
 /* Prescribed dt: attempt k of instance inst uses dt[k]; no CFL feedback and
-   no rejection (NULL/0 restores the feedback of common/main.cu:214-230). */
+   no rejection (NULL/0 restores the feedback of common/main.cu:214-230).
+   A sequence of n entries runs attempts up to index n-1; a step call that
+   needs an attempt past the end returns FIVONAGI_ERR_DTSEQ.  The sequence
+   must hold an entry for the next attempt (FIVONAGI_ERR_ARG otherwise, with
+   nothing changed). */
 int fivonagi_set_dt_sequence(fivonagi_ctx *ctx, int inst, const double *dt, int64_t n);
 
 /* The while (T < Tprint) body of common/main.cu:310-635, nsteps accepted
@@ -129,8 +133,10 @@ int fivonagi_sync(fivonagi_ctx *ctx);
 /* Autotuner (no counterpart in the reference): time trial_steps step
    attempts for each candidate height of the work items (rows per wave), then
    for each height skew at the best height, keep the fastest, and restore the
-   state and counters as they were.  rows_per_wave_out (may be NULL) receives
-   the height; fivonagi_info reports the skew. */
+   state and counters as they were.  The trials run with data collection and
+   prescribed dt sequences switched off (both are restored, untouched).
+   rows_per_wave_out (may be NULL) receives the height; fivonagi_info reports
+   the skew. */
 int fivonagi_autotune(fivonagi_ctx *ctx, int trial_steps, int32_t *rows_per_wave_out);
 
 /* Work split (no counterpart in the reference): skew of the work-item

@@ -34,7 +34,8 @@ def test_library_has_gfx950_code_object():
 
 
 STRUCTS = (("fivonagi_params", fa.Params), ("fivonagi_info_t", fa.Info),
-           ("fivonagi_objective", fa.Objective), ("fivonagi_adjoint_opts", fa.AdjointOpts))
+           ("fivonagi_objective", fa.Objective), ("fivonagi_adjoint_opts", fa.AdjointOpts),
+           ("fivonagi_collect_opts", fa.CollectOpts))
 
 
 def test_struct_layout_matches_header(tmp_path):
