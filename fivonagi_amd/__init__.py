@@ -23,7 +23,7 @@ EXPORTS = (
     "fivonagi_step_adjoint", "fivonagi_adjoint_state", "fivonagi_autotune", "fivonagi_set_skew",
     "fivonagi_set_collect", "fivonagi_measure", "fivonagi_measure_f64", "fivonagi_measure_rows",
     "fivonagi_measure_rows_f64", "fivonagi_cut_errors", "fivonagi_cut_errors_f64",
-    "fivonagi_cfl_subset",
+    "fivonagi_cfl_subset", "fivonagi_step_noalloc",
     "fivonagi_destroy", "fivonagi_last_error",
 )
 
@@ -74,6 +74,14 @@ class CollectOpts(C.Structure):
                 ("cut_half", C.c_int32), ("pad", C.c_int32), ("tan_theta", C.c_double)]
 
 
+class NoallocState(C.Structure):
+    """fivonagi_noalloc_state (include/fivonagi.h)."""
+    _fields_ = [("T", C.c_double), ("dt", C.c_double),
+                ("n_accepted", C.c_int64), ("n_rejected", C.c_int64), ("n_attempts", C.c_int64),
+                ("md_x", C.c_int32), ("md_y", C.c_int32),
+                ("error", C.c_int32), ("pad", C.c_int32)]
+
+
 class AdjointOpts(C.Structure):
     """fivonagi_adjoint_opts (include/fivonagi.h)."""
     _fields_ = [("checkpoint_every", C.c_int32), ("plain_forward", C.c_int32)]
@@ -122,6 +130,8 @@ def load():
     lib.fivonagi_cut_errors.argtypes = [fp, fp, C.c_int, fp]
     lib.fivonagi_cfl_subset.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_uint8)]
     lib.fivonagi_cut_errors_f64.argtypes = [dp, dp, C.c_int, fp]
+    lib.fivonagi_step_noalloc.argtypes = [P, C.POINTER(vp), C.c_int64, C.POINTER(NoallocState),
+                                          vp]
     lib.fivonagi_destroy.argtypes = [vp]
     lib.fivonagi_destroy.restype = None
     lib.fivonagi_last_error.argtypes = []
@@ -134,7 +144,7 @@ def load():
                  "fivonagi_autotune", "fivonagi_set_skew", "fivonagi_set_collect",
                  "fivonagi_measure", "fivonagi_measure_f64", "fivonagi_measure_rows",
                  "fivonagi_measure_rows_f64", "fivonagi_cut_errors", "fivonagi_cut_errors_f64",
-                 "fivonagi_cfl_subset"):
+                 "fivonagi_cfl_subset", "fivonagi_step_noalloc"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -398,3 +408,18 @@ def cfl_subset(nx, ny):
     _check(load().fivonagi_cfl_subset(int(nx), int(ny), m.ctypes.data_as(C.POINTER(C.c_uint8))))
     return m
 
+
+def step_noalloc(prob, q_dev, nsteps, state=None, stream=None):
+    """fivonagi_step_noalloc: nsteps accepted steps of prob's instance on a
+    state the caller keeps on the device.  q_dev: six device addresses (the
+    state's three fields, then a work buffer of the same size); state: a
+    NoallocState (None: start at prob's T0 and dt0).  Returns the state after
+    the call; the fields end in q_dev[0..2]."""
+    if state is None:
+        state = NoallocState()
+        state.T, state.dt = prob.T0, prob.dt0
+    p = prob.cparams()
+    qs = (C.c_void_p * 6)(*[C.c_void_p(int(a)) for a in q_dev])
+    _check(load().fivonagi_step_noalloc(C.byref(p), qs, int(nsteps), C.byref(state),
+                                        C.c_void_p(stream)))
+    return state

@@ -38,6 +38,7 @@ extern "C" hipError_t fv_adj_march_reverse(const fv::AdjDesc* ds, const fv::AdjI
 extern "C" hipError_t fv_adj_march_occupancy(int* x_fwd, int* x_rev, int* y_rev);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, int phase, hipStream_t stream);
+extern "C" hipError_t fv_scratch_get(void** ptr, size_t* bytes);
 
 namespace {
 
@@ -1388,6 +1389,133 @@ int fivonagi_adjoint_state(fivonagi_ctx* c, int inst, float* l1, float* l2, floa
     if (dst[f])
       HIP_TRY(hipMemcpy(dst[f], c->adj_lam[inst] + (size_t)f * n, sizeof(float) * n,
                         hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fivonagi_step_noalloc(const fivonagi_params* p, float* const q[6], int64_t nsteps,
+                          fivonagi_noalloc_state* st, void* hip_stream) {
+  if (!p || !q || !st || nsteps < 0) return fail(FIVONAGI_ERR_ARG, "bad argument");
+  for (int f = 0; f < 6; ++f)
+    if (!q[f]) return fail(FIVONAGI_ERR_ARG, "null state pointer");
+  if (int rc = check_params(*p, false)) return rc;
+  if (!(st->dt > 0) || !std::isfinite(st->T)) return fail(FIVONAGI_ERR_ARG, "state: dt must be > 0");
+  if (st->md_x < 0 || st->md_y < 0) return fail(FIVONAGI_ERR_ARG, "state: MDX, MDY must be >= 0");
+  hipStream_t stream = (hipStream_t)hip_stream;
+  void* base = nullptr;
+  size_t cap = 0;
+  HIP_TRY(fv_scratch_get(&base, &cap));
+
+  // host side of the instance and its work list; `tmp` is a host-only
+  // context holding what build_items reads (it owns no device memory)
+  HostInst h;
+  host_inst(*p, false, h);
+  fivonagi_ctx tmp;
+  tmp.n_inst = 1;
+  tmp.params.assign(1, *p);
+  tmp.desc.assign(1, h.d);
+  tmp.fold_bits.assign(1, h.fold_bits);
+  HIP_TRY(hipGetDevice(&tmp.device));
+  device_shape(tmp.device, false, &tmp.cus, &tmp.per_cu);
+  build_items(&tmp, auto_rows(tmp.params, tmp.cus * tmp.per_cu), tmp.skew, tmp.rotate);
+  fv::InstDesc d = tmp.desc[0];
+
+  // the buffer's layout, each part 256-byte aligned
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = (off + bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_desc = take(sizeof(fv::InstDesc));
+  const size_t o_ctl = take(2 * sizeof(fv::Ctl));
+  const size_t o_sync = take(3 * sizeof(fv::SyncSlots));
+  const size_t o_ec = take(sizeof(float) * h.ec.size());
+  const size_t o_es = take(sizeof(float) * h.es.size());
+  const size_t o_tab = take(sizeof(float) * h.tab.size());
+  const size_t o_items = take(sizeof(fv::WorkItem) * tmp.items.size());
+  const size_t o_mask = take(sizeof(uint64_t) * tmp.rowmask.size());
+  if (off > cap)
+    return fail(FIVONAGI_ERR_ARG, "fivonagi_step_noalloc: the grid's tables need " +
+                                      std::to_string(off) + " bytes, more than the " +
+                                      std::to_string(cap) + "-byte buffer");
+  char* dev = (char*)base;
+  d.q[0][0] = q[0]; d.q[0][1] = q[1]; d.q[0][2] = q[2];
+  d.q[1][0] = q[3]; d.q[1][1] = q[4]; d.q[1][2] = q[5];
+  d.ctl = (fv::Ctl*)(dev + o_ctl);
+  d.sync = (fv::SyncSlots*)(dev + o_sync);
+  d.edge_c = dev + o_ec;
+  d.edge_s = dev + o_es;
+  d.hifu_tab = h.tab.empty() ? nullptr : dev + o_tab;
+
+  // control blocks of the first attempt (both blocks alike; the slot sets
+  // start at zero: attempt -1 never ran)
+  fv::Ctl c0;
+  initial_ctl(*p, d, c0);
+  c0.dt = st->dt;
+  c0.T = st->T;
+  c0.n_acc = st->n_accepted;
+  c0.n_rej = st->n_rejected;
+  c0.n_att = st->n_attempts;
+  c0.mdx = st->md_x;
+  c0.mdy = st->md_y;
+  c0.target_acc = st->n_accepted + nsteps;
+  set_ctl_derived(c0, d);
+
+  // every table through one copy of a zero-initialised host image (the
+  // reduction slots are the zeros)
+  std::vector<char> img(off, 0);
+  std::memcpy(img.data() + o_desc, &d, sizeof(d));
+  std::memcpy(img.data() + o_ctl, &c0, sizeof(c0));
+  std::memcpy(img.data() + o_ctl + sizeof(fv::Ctl), &c0, sizeof(c0));
+  table_bytes(h.ec, false, img.data() + o_ec);
+  table_bytes(h.es, false, img.data() + o_es);
+  if (!h.tab.empty()) table_bytes(h.tab, false, img.data() + o_tab);
+  std::memcpy(img.data() + o_items, tmp.items.data(), sizeof(fv::WorkItem) * tmp.items.size());
+  std::memcpy(img.data() + o_mask, tmp.rowmask.data(), sizeof(uint64_t) * tmp.rowmask.size());
+  HIP_TRY(hipMemcpyAsync(dev, img.data(), off, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));   // img may go once the copy is done
+
+  const fv::InstDesc* d_desc = (const fv::InstDesc*)(dev + o_desc);
+  const fv::WorkItem* d_items = (const fv::WorkItem*)(dev + o_items);
+  const uint64_t* d_mask = (const uint64_t*)(dev + o_mask);
+  const int n_items = (int)tmp.items.size();
+  const int ext = h.ext ? 1 : 0;
+  int64_t a = 0;                 // attempts launched by this call (phase = a mod 6)
+  fv::Ctl now = c0;
+  int64_t remaining = nsteps;
+  for (int round = 0; remaining > 0; ++round) {
+    if (round > 10000) return fail(FIVONAGI_ERR_STALL, "steps keep getting rejected");
+    for (int64_t k = 0; k < remaining; ++k, ++a) {
+      const int phase = (int)(a % 6);
+      if (h.taylor_ring > 0)
+        HIP_TRY(fv_launch_taylor_ghosts(d_desc, 1, h.taylor_ring, phase, stream));
+      HIP_TRY(fv_launch_step(d_desc, d_items, d_mask, n_items, phase, 1, d.ctl, d.sync, ext, 0,
+                             stream));
+    }
+    // the control block after the last attempt (fv_settle_kernel), read back
+    const int phase = (int)(a % 6);
+    HIP_TRY(fv_launch_settle(d_desc, 1, phase, stream));
+    HIP_TRY(hipMemcpyAsync(&now, d.ctl + ((phase + 1) & 1), sizeof(fv::Ctl),
+                           hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (now.err) break;
+    remaining = now.target_acc - now.n_acc;
+  }
+  // the state ends in q[0..2]
+  if (now.cur == 1)
+    for (int f = 0; f < 3; ++f)
+      HIP_TRY(hipMemcpyAsync(q[f], q[3 + f], sizeof(float) * (size_t)p->nx * p->ny,
+                             hipMemcpyDeviceToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  st->T = now.T;
+  st->dt = now.dt;
+  st->n_accepted = now.n_acc;
+  st->n_rejected = now.n_rej;
+  st->n_attempts = now.n_att;
+  st->md_x = now.mdx;
+  st->md_y = now.mdy;
+  st->error = now.err;
+  if (now.err == 2) return fail(FIVONAGI_ERR_EIGEN, "imaginary eigenvalue (tmp3 < 0)");
   return 0;
 }
 

@@ -243,6 +243,38 @@ int fivonagi_cut_errors(const float *numeric, const float *analytic, int length,
 int fivonagi_cut_errors_f64(const double *numeric, const double *analytic, int length,
                             float errors[3]);
 
+/* ---- step without device allocation ---------------------------------------
+   No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  fivonagi_step for one FP32 instance whose state the
+   caller keeps on the device: the instance description, control blocks, CFL
+   reduction slots, edge tables and work lists live in one fixed device buffer
+   of FIVONAGI_NOALLOC_SCRATCH_BYTES per GPU, set up on the first call on a
+   device and held until the process ends.  The call allocates no device
+   memory (no hipMalloc, hipFree or allocating library routine).
+     q[0..2]  device pointers: rho', rho u, rho v (nx*ny floats each, offset
+              i + j*nx, ghost frame inside): the state, in and out;
+     q[3..5]  device pointers: a work buffer of the same size (overwritten);
+     st       in: T and dt of the next attempt, the counters and MDX, MDY;
+              out: the same after the call, and error;
+     nsteps   accepted steps (rejected attempts are retried); blocking.
+   Every boundary kind and the moving window run as in fivonagi_init; the
+   edge arrays of p are read during the call only.  No data collection, no
+   prescribed dt.  A grid whose tables do not fit the buffer (the CFL lane
+   masks take 8 bytes per 58 cells: beyond about 60 million cells, e.g.
+   8192 x 8192) returns FIVONAGI_ERR_ARG.  All calls on one GPU share
+   its buffer, which keeps the last call's tables: calls that overlap on one
+   GPU (from several host threads, or on streams that run concurrently) race. */
+#define FIVONAGI_NOALLOC_SCRATCH_BYTES (8u << 20)
+typedef struct {
+  double T, dt;            /* time and dt of the next attempt (dt > 0)        */
+  int64_t n_accepted, n_rejected, n_attempts;
+  int32_t md_x, md_y;      /* moving-window offset MDX, MDY                   */
+  int32_t error, pad;      /* out: 0, or FIVONAGI_ERR_EIGEN                   */
+} fivonagi_noalloc_state;
+int fivonagi_step_noalloc(const fivonagi_params *p, float *const q[6], int64_t nsteps,
+                          fivonagi_noalloc_state *st, void *hip_stream);
+
 /* clean_gpu (common/data_definitions.h:415-517) */
 void fivonagi_destroy(fivonagi_ctx *ctx);
 

@@ -35,7 +35,8 @@ def test_library_has_gfx950_code_object():
 
 STRUCTS = (("fivonagi_params", fa.Params), ("fivonagi_info_t", fa.Info),
            ("fivonagi_objective", fa.Objective), ("fivonagi_adjoint_opts", fa.AdjointOpts),
-           ("fivonagi_collect_opts", fa.CollectOpts))
+           ("fivonagi_collect_opts", fa.CollectOpts),
+           ("fivonagi_noalloc_state", fa.NoallocState))
 
 
 def test_struct_layout_matches_header(tmp_path):
@@ -73,6 +74,16 @@ def test_argument_errors_do_not_touch_the_gpu():
     assert lib.fivonagi_init(ctypes.byref(h), ctypes.byref(p), 1, arr) == fa.ERR_ARG
     assert lib.fivonagi_step(None, 1) == fa.ERR_ARG
     assert lib.fivonagi_sync(None) == fa.ERR_ARG
+    # the no-allocation step checks its arguments before any HIP call
+    st = fa.NoallocState()
+    st.dt = 0.01
+    qs = (ctypes.c_void_p * 6)()
+    assert lib.fivonagi_step_noalloc(None, qs, 1, ctypes.byref(st), None) == fa.ERR_ARG
+    assert lib.fivonagi_step_noalloc(ctypes.byref(p), qs, 1, ctypes.byref(st), None) == fa.ERR_ARG
+    assert b"null state" in lib.fivonagi_last_error()
+    qs = (ctypes.c_void_p * 6)(*[ctypes.c_void_p(4096)] * 6)
+    assert lib.fivonagi_step_noalloc(ctypes.byref(p), qs, 1, ctypes.byref(st), None) == fa.ERR_ARG
+    assert b">= 8" in lib.fivonagi_last_error()
 
 
 @pytest.mark.parametrize("fp64,nx,ny,ok", [(False, 32768, 32768, True), (False, 32768, 32769, False),
