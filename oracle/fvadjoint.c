@@ -26,6 +26,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "fvoracle.h"
 
@@ -291,9 +294,6 @@ typedef struct {
   int nx, ny;
   size_t n;
   double *b[3], *y[3], *xy[3];     /* q_b, q_y, q_xy of the current step */
-  line_ws wy, wx;
-  double (*lq)[3], (*lo)[3], (*la)[3], (*lb)[3];
-  unsigned char *int_y, *int_x;
 } step_ws;
 
 static int ws_alloc(step_ws *s, int nx, int ny) {
@@ -305,21 +305,11 @@ static int ws_alloc(step_ws *s, int nx, int ny) {
     s->xy[f] = calloc(s->n, sizeof(double));
     if (!s->b[f] || !s->y[f] || !s->xy[f]) return -1;
   }
-  int L = nx > ny ? nx : ny;
-  if (line_alloc(&s->wy, ny) || line_alloc(&s->wx, nx)) return -1;
-  s->lq = calloc((size_t)L, sizeof(*s->lq));
-  s->lo = calloc((size_t)L, sizeof(*s->lo));
-  s->la = calloc((size_t)L, sizeof(*s->la));
-  s->lb = calloc((size_t)L, sizeof(*s->lb));
-  s->int_y = calloc((size_t)L, 1);
-  s->int_x = calloc((size_t)L, 1);
-  return (s->lq && s->lo && s->la && s->lb && s->int_y && s->int_x) ? 0 : -1;
+  return 0;
 }
 
 static void ws_free(step_ws *s) {
   for (int f = 0; f < 3; ++f) { free(s->b[f]); free(s->y[f]); free(s->xy[f]); }
-  line_free(&s->wy); line_free(&s->wx);
-  free(s->lq); free(s->lo); free(s->la); free(s->lb); free(s->int_y); free(s->int_x);
 }
 
 /* ghost source of cell (i,j) for the copy rules (common/boundary.h:27-81) */
@@ -401,43 +391,59 @@ static void boundary_bw(const orc_params *p, double *const lb[3], double *const 
 }
 
 /* the y sweep on columns (normal = q3, tangential = q2) or the x sweep on rows
-   (normal = q2, tangential = q3); fw: src -> dst; bw: lam_out -> lam_in */
+   (normal = q2, tangential = q3); fw: src -> dst; bw: lam_out -> lam_in.
+   The lines are independent: each thread works through its own lines with
+   its own line workspace, so the result does not depend on the threads. */
 static int sweep_lines(step_ws *s, const orc_params *p, int axis, double *const src[3],
                        double *const dst[3], double dtd, int backward, double *const lam_out[3],
                        double *const lam_in[3]) {
   int NX = s->nx, NY = s->ny;
   int nlines = axis == 1 ? NX : NY, L = axis == 1 ? NY : NX;
-  line_ws *ws = axis == 1 ? &s->wy : &s->wx;
   int fn = axis == 1 ? 2 : 1, ft = axis == 1 ? 1 : 2;
-  for (int line = 0; line < nlines; ++line) {
-    for (int k = 0; k < L; ++k) {
-      int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
-      size_t o = (size_t)i + (size_t)j * NX;
-      s->lq[k][0] = src[0][o]; s->lq[k][1] = src[fn][o]; s->lq[k][2] = src[ft][o];
-      s->int_y[k] = (i >= 2 && i <= NX - 3 && j >= 2 && j <= NY - 3);
+  int err = 0;
+#pragma omp parallel reduction(| : err)
+  {
+    line_ws ws;
+    double(*lq)[3] = calloc((size_t)L, sizeof(*lq)), (*lo)[3] = calloc((size_t)L, sizeof(*lo));
+    double(*la)[3] = calloc((size_t)L, sizeof(*la)), (*lb)[3] = calloc((size_t)L, sizeof(*lb));
+    unsigned char *in = calloc((size_t)L, 1);
+    int ok = !line_alloc(&ws, L) && lq && lo && la && lb && in;
+    if (!ok) err |= 2;
+#pragma omp for schedule(static)
+    for (int line = 0; line < nlines; ++line) {
+      if (!ok) continue;
+      for (int k = 0; k < L; ++k) {
+        int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
+        size_t o = (size_t)i + (size_t)j * NX;
+        lq[k][0] = src[0][o]; lq[k][1] = src[fn][o]; lq[k][2] = src[ft][o];
+        in[k] = (i >= 2 && i <= NX - 3 && j >= 2 && j <= NY - 3);
+      }
+      if (line_fw(&ws, lq, lo, in, p->beta, dtd)) { err |= 1; continue; }
+      if (!backward) {
+        for (int k = 0; k < L; ++k) {
+          int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
+          size_t o = (size_t)i + (size_t)j * NX;
+          dst[0][o] = lo[k][0]; dst[fn][o] = lo[k][1]; dst[ft][o] = lo[k][2];
+        }
+      } else {
+        for (int k = 0; k < L; ++k) {
+          int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
+          size_t o = (size_t)i + (size_t)j * NX;
+          la[k][0] = lam_out[0][o]; la[k][1] = lam_out[fn][o]; la[k][2] = lam_out[ft][o];
+        }
+        line_bw(&ws, la, lb, in, p->beta, dtd);
+        for (int k = 0; k < L; ++k) {
+          int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
+          size_t o = (size_t)i + (size_t)j * NX;
+          lam_in[0][o] = lb[k][0]; lam_in[fn][o] = lb[k][1]; lam_in[ft][o] = lb[k][2];
+        }
+      }
     }
-    if (line_fw(ws, s->lq, s->lo, s->int_y, p->beta, dtd)) return ORC_ERR_EIGEN;
-    if (!backward) {
-      for (int k = 0; k < L; ++k) {
-        int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
-        size_t o = (size_t)i + (size_t)j * NX;
-        dst[0][o] = s->lo[k][0]; dst[fn][o] = s->lo[k][1]; dst[ft][o] = s->lo[k][2];
-      }
-    } else {
-      for (int k = 0; k < L; ++k) {
-        int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
-        size_t o = (size_t)i + (size_t)j * NX;
-        s->la[k][0] = lam_out[0][o]; s->la[k][1] = lam_out[fn][o]; s->la[k][2] = lam_out[ft][o];
-      }
-      line_bw(ws, s->la, s->lb, s->int_y, p->beta, dtd);
-      for (int k = 0; k < L; ++k) {
-        int i = axis == 1 ? line : k, j = axis == 1 ? k : line;
-        size_t o = (size_t)i + (size_t)j * NX;
-        lam_in[0][o] = s->lb[k][0]; lam_in[fn][o] = s->lb[k][1]; lam_in[ft][o] = s->lb[k][2];
-      }
-    }
+    line_free(&ws);
+    free(lq); free(lo); free(la); free(lb); free(in);
   }
-  return 0;
+  if (err & 2) return ORC_ERR_ARG;
+  return err ? ORC_ERR_EIGEN : 0;
 }
 
 /* apps/na/source.h:84-104 (q1 passes through) */
@@ -531,6 +537,9 @@ int orc_line_step_f64(const orc_params *p, double *q1, double *q2, double *q3, d
 int orc_adjoint_f64(const orc_params *p, const double *q0, int64_t nsteps, const orc_box *bx,
                     double *J, double *grad_amp, double *grad_phase, double *grad_q0) {
   if (check(p, nsteps, bx) || !q0 || !J) return ORC_ERR_ARG;
+#ifdef _OPENMP
+  if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
   step_ws s;
   if (ws_alloc(&s, p->nx, p->ny)) { ws_free(&s); return ORC_ERR_ARG; }
   size_t n = s.n;
