@@ -611,3 +611,183 @@ def test_taylor_collection_zeroes_before_T0_and_overflows(oracle):
     assert oracle.collect(op, q1, -0.5, 3) == 0 and not op.measure.any()
     op.n0frame = 0
     assert oracle.collect(op, q1, 1.0, p.nx + 1) == oracle.ERR_COLLECT
+
+
+# -------------------------------------------------------- dt rule (update_dt) --
+# common/main.cu:199-230: cfl = max over the counted cells of |s| dt/d (both
+# sweeps); accept when cfl < CFLMAX; then dt <- dt CFLWHISH / cfl, and on a
+# rejection dt <- 0.9 dt when that is smaller.  T grows by the dt of accepted
+# steps only (:232-238).  A uniform flow leaves the state unchanged and has
+# closed-form speeds (test_uniform_flow_speeds): along x |U| + sqrt(1 - U^2),
+# along y (no normal flow) 1.
+
+def _uniform_flow(U, dx, dy, dt0, cfl_wish, nx=32, ny=24):
+    p = wl.gauss_pulse(nx, ny, amp=0.0, dt0=dt0)
+    p.dx, p.dy, p.cfl_wish = dx, dy, cfl_wish
+    q0 = np.zeros((3, ny, nx))
+    q0[1] = U                                  # rho = 1 moving at U along x
+    p.q0 = q0
+    return p
+
+
+def _uniform_cfl_per_dt(U, dx, dy):
+    return max((abs(U) + math.sqrt(1 - U * U)) / dx, 1.0 / dy)
+
+
+@pytest.mark.parametrize("dy", [0.06, 0.03])       # the x sweep, then the y sweep, decides
+def test_dt_rule_accepted_step(oracle, dy):
+    U, dx, dt0, cw = 0.3, 0.05, 0.004, 0.9
+    p = _uniform_flow(U, dx, dy, dt0, cw)
+    q, st, rc = oracle.run(to_oracle(p), p.q0, 1)
+    assert rc == 0 and st.n_accepted == 1 and st.n_rejected == 0
+    c = _uniform_cfl_per_dt(U, dx, dy)
+    assert st.last_cfl == pytest.approx(c * dt0, rel=1e-14)
+    assert st.dt == pytest.approx(cw / c, rel=1e-14)          # dt0 * cw / (c dt0)
+    assert st.T == pytest.approx(dt0, rel=1e-15)
+    # cfl_max plays no part in the new dt of an accepted step
+    p.cfl_max = 3.0
+    _, st2, _ = oracle.run(to_oracle(p), p.q0, 1)
+    assert st2.dt == st.dt
+
+
+def test_dt_rule_grows_without_the_rejection_cap(oracle):
+    # cfl 0.3 is accepted: the next dt is 3 dt0 (the 0.9 cap is for
+    # rejections only), and the step after it runs at cfl = cfl_wish
+    U, dx, dy, cw = 0.3, 0.05, 0.06, 0.9
+    c = _uniform_cfl_per_dt(U, dx, dy)
+    dt0 = 0.3 / c
+    p = _uniform_flow(U, dx, dy, dt0, cw)
+    _, st, rc = oracle.run(to_oracle(p), p.q0, 2)
+    assert rc == 0 and st.n_rejected == 0
+    assert st.T == pytest.approx(dt0 + 3 * dt0, rel=1e-14)
+    assert st.last_cfl == pytest.approx(cw, rel=1e-14)
+
+
+@pytest.mark.parametrize("cw,factor", [(0.9, 0.9 / 1.05), (0.99, 0.9)])
+def test_dt_rule_after_a_rejection(oracle, cw, factor):
+    # cfl 1.05 >= CFLMAX = 1: rejected, the state left alone; the retry runs
+    # at dt0 cw / 1.05 when that is below 0.9 dt0, else at 0.9 dt0.  Its T
+    # shows the dt it ran with.
+    U, dx, dy = 0.3, 0.05, 0.06
+    c = _uniform_cfl_per_dt(U, dx, dy)
+    dt0 = 1.05 / c
+    p = _uniform_flow(U, dx, dy, dt0, cw)
+    q, st, rc = oracle.run(to_oracle(p), p.q0, 1)
+    assert rc == 0 and st.n_rejected == 1 and st.n_accepted == 1 and st.n_attempts == 2
+    assert st.T == pytest.approx(factor * dt0, rel=1e-14)
+    assert st.dt == pytest.approx(cw / c, rel=1e-14)
+    np.testing.assert_array_equal(q, p.q0)
+
+
+def test_dt_rule_on_a_step_of_a_pulse(oracle):
+    # a non-uniform state on a 48x40 grid (a fold subset that is not the even
+    # offsets): the cfl of the step is the largest |s| dt/d over the counted
+    # cells (subset, i >= 1 for x interfaces, j >= 1 for y ones), with the y
+    # speeds on the state after the ghost cells and the x speeds on the y
+    # sweep's result, each from the per-interface solve
+    p = wl.gauss_pulse(48, 40, sigma_cells=3.0, amp=0.05, dt0=0.02)
+    op = to_oracle(p)
+    q0 = p.q0.astype(np.float64)
+    _, st, rc = oracle.run(op, q0, 1)
+    assert rc == 0 and st.n_accepted == 1 and st.n_rejected == 0
+    sub = oracle.cfl_subset(p.nx, p.ny)
+    assert 0 < sub.mean() < 0.5 and not np.array_equal(sub, _even(p.nx, p.ny))
+    qb, _ = oracle.boundary(op, q0, p.T0)
+    qy, _, _ = oracle.sweep(op, 1, qb, p.dt0)
+    cmax = 0.0
+    for j in range(p.ny):
+        for i in range(1, p.nx):
+            if not sub[j, i]:
+                continue
+            if j >= 1:   # y interface (i, j - 1/2): normal rho v, tangential rho u
+                _, r = oracle.riemann([qb[0, j - 1, i], qb[2, j - 1, i], qb[1, j - 1, i]],
+                                      [qb[0, j, i], qb[2, j, i], qb[1, j, i]], p.beta)
+                cmax = max(cmax, np.abs(r["s"]).max() * p.dt0 / p.dy)
+            _, r = oracle.riemann([qy[0, j, i - 1], qy[1, j, i - 1], qy[2, j, i - 1]],
+                                  [qy[0, j, i], qy[1, j, i], qy[2, j, i]], p.beta)
+            cmax = max(cmax, np.abs(r["s"]).max() * p.dt0 / p.dx)
+    assert st.last_cfl == pytest.approx(cmax, rel=1e-14)
+    assert st.dt == pytest.approx(p.dt0 * p.cfl_wish / cmax, rel=1e-14)
+
+
+def _even(nx, ny):
+    return (np.arange(nx * ny) % 2 == 0).reshape(ny, nx).astype(np.uint8)
+
+
+# ----------------------------------------------- hifu-beam data collection --
+# apps/hifu-beam/data_collect.h:7-107, after the accepted step n of a frame
+# that started at n0 (entry nf = n - 1 - n0), on row JCUT of the new state:
+#   row 0 [nf] = T, row 1 [nf] = cfl;
+#   rows 2..5: microphone k hears u1 at the cell i with i + MDX == mic[k]
+#     (row 2 + k, entry nf); at nf == 0 the row is cleared first, and a
+#     microphone outside the window has its row cleared;
+#   u1max: every cell with |(i + MDX)/ETA - T| <= 0.5 keeps max(u1, m) at the
+#     flat index i + MDX + 6 NX (written only inside the grid).
+
+def _hifu_collect_problem(oracle, nx=96, ny=12, mic=(10, 30, 50, 200), eta=20.0):
+    p = wl.hifu(nx=nx, ny=ny, ispam=2.0)
+    op = to_oracle(p)
+    op.collect = dict(jcut=2, mic=mic, eta=eta)
+    op.measure = np.zeros((ny, nx))
+    return p, op
+
+
+def test_hifu_collection_microphones_and_peak(oracle):
+    p, op = _hifu_collect_problem(oracle)
+    nx, ny, eta = p.nx, p.ny, 20.0
+    I = np.arange(nx)
+    u1 = 1e-3 * np.sin(I / 5.0)
+    q1 = np.zeros((ny, nx))
+    q1[2] = u1                                   # JCUT = 2
+    q1[3] = 1.0                                  # other rows are not read
+    op.measure[:] = -5.0                         # nf = 0 clears the microphone rows
+    op.measure[6:] = 0.0
+    T = 2.0                                      # cells 30..50: |i/20 - 2| <= 0.5
+    assert oracle.collect(op, q1, T, n_accepted=1, cfl=0.7) == 0
+    m = op.measure
+    assert m[0, 0] == T and m[1, 0] == 0.7
+    for k, mk in enumerate((10, 30, 50)):
+        row = np.zeros(nx)
+        row[0] = u1[mk]
+        np.testing.assert_array_equal(m[2 + k], row)
+    assert not m[5].any()                        # microphone 200 is outside the grid
+    peak = np.zeros(nx)
+    w = np.abs(I / eta - T) <= 0.5
+    peak[w] = np.maximum(u1[w], 0.0)
+    np.testing.assert_array_equal(m[6], peak)
+    assert w.sum() == 21 and not m[7:].any()
+    # the next step of the frame, the window moved 4 cells: microphone k
+    # hears column mic[k] - 4 at entry 1, the maximum keeps the larger value
+    u1b = 1.5e-3 * np.cos(I / 7.0)
+    q1[2] = u1b
+    T2 = 2.1
+    assert oracle.collect(op, q1, T2, n_accepted=2, mdx=4, cfl=0.8) == 0
+    assert m[0, 1] == T2 and m[1, 1] == 0.8 and m[0, 0] == T
+    for k, mk in enumerate((10, 30, 50)):
+        assert m[2 + k, 0] == u1[mk] and m[2 + k, 1] == u1b[mk - 4]
+    w2 = np.abs((I + 4) / eta - T2) <= 0.5
+    peak2 = peak.copy()
+    idx = I[w2] + 4
+    peak2[idx] = np.maximum(u1b[w2], peak[idx])
+    np.testing.assert_array_equal(m[6], peak2)
+
+
+def test_hifu_collection_peak_past_the_grid_and_overflow(oracle):
+    # u1max index i + MDX + 6 NX is written only while it is inside the grid
+    # (NX NY cells); before T > 0 the grid is zeroed; more than NX steps in a
+    # frame is simError 4
+    p, op = _hifu_collect_problem(oracle, nx=64, ny=8)
+    nx = p.nx
+    q1 = np.zeros((p.ny, nx))
+    q1[2] = 0.25
+    mdx = 2 * nx - 10                            # i + mdx + 6 nx >= 8 nx for i >= 10
+    T = (5 + mdx) / 20.0                         # cells 0..15 in the peak window
+    assert oracle.collect(op, q1, T, n_accepted=1, mdx=mdx) == 0
+    flat = op.measure.reshape(-1)
+    written = np.nonzero(flat[6 * nx:])[0]
+    np.testing.assert_array_equal(written, np.arange(0, 10) + mdx)
+    assert np.all(flat[6 * nx + mdx: 6 * nx + mdx + 10] == 0.25)
+    op.measure[:] = 3.0
+    assert oracle.collect(op, q1, -0.1, n_accepted=2) == 0 and not op.measure.any()
+    op.n0frame = 0
+    assert oracle.collect(op, q1, 1.0, n_accepted=nx + 1) == oracle.ERR_COLLECT
