@@ -70,6 +70,30 @@ class Hip:
         return ms.value
 
 
+def warm_up(ctx, warmup, steps):
+    """Untimed: the warmup attempts, then the timed pattern once (its graph
+    replays are captured here, not inside the events), each followed by
+    attempts up to a phase-0 boundary of the six-launch rotation, so the
+    timed attempts replay whole captured graphs from their first launch."""
+    for n in (warmup, steps):
+        ctx.enqueue(n)
+        if ctx.sync() != 0:
+            raise RuntimeError("warmup failed")
+        ctx.enqueue((-ctx.info(0)["phase"]) % 6)
+    if ctx.sync() != 0:
+        raise RuntimeError("warmup failed")
+
+
+def profile_record(workload_tag):
+    """The kernel's measured HBM bytes and VALU figures per launch, from the
+    rocprofv3 counter runs recorded under profiles/ ({} if not measured)."""
+    path = os.path.join(ROOT, "profiles", f"{KERNEL}_{workload_tag}_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
 def traffic_from_profiles(workload_tag):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE,
     each scaled by the counter's reading for a copy of known size), recorded
@@ -81,12 +105,49 @@ def traffic_from_profiles(workload_tag):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def native_oracle():
+    """The oracle's sources built for this host's CPU (gcc -O3 -march=native
+    -fopenmp) in a temporary directory, so the CPU baseline uses every vector
+    extension of the machine it runs on; (path, flags)."""
+    import subprocess
+    import tempfile
+    src = os.path.join(ROOT, "oracle")
+    out = os.path.join(tempfile.mkdtemp(prefix="fv_oracle_native_"), "liboracle_native.so")
+    flags = ["-std=c11", "-O3", "-march=native", "-fPIC", "-shared", "-fopenmp",
+             "-ffp-contract=off"]
+    subprocess.run(["gcc", *flags, "-o", out, os.path.join(src, "fvoracle.c"),
+                    os.path.join(src, "fvadjoint.c"), "-lm"], check=True)
+    return out, " ".join(flags)
+
+
+def cpu_threads():
+    """Every CPU of this host's share: OMP_NUM_THREADS where the host sets
+    it (the CPU share of one GPU on a shared GPU host), else every CPU in
+    this process's affinity mask."""
+    visible = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(env, visible) if env > 0 else visible), visible
+
+
 def cpu_baseline(prob, seconds=15.0):
-    """The FP32 build of the CPU oracle (a port of the reference step), on this
-    host's cores, over a bounded number of steps of the same grid."""
+    """The FP32 build of the CPU oracle (a port of the reference step) on this
+    host's cores, built for this host, over a bounded number of steps of the
+    same grid; and the same on one core."""
     from oracle import pyoracle as po
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16))
+    lib, flags = native_oracle()
+    threads, visible = cpu_threads()
     op = po.Problem(prob.nx, prob.ny, prob.dx, prob.dy, prob.beta, prob.delta, prob.cfl_max,
                     prob.cfl_wish, prob.dt0, prob.T0, bc=prob.bc, cfl_mode=prob.cfl_mode,
                     form=po.FORM_RHO1, omega=prob.omega, edge_amp=prob.edge_amp,
@@ -94,13 +155,13 @@ def cpu_baseline(prob, seconds=15.0):
     q = prob.q0.astype(np.float32)
     # warm up (threads, first touch), calibrate on a few more steps, then time
     # one bounded run of the same grid
-    q, st, rc = po.run(op, q, 3, dtype=np.float32, fast=True)
+    q, st, rc = po.run(op, q, 3, dtype=np.float32, fast=lib)
     t0 = time.perf_counter()
-    q, st, rc = po.run(op, q, 5, dtype=np.float32, state=st, fast=True)
+    q, st, rc = po.run(op, q, 5, dtype=np.float32, state=st, fast=lib)
     per_step = max((time.perf_counter() - t0) / 5, 1e-4)
     steps = max(5, int(seconds / per_step))
     t0 = time.perf_counter()
-    q, st, rc = po.run(op, q, steps, dtype=np.float32, state=st, fast=True)
+    q, st, rc = po.run(op, q, steps, dtype=np.float32, state=st, fast=lib)
     el = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError(f"CPU oracle failed: {rc}")
@@ -108,18 +169,19 @@ def cpu_baseline(prob, seconds=15.0):
     # the same on one core, over a shorter sample
     op.nthreads = 1
     t0 = time.perf_counter()
-    q, st, rc = po.run(op, q, 2, dtype=np.float32, state=st, fast=True)
+    q, st, rc = po.run(op, q, 2, dtype=np.float32, state=st, fast=lib)
     per_step1 = max((time.perf_counter() - t0) / 2, 1e-4)
     steps1 = max(2, int(seconds * 0.4 / per_step1))
     t0 = time.perf_counter()
-    q, st, rc1 = po.run(op, q, steps1, dtype=np.float32, state=st, fast=True)
+    q, st, rc1 = po.run(op, q, steps1, dtype=np.float32, state=st, fast=lib)
     el1 = time.perf_counter() - t0
     if rc1 != 0:
         raise RuntimeError(f"CPU oracle failed: {rc1}")
     return {"value": value, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "cpus_visible": visible, "build": "gcc " + flags,
             "sample": f"{steps} accepted steps of the same {prob.nx}x{prob.ny} grid from the "
-                      f"quiescent start, FP32 CPU oracle (-O3, OpenMP over rows), {el:.1f} s; "
-                      f"1 core: {steps1} further steps, {el1:.1f} s",
+                      f"quiescent start, FP32 CPU oracle (OpenMP over rows, {threads} threads), "
+                      f"{el:.1f} s; 1 core: {steps1} further steps, {el1:.1f} s",
             "value_1core": prob.nx * prob.ny * steps1 / el1 / 1e9}
 
 
@@ -154,9 +216,7 @@ def bench_batched(args, hip):
     ctx.set_stream(stream.value)
     rows = ctx.autotune(30)
     split = ctx.info(0)
-    ctx.enqueue(args.warmup)
-    if ctx.sync() != 0:
-        raise RuntimeError("warmup failed")
+    warm_up(ctx, args.warmup, args.steps)
     before = [ctx.info(k) for k in range(len(probs))]
     e0, e1 = hip.event(), hip.event()
     hip.record(e0, stream)
@@ -321,11 +381,17 @@ def bench_adjoint(args, hip):
     }
 
 
+ADJ_BYTES_PER_CELL_STEP = 84   # forward 24 + recompute 24 + adjoint 36 (read q, lambda; write lambda)
+
+
 def bench_batched_adjoint(args, hip):
     """32 instances of 1024x1024 (own edge profiles, wavelengths and focal
-    boxes): batched forward+adjoint (one launch per pass for all instances,
-    graph-replayed forward) against 32 sequential plain-form solves (one
-    instance at a time, one launch per forward step with a host readback of dt)."""
+    boxes): the batched forward+adjoint (one launch per pass for all
+    instances, graph-replayed forward) against 32 sequential plain-form
+    solves (one instance at a time, one launch per forward step with a host
+    read of dt).  The plain time is split by timing each solve again with the
+    graph-replayed forward: the difference is the per-step launch and
+    synchronisation cost, the rest is kernel time of one instance at a time."""
     import fivonagi_amd as fa
     from fivonagi_amd import workloads as wl
     probs, boxes = wl.adjoint_batch(32)
@@ -335,17 +401,24 @@ def bench_batched_adjoint(args, hip):
     J, ga, gp = ctx.step_adjoint(nsteps, boxes)
     tb = time.perf_counter() - t0
     ctx.close()
-    n_plain = args.plain_instances
-    t0 = time.perf_counter()
-    Jp = []
-    for k in range(n_plain):
-        c = fa.Context(probs[k])
-        jk, _, _ = c.step_adjoint(nsteps, boxes[k], plain_forward=True)
-        Jp.append(jk[0])
-        c.close()
-        print(f"plain solve {k + 1}/{n_plain} done", file=sys.stderr, flush=True)
-    tp = (time.perf_counter() - t0) * len(probs) / n_plain
+    n_plain = min(args.plain_instances, len(probs))
+
+    def solves(plain):
+        t0 = time.perf_counter()
+        out = []
+        for k in range(n_plain):
+            c = fa.Context(probs[k])
+            jk, _, _ = c.step_adjoint(nsteps, boxes[k], plain_forward=plain)
+            out.append(jk[0])
+            c.close()
+            print(f"{'plain' if plain else 'graph-forward'} solve {k + 1}/{n_plain} done",
+                  file=sys.stderr, flush=True)
+        return (time.perf_counter() - t0) * len(probs) / n_plain, out
+    tp, Jp = solves(True)
+    tg, _ = solves(False)
     rel = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(J[:n_plain], Jp))
+    cells = sum(p.nx * p.ny for p in probs)
+    achieved = ADJ_BYTES_PER_CELL_STEP * cells * nsteps / tb / 1e9
     return {
         "metric": METRIC, "value": round(len(probs) / tb, 4), "unit": "gradients/s",
         "n_gpus": 1, "steps": nsteps, "warmup": 0, "ms_per_step": round(tb * 1e3 / nsteps, 4),
@@ -356,8 +429,16 @@ def bench_batched_adjoint(args, hip):
                    "plain_seconds_for_32": round(tp, 3),
                    "plain_instances_timed": n_plain,
                    "speedup_vs_plain": round(tp / tb, 3),
+                   "plain_split": {
+                       "graph_forward_seconds_for_32": round(tg, 3),
+                       "per_step_launch_and_sync_seconds": round(tp - tg, 3),
+                       "speedup_from_batching_alone": round(tg / tb, 3)},
                    "J_rel_diff_batched_vs_plain": rel},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "forward + recompute + adjoint march kernels",
+                     "bytes_per_cell_step": ADJ_BYTES_PER_CELL_STEP},
+        "cpu_baseline": None,
     }
 
 
@@ -417,10 +498,7 @@ def main():
     ctx.set_stream(stream.value)
     info0 = ctx.info(0)
 
-    ctx.enqueue(args.warmup)
-    rc = ctx.sync()
-    if rc != 0:
-        raise RuntimeError(f"warmup failed: {rc}")
+    warm_up(ctx, args.warmup, args.steps)
     before = ctx.info(0)
     e0, e1 = hip.event(), hip.event()
     hip.record(e0, stream)

@@ -56,7 +56,7 @@ class Info(C.Structure):
         ("n_accepted", C.c_int64), ("n_rejected", C.c_int64), ("n_attempts", C.c_int64),
         ("error", C.c_int32), ("cur", C.c_int32),
         ("n_items", C.c_int32), ("rows_per_wave", C.c_int32),
-        ("skew", C.c_double), ("rotate", C.c_int32), ("pad", C.c_int32),
+        ("skew", C.c_double), ("rotate", C.c_int32), ("phase", C.c_int32),
         ("md_x", C.c_int32), ("md_y", C.c_int32),
         ("fp64", C.c_int32), ("pad2", C.c_int32),
     ]

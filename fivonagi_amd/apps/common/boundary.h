@@ -1,93 +1,110 @@
-// apps/common/boundary.h -- ghost-cell rules as device code.  The fused step
-// kernel evaluates the boundary while it loads a row of the input state, so
-// the ghost ring costs no separate pass.
+// apps/common/boundary.h -- the boundary plug-in interface of the fused step
+// and the default app's rule, zero-gradient ghost cells
+// (common/boundary.h:11-82 of the reference).
 //
-// Rules restated (reads come from the input state only, like the reference,
-// which reads the other texture -- so corner ghosts lag one step):
-//   zero-gradient ..... common/boundary.h:27-81; later rules win:
-//                       right (i >= nx-2) -> (nx-3, j), bottom (j <= 1) -> (i, 2),
-//                       top (j >= ny-2) -> (i, ny-3), left (i <= 1) -> (2, j)
-//   edge drive ........ apps/hifu-beam/boundary.h:77-93 with nu = (1, 0):
-//                       left ghost columns hold p, p*(p+1), 0 with
-//                       p = amp_j * sin(omega*T + phase_j); other edges zero-gradient
-//   Taylor shock ...... apps/taylor-angle-xy/boundary.h:38-55: every ghost cell
-//                       holds the closed form rho' = rhoa*tanh(c*(x cos + y sin - T))
-//   HIFU app .......... apps/hifu-beam/boundary.h:20-133: focused packet on the
-//                       aperture rows of the left ghost columns, mirrored beam
-//                       axis (rows 0, 1 <- 4, 3 with rho v negated), zero corners
+// An app's boundary.h defines one policy: a struct with the members below,
+// which the step kernel calls while it loads a row of the input state (so the
+// ghost ring costs no pass of its own).  Rules read the input state only, as
+// the reference reads the other texture, so corner ghosts lag one step.
+//
+//   kId            the instance's boundary kind (FIVONAGI_BC_*)
+//   kRowData       the policy reads per-row tables (RowData) with each row
+//   kPrepass       ghost cells written before the step by prepass() (their
+//                  values depend on T and the cell only); the step copies them
+//   kExtOnly       the app always runs with the extended step (its instances
+//                  set it), so the plain step need not carry its march
+//   lane(ic, nx, m)                  once per lane: the source columns
+//   offset(rc, ny, m, row_bytes)     per row: byte offset of the source cell
+//   load_row(tb, rc, h)              per row: the row's table entries
+//   patch(c, rc, q, h)               per row: values that are not copies
+//
+// A new app adds a directory with its boundary.h (and parameters.h, init.h),
+// and one line to apps/apps.h.
 #ifndef FIVONAGI_APPS_COMMON_BOUNDARY_H
 #define FIVONAGI_APPS_COMMON_BOUNDARY_H
 
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
+#include "../../csrc/fv_types.h"
 #include "../na/solver.h"
 
 namespace fvbc {
 
-struct Source {       // where a ghost cell's value comes from
-  int ii, jj;         // cell of the input state to copy
-  bool drive;         // left-edge drive value instead of a copy
-  bool closed_form;   // Taylor closed form instead of a copy
+// Moving window (common/main.cu:109-135, :167-176): once T >= MDT the window
+// sits int((T - MDT) * MDV / d) cells from its start; after an accepted step
+// ending at T it moves by the part not applied yet (md = the current offset).
+__device__ __forceinline__ int md_shift(double T, double md_t, double v, double d, int md) {
+  return T >= md_t ? (int)((T - md_t) * v / d) - md : 0;
+}
+
+// A lane's source columns: bA on rows 2..ny-3, bB on the ghost rows (bytes).
+struct LaneMap {
+  uint32_t bA, bB;
+  bool left;         // the lane's column is a left ghost column (i <= 1)
+  int ic;            // the lane's column, clamped to the grid
 };
 
-__device__ __forceinline__ bool is_ghost(int i, int j, int nx, int ny) {
-  return i <= 1 || j <= 1 || i >= nx - 2 || j >= ny - 2;
-}
+// The instance's per-row tables (device pointers, the context's precision):
+// edge_c, edge_s = ampc, amps per row; tab = 4 entries per row (hifu).
+struct Tables {
+  const void* edge_c;
+  const void* edge_s;
+  const void* tab;
+};
 
-// kind: 0 zero-gradient, 1 edge drive, 2 Taylor (fv::BoundaryKind)
-__device__ __forceinline__ Source map(int kind, int i, int j, int nx, int ny) {
-  Source s{i, j, false, false};
-  if (kind == 2) {
-    s.closed_form = is_ghost(i, j, nx, ny);
-  } else if (i <= 1) {
-    if (kind == 1) s.drive = true; else s.ii = 2;
-  } else if (j >= ny - 2) {
-    s.jj = ny - 3;
-  } else if (j <= 1) {
-    s.jj = 2;
-  } else if (i >= nx - 2) {
-    s.ii = nx - 3;
+// Per-row table entries: edge drive ampc, amps (amp_j cos/sin phase_j, or the
+// aperture's AMPL cos/sin(ISPW phi_j)); aperture phi_j, direction, on/off.
+template <typename R>
+struct RowData {
+  R ec, es, ph, nux, nuy, on;
+};
+
+// Values uniform over a wave of one attempt, for patch().
+template <typename R>
+struct PatchCtx {
+  R sin_t, cos_t;    // sin/cos(omega T) of the attempt
+  double T;          // T of the attempt
+  R hk;              // hifu: FREC/4
+  int mdx;           // moving-window offset MDX
+  int x0;            // first output column of the wave's strip
+  int nx, ny;
+  int ic;            // the lane's clamped column
+  bool left;         // the lane's column is a left ghost column
+};
+
+template <typename R>
+struct Cell {
+  R q1, q2, q3;      // rho', rho u, rho v
+};
+
+// common/boundary.h:27-81: later rules win --
+// right (i >= nx-2) -> (nx-3, j), bottom (j <= 1) -> (i, 2),
+// top (j >= ny-2) -> (i, ny-3), left (i <= 1) -> (2, j).
+struct ZeroGradient {
+  static constexpr int kId = fv::kBcZeroGrad;
+  static constexpr bool kRowData = false;
+  static constexpr bool kPrepass = false;
+  static constexpr bool kExtOnly = false;
+  __device__ static void lane(int ic, int nx, LaneMap& m, uint32_t size) {
+    m.ic = ic;
+    m.left = ic <= 1;
+    const int colA = m.left ? 2 : (ic >= nx - 2 ? nx - 3 : ic);
+    const int colB = m.left ? 2 : ic;   // ghost rows: the left rule keeps its own row
+    m.bA = size * (uint32_t)colA;
+    m.bB = size * (uint32_t)colB;
   }
-  return s;
-}
-
-// Left-edge drive: p = amp*sin(theta + phase) = ampc*sin(theta) + amps*cos(theta)
-// with ampc = amp*cos(phase), amps = amp*sin(phase) precomputed per row and
-// sin/cos(theta = omega*T) computed once per step.
-template <typename R>
-__device__ __forceinline__ void drive(R ampc, R amps, R sin_t, R cos_t, R& q1, R& q2, R& q3) {
-  const R p = fvna::fma_(ampc, sin_t, amps * cos_t);
-  q1 = p;
-  q2 = p * (p + R(1));
-  q3 = R(0);
-}
-
-// HIFU focused packet (apps/hifu-beam/boundary.h:10-17, 84-93):
-// pack = AMPL sin(ISPW (T + phi)) exp(-opT^10), opT = FREC (T + phi)/4, with
-// AMPL sin(ISPW (T + phi)) = ampc sin(ISPW T) + amps cos(ISPW T) from the
-// per-row ampc = AMPL cos(ISPW phi), amps = AMPL sin(ISPW phi);
-// u1 = pack, (u2, u3) = nu pack (pack + 1).
-template <typename R>
-__device__ __forceinline__ void hifu_packet(R ampc, R amps, R sin_t, R cos_t, R opT, R nux, R nuy,
-                                            R& q1, R& q2, R& q3) {
-  const R o2 = opT * opT, o4 = o2 * o2, o8 = o4 * o4;
-  const R p = fvna::fma_(ampc, sin_t, amps * cos_t) * exp(-(o8 * o2));
-  const R pp = p * (p + R(1));
-  q1 = p;
-  q2 = nux * pp;
-  q3 = nuy * pp;
-}
-
-template <typename R>
-__device__ __forceinline__ void taylor(double rhoa, double c, double cs, double sn,
-                                       double yoff, int jc, double dx, double dy,
-                                       double T, int i, int j, R& q1, R& q2, R& q3) {
-  const double arg = c * ((double)i * dx * cs + ((double)(j - jc) * dy + yoff) * sn - T);
-  const double r = rhoa * tanh(arg);
-  q1 = (R)r;
-  q2 = (R)(r * cs);
-  q3 = (R)(r * sn);
-}
+  __device__ static uint32_t offset(int rc, int ny, const LaneMap& m, uint32_t row_bytes) {
+    const bool grow = (rc <= 1) || (rc >= ny - 2);
+    const int jm = rc >= ny - 2 ? ny - 3 : (rc <= 1 ? 2 : rc);
+    const uint32_t rowL = (uint32_t)rc * row_bytes, rowM = (uint32_t)jm * row_bytes;
+    return grow ? (m.left ? rowL : rowM) + m.bB : rowM + m.bA;
+  }
+  template <typename R>
+  __device__ static void load_row(const Tables&, int, RowData<R>&) {}
+  template <typename R>
+  __device__ static void patch(const PatchCtx<R>&, int, Cell<R>&, const RowData<R>&) {}
+};
 
 }  // namespace fvbc
 

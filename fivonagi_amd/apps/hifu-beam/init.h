@@ -1,0 +1,18 @@
+/* apps/hifu-beam/init.h -- the hifu-beam app's initial state
+ * (apps/hifu-beam/init.h:10-23 of the reference): a quiescent medium.  Host
+ * code (C99); SoA grids of nx*ny, offset i + j*nx. */
+#ifndef FIVONAGI_APPS_HIFU_BEAM_INIT_H
+#define FIVONAGI_APPS_HIFU_BEAM_INIT_H
+
+#include <string.h>
+
+#include "../../../include/fivonagi.h"
+
+static inline void fivonagi_hifu_init(const fivonagi_params *p, float *q1, float *q2, float *q3) {
+  const size_t n = (size_t)p->nx * (size_t)p->ny;
+  memset(q1, 0, sizeof(float) * n);
+  memset(q2, 0, sizeof(float) * n);
+  memset(q3, 0, sizeof(float) * n);
+}
+
+#endif /* FIVONAGI_APPS_HIFU_BEAM_INIT_H */

@@ -45,6 +45,18 @@ def build_lib(verbose=False):
     return LIB
 
 
+DRIVER_SRC = os.path.join(HERE, "apps", "hifu-beam", "driver.c")
+DRIVER = os.path.join(HERE, "apps", "hifu-beam", "driver")
+
+
+def build_driver():
+    """The hifu-beam app's C host program, linked against libfivonagi.so."""
+    _run(["gcc", "-std=c99", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+          "-L", HERE, "-lfivonagi", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../..", "-lm",
+          "-o", DRIVER])
+    return DRIVER
+
+
 def build_oracle():
     """The CPU oracle is test infrastructure; building it is not using it."""
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
@@ -52,6 +64,7 @@ def build_oracle():
 
 def build_all(verbose=False):
     build_lib(verbose)
+    build_driver()
     build_oracle()
 
 

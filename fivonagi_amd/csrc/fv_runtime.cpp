@@ -23,8 +23,8 @@
 
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
-                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int ext,
-                                     int fp64, hipStream_t stream);
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int bc0,
+                                     int ext, int fp64, hipStream_t stream);
 extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, int phase,
                                        hipStream_t stream);
 extern "C" hipError_t fv_launch_collect(const fv::InstDesc* insts, int n_inst, int max_nx,
@@ -116,12 +116,14 @@ struct fivonagi_ctx {
   fv::Ctl* d_ctl = nullptr;                // 2 control blocks per instance
   fv::SyncSlots* d_sync = nullptr;         // 3 reduction slot sets per instance
   int64_t attempts = 0;                    // step attempts launched (the phase is this mod 6)
-  int graph_len = 0;
+  // captured step launches: graphs of kGraphLong and kGraphShort attempts
+  // (multiples of 6, so a graph starting at phase 0 ends at phase 0)
+  static constexpr int kGraphLong = 30, kGraphShort = 6;
   int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
   int collect_nx = 0;                      // widest instance with data collection (0: none)
   bool unlimited = false;                  // targets set to "run every attempt"
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};          // long, short
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   std::vector<float*> adj_lam;             // dJ/dq0 of the last adjoint call (per instance)
   struct CollectBufs { void* measure = nullptr; void* stage = nullptr; };
   std::vector<CollectBufs> collect_bufs;   // per instance, kept once allocated
@@ -138,11 +140,12 @@ struct fivonagi_ctx {
     return p;
   }
   void drop_graph() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    graph_exec = nullptr;
-    graph = nullptr;
-    graph_len = 0;
+    for (int k = 0; k < 2; ++k) {
+      if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
+      if (graph[k]) (void)hipGraphDestroy(graph[k]);
+      graph_exec[k] = nullptr;
+      graph[k] = nullptr;
+    }
   }
   ~fivonagi_ctx() {
     drop_graph();
@@ -283,7 +286,8 @@ void build_items(fivonagi_ctx* c, int auto_h, float skew, bool rotate) {
         w.y1 = bounds[ch + 1];
         w.mrow = (int)c->rowmask.size();
         w.local = n;
-        w.pad[0] = w.pad[1] = 0;
+        w.bc = p.bc;
+        w.pad = 0;
         for (int j = w.y0; j < w.y1; ++j) {
           uint64_t m = 0;
           for (int l = fv::kHalo; l < fv::kLanes - fv::kHalo; ++l) {
@@ -390,8 +394,8 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
     if (e != hipSuccess) return e;
   }
   hipError_t e = fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(),
-                                phase, c->n_inst, c->d_ctl, c->d_sync, c->ext_any ? 1 : 0,
-                                c->fp64 ? 1 : 0, c->stream);
+                                phase, c->n_inst, c->d_ctl, c->d_sync, c->params[0].bc,
+                                c->ext_any ? 1 : 0, c->fp64 ? 1 : 0, c->stream);
   if (e != hipSuccess || c->collect_nx == 0) return e;
   return fv_launch_collect(c->d_desc, c->n_inst, c->collect_nx, phase, c->fp64 ? 1 : 0,
                            c->stream);
@@ -410,35 +414,43 @@ int collect_status(fivonagi_ctx* c) {
   return 0;
 }
 
+// Capture G attempts (phases 0..G-1 mod 6) into graph slot k.
+int capture_graph(fivonagi_ctx* c, int k, int G) {
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int a = 0; a < G; ++a) {
+    hipError_t e = launch_phase(c, a % 6);
+    if (e != hipSuccess) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(e, "fv_launch_step (capture)");
+    }
+  }
+  HIP_TRY(hipStreamEndCapture(c->stream, &c->graph[k]));
+  HIP_TRY(hipGraphInstantiate(&c->graph_exec[k], c->graph[k], nullptr, nullptr, 0));
+  return 0;
+}
+
+// n attempts: single launches up to a phase-0 boundary, then replays of the
+// long graph, then of the short one, then at most 5 single launches.
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
   c->ctl_cache_ok = false;
-  const int G = 30;   // attempts per captured graph (a multiple of 6: phases 0..5 repeat)
-  // single launches up to a phase-0 boundary, then whole graphs, then singles
-  while (n > 0 && (c->attempts % 6 != 0 || n < G)) {
+  while (n > 0 && c->attempts % 6 != 0) {
     hipError_t e = launch_phase(c, (int)(c->attempts % 6));
     if (e != hipSuccess) return hip_fail(e, "fv_launch_step");
     ++c->attempts;
     --n;
   }
-  if (n >= G && !c->graph_exec) {
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < G; ++k) {
-      hipError_t e = launch_phase(c, k % 6);
-      if (e != hipSuccess) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(c->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        return hip_fail(e, "fv_launch_step (capture)");
-      }
+  const int lens[2] = {fivonagi_ctx::kGraphLong, fivonagi_ctx::kGraphShort};
+  for (int k = 0; k < 2; ++k) {
+    const int G = lens[k];
+    if (n >= G && !c->graph_exec[k])
+      if (int rc = capture_graph(c, k, G)) return rc;
+    while (n >= G) {
+      HIP_TRY(hipGraphLaunch(c->graph_exec[k], c->stream));
+      c->attempts += G;
+      n -= G;
     }
-    HIP_TRY(hipStreamEndCapture(c->stream, &c->graph));
-    HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-    c->graph_len = G;
-  }
-  while (n >= G) {
-    HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
-    c->attempts += G;
-    n -= G;
   }
   while (n > 0) {
     hipError_t e = launch_phase(c, (int)(c->attempts % 6));
@@ -1144,6 +1156,7 @@ int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
   info->n_items = c->desc[inst].n_items;
   info->skew = c->skew_used;
   info->rotate = c->rotate_used ? 1 : 0;
+  info->phase = (int)(c->attempts % 6);
   info->md_x = t.mdx;
   info->md_y = t.mdy;
   info->fp64 = c->fp64 ? 1 : 0;
@@ -1489,8 +1502,8 @@ int fivonagi_step_noalloc(const fivonagi_params* p, float* const q[6], int64_t n
       const int phase = (int)(a % 6);
       if (h.taylor_ring > 0)
         HIP_TRY(fv_launch_taylor_ghosts(d_desc, 1, h.taylor_ring, phase, stream));
-      HIP_TRY(fv_launch_step(d_desc, d_items, d_mask, n_items, phase, 1, d.ctl, d.sync, ext, 0,
-                             stream));
+      HIP_TRY(fv_launch_step(d_desc, d_items, d_mask, n_items, phase, 1, d.ctl, d.sync, p->bc,
+                             ext, 0, stream));
     }
     // the control block after the last attempt (fv_settle_kernel), read back
     const int phase = (int)(a % 6);

@@ -32,8 +32,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "../apps/common/boundary.h"
+#include <type_traits>
+
+#include "../apps/apps.h"
+#include "../apps/hifu-beam/data_collect.h"
+#include "../apps/taylor-angle-xy/data_collect.h"
 #include "../apps/na/solver.h"
+#include "../apps/na/source.h"
 #include "fv_types.h"
 
 namespace fv {
@@ -250,12 +255,7 @@ __device__ __forceinline__ int next_cur(const Fold<R>& f) {
   return f.state == kFoldAccept ? (f.p.cur ^ 1) : f.p.cur;
 }
 
-// Moving window (common/main.cu:109-135, :167-176): once T >= MDT the window
-// sits int((T - MDT) * MDV / d) cells from its start; after an accepted step
-// ending at T it moves by the part not applied yet (md = the current offset).
-__device__ __forceinline__ int md_shift(double T, double md_t, double v, double d, int md) {
-  return T >= md_t ? (int)((T - md_t) * v / d) - md : 0;
-}
+using fvbc::md_shift;
 
 // Control block of attempt a+1 from attempt a's fold: common/main.cu:199-230
 // with a clean retry on rejection.  log: record dt and T of an accepted step
@@ -368,6 +368,8 @@ struct XY {          // one cell of q_xy, with its velocities for the source
   R q1, q2, q3, u, v;
 };
 
+// One work item of attempt `phase` (mod 6) of an instance whose ghost cells
+// follow the boundary policy BC (apps/<app>/boundary.h).
 // kOne: the launch covers a single instance, whose control block and
 // reduction slots of this attempt are passed directly (ctl_in = its control
 // block of attempt a = ctl + a % 2, sync_in = the slots of attempt a-1 =
@@ -380,28 +382,14 @@ struct XY {          // one cell of q_xy, with its velocities for the source
 //    (i + sx, j + sy) is off the grid (moveDomain's else branch); every cell
 //    of the output is written exactly once.  A rejected attempt's output is
 //    discarded, as without the window.
-//  - HIFU boundary (apps/hifu-beam/boundary.h:20-133): rows 0, 1 read rows
-//    4, 3 with rho v negated, the 2x2 corners are 0, and the aperture rows of
-//    the left ghost columns take the focused packet while MDX == 0.
+//  - the HIFU app's packet and data collection staging.
 // R: float (FP32 path) or double (FP64 instances: state, tables, arithmetic
 // and the cfl reduction in double; the reference's PRECISION=2).
-template <typename R, bool kOne, bool kExt>
-__global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
-                                                     const WorkItem* __restrict__ items,
-                                                     const uint64_t* __restrict__ rowmask,
-                                                     int n_items, int phase,
-                                                     const Ctl* __restrict__ ctl_in,
-                                                     const SyncSlots* __restrict__ sync_in) {
-  // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
-  // run of work items so vertically adjacent chunks of a strip (which share
-  // halo rows) hit the same L2.
-  int item;
-  {
-    const int b = blockIdx.x;
-    const int q = n_items >> 3, rem = n_items & 7, x = b & 7, k = b >> 3;
-    item = x * q + min(x, rem) + k;
-  }
-  const WorkItem it = items[item];
+template <typename R, bool kOne, bool kExt, class BC>
+__device__ __forceinline__ void step_impl(const InstDesc* __restrict__ insts, const WorkItem& it,
+                                          const uint64_t* __restrict__ rowmask, int phase,
+                                          const Ctl* __restrict__ ctl_in,
+                                          const SyncSlots* __restrict__ sync_in) {
   const InstDesc& D = insts[kOne ? 0 : it.inst];
   const int local = it.local;
   const int lane = threadIdx.x;
@@ -413,10 +401,8 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // the fold's loads; the asm barrier keeps the compiler from re-loading a
   // pointer by the buffer index once accept/reject is known
   const int nx = D.nx, ny = D.ny;
-  const int bc = D.bc;
   typedef typename Mem<R>::g gR;
   typedef typename Mem<R>::cg cgR;
-  typedef typename Mem<R>::k kR;
   R* qb[2][3];
 #pragma unroll
   for (int h = 0; h < 2; ++h)
@@ -424,15 +410,15 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     for (int f = 0; f < 3; ++f) qb[h][f] = (R*)D.q[h][f];
   double cfl_max = D.cfl_max;
   const double* dt_seq = D.dt_seq;
-  const R* ec_p = (const R*)D.edge_c;
-  const R* es_p = (const R*)D.edge_s;
+  fvbc::Tables tb{D.edge_c, D.edge_s, D.hifu_tab};
   const FoldRaw<R> FR = fold_load<R>(kOne ? ctl_in : D.ctl + (phase & 1),
                                      kOne ? sync_in : D.sync + (phase + 2) % 3);
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int f = 0; f < 3; ++f) asm volatile("" : "+s"(qb[h][f]));
-  asm volatile("" : "+s"(cfl_max), "+s"(dt_seq), "+s"(ec_p), "+s"(es_p));
+  asm volatile("" : "+s"(cfl_max), "+s"(dt_seq));
+  if (BC::kRowData) asm volatile("" : "+s"(tb.edge_c), "+s"(tb.edge_s));
   const Fold<R> F = fold_decide(FR, cfl_max, dt_seq != nullptr);
   const int cur = next_cur(F);
 
@@ -442,8 +428,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   gR* __restrict__ o1 = (gR*)(cur ? qb[0][0] : qb[1][0]);
   gR* __restrict__ o2 = (gR*)(cur ? qb[0][1] : qb[1][1]);
   gR* __restrict__ o3 = (gR*)(cur ? qb[0][2] : qb[1][2]);
-  kR* __restrict__ edge_c = (kR*)ec_p;
-  kR* __restrict__ edge_s = (kR*)es_p;
 
   const int i = it.x0 - kHalo + lane;
   const bool col_in = (i >= 0) && (i < nx);
@@ -455,55 +439,28 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   // CFL lane masks of the item's rows: cm[j], j in [y0, y1)
   const uint64_t* __restrict__ cm = rowmask + (it.mrow - y0);
 
-  // Ghost-cell source columns (apps/common/boundary.h): colA on rows
-  // 2..ny-3, colB on ghost rows (top/bottom rules beat the right rule; the
-  // left rule beats everything and keeps its own row).  Taylor ghost cells
-  // were written into the input by fv_taylor_ghost_kernel: plain copies here.
-  const bool taylor = (bc == kBcTaylor);
-  const bool left = !taylor && (ic <= 1);
-  const int colA = taylor ? ic : (left ? 2 : (ic >= nx - 2 ? nx - 3 : ic));
-  const int colB = taylor ? ic : (left ? 2 : ic);
+  // the lane's ghost-cell source columns (the app's boundary policy; Taylor
+  // ghost cells were written into the input by fv_taylor_ghost_kernel and are
+  // plain copies here)
   constexpr uint32_t kSz = sizeof(R);
-  const uint32_t bA = kSz * (uint32_t)colA, bB = kSz * (uint32_t)colB;
+  fvbc::LaneMap lm;
+  BC::lane(ic, nx, lm, kSz);
   const uint32_t row_bytes = kSz * (uint32_t)nx;
-  const bool hifu = kExt && bc == kBcHifu;
-  kR* __restrict__ htab = (kR*)D.hifu_tab;
-  struct HRow { R ph, nux, nuy, on; };       // aperture row data (hifu)
 
-  // raw (unpatched) load of row r, with the row's drive coefficients; the
+  // raw (unpatched) load of row r, with the row's table entries; the
   // boundary patch is applied when the row is consumed one iteration later,
   // so the load latency overlaps a full step
-  auto load_raw = [&](int r, R& a, R& b, R& c, R& ec, R& es, HRow& h) {
+  auto load_raw = [&](int r, R& a, R& b, R& c, fvbc::RowData<R>& h) {
     const int rc = min(max(r, 0), ny - 1);
-    const bool grow = (rc <= 1) || (rc >= ny - 2);
-    uint32_t o;
-    if (hifu) {
-      // bottom rows mirror rows 4, 3; top rows copy ny-3; columns as colA
-      const int jm = rc <= 1 ? 4 - rc : (rc >= ny - 2 ? ny - 3 : rc);
-      o = (uint32_t)jm * row_bytes + bA;
-    } else {
-      const int jm = taylor ? rc : (rc >= ny - 2 ? ny - 3 : (rc <= 1 ? 2 : rc));
-      const uint32_t rowL = (uint32_t)rc * row_bytes, rowM = (uint32_t)jm * row_bytes;
-      if (grow) o = (left ? rowL : rowM) + bB;   // ghost rows
-      else o = rowM + bA;                        // rows 2..ny-3: rc == jm
-    }
+    const uint32_t o = BC::offset(rc, ny, lm, row_bytes);
     a = ldg(in1, o);
     b = ldg(in2, o);
     c = ldg(in3, o);
-    if (bc == kBcEdgeDrive || hifu) {
-      ec = edge_c[rc];
-      es = edge_s[rc];
-    }
-    if (hifu) {
-      h.ph = htab[4 * rc];
-      h.nux = htab[4 * rc + 1];
-      h.nuy = htab[4 * rc + 2];
-      h.on = htab[4 * rc + 3];
-    }
+    if (BC::kRowData) BC::template load_row<R>(tb, rc, h);
   };
-  R n1, n2, n3, nec = R(0), nes = R(0);
-  HRow nh{R(0), R(0), R(0), R(0)};
-  load_raw(it.y0 - kHalo, n1, n2, n3, nec, nes, nh);   // in flight during advance()
+  R n1, n2, n3;
+  fvbc::RowData<R> nh{R(0), R(0), R(0), R(0), R(0), R(0)};
+  load_raw(it.y0 - kHalo, n1, n2, n3, nh);   // in flight during advance()
 
   // ---- control block of this attempt ----
   Ctl P = advance<R>(D, F, local == 0);
@@ -533,11 +490,6 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     sin_t = unif(R(P.drive_s)); cos_t = unif(R(P.drive_c));
     hbeta = R(D.hbeta_d);
   }
-  // the edge source runs while the window has not moved in x
-  // (apps/hifu-beam/boundary.h:82); after that the left ghost columns are
-  // plain zero-gradient copies
-  const bool drive = (bc == kBcEdgeDrive) && (!kExt || P.mdx == 0);
-  const bool drive_lane = left && drive;
   // shift of the window after this attempt, if it is accepted
   int sx = 0, sy = 0;
   if (kExt && D.md_on) {
@@ -548,35 +500,25 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   typename Mem<R>::g* __restrict__ stage = (typename Mem<R>::g*)D.stage;
   const bool stage_on = kExt && stage != nullptr;
   const int jcut = D.jcut;
-  // HIFU: the packet runs while the window has not moved (boundary.h:82)
-  const bool hdrive = hifu && P.mdx == 0 && it.x0 == 0;
-  const double Td = P.T;
-  const R hk = sizeof(R) == 4 ? R(D.hifu_k) : R(D.hifu_k_d);
-  auto patch = [&](Row<R>& q, R ec, R es, const HRow& h, int r) {
-    if (drive) {
-      R a, b, c;
-      fvbc::drive(ec, es, sin_t, cos_t, a, b, c);
-      q.q1 = drive_lane ? a : q.q1;
-      q.q2 = drive_lane ? b : q.q2;
-      q.q3 = drive_lane ? c : q.q3;
-    }
-    if (hifu) {
-      const int rc = min(max(r, 0), ny - 1);
-      q.q3 = rc <= 1 ? -q.q3 : q.q3;          // reflective beam axis
-      if (hdrive && h.on != R(0) && rc >= 2 && rc <= ny - 3) {
-        R a, b, c;
-        // T + phi in FP64: the envelope's 10th power magnifies its rounding
-        fvbc::hifu_packet(ec, es, sin_t, cos_t, hk * R(Td + (double)h.ph), h.nux, h.nuy,
-                          a, b, c);
-        q.q1 = left ? a : q.q1;
-        q.q2 = left ? b : q.q2;
-        q.q3 = left ? c : q.q3;
-      }
-      const bool corner = (rc <= 1 || rc >= ny - 2) && (ic <= 1 || ic >= nx - 2);
-      q.q1 = corner ? R(0) : q.q1;
-      q.q2 = corner ? R(0) : q.q2;
-      q.q3 = corner ? R(0) : q.q3;
-    }
+  // what the boundary policy sees of this attempt; the moving window's MDX
+  // only exists in the extended instantiation (else it is 0)
+  fvbc::PatchCtx<R> pc;
+  pc.sin_t = sin_t;
+  pc.cos_t = cos_t;
+  pc.T = P.T;
+  pc.hk = sizeof(R) == 4 ? R(D.hifu_k) : R(D.hifu_k_d);
+  pc.mdx = kExt ? P.mdx : 0;
+  pc.x0 = it.x0;
+  pc.nx = nx;
+  pc.ny = ny;
+  pc.ic = ic;
+  pc.left = lm.left;
+  auto patch = [&](Row<R>& q, const fvbc::RowData<R>& h, int r) {
+    fvbc::Cell<R> c{q.q1, q.q2, q.q3};
+    BC::patch(pc, min(max(r, 0), ny - 1), c, h);
+    q.q1 = c.q1;
+    q.q2 = c.q2;
+    q.q3 = c.q3;
     const R ir = fvna::rcp(R(1) + q.q1);
     q.un = q.q3 * ir;
     q.ut = q.q2 * ir;
@@ -588,18 +530,11 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     // u(i-1) + u(i+1): one cross-lane move, the other folds into the add
     const R uwe = shr1(mid.u) + shl1(mid.u);
     const R vwe = shr1(mid.v) + shl1(mid.v);
-    // the x and y increments are summed before they meet q: q + (dx + dy)
-    // rounds once at q's scale, where (q + dx) + dy rounds twice.  In FP32
-    // the two roundings land low on average (by 0.1 ulp of q per update over
-    // a 3000-step edge-drive run where they differ), which lowers the states
-    // at the shocks, hence the maximum wave speed, and the dt feedback turns
-    // that into a drift of T.  Same sum as source.h:86-104.
-    const R i2 = fvna::fma_(ddx2, fvna::fma_(R(-2), mid.u, uwe),
-                            ddy2 * (fvna::fma_(R(-2), mid.u, lo.u) + hi.u));
-    const R i3 = fvna::fma_(ddx2, fvna::fma_(R(-2), mid.v, vwe),
-                            ddy2 * (fvna::fma_(R(-2), mid.v, lo.v) + hi.v));
-    R r2 = fvna::add_nc(mid.q2, i2);
-    R r3 = fvna::add_nc(mid.q3, i3);
+    // apps/na/source.h: x and y increments summed, then added to q once
+    const R i2 = fvna::source_increment(ddx2, ddy2, mid.u, uwe, lo.u, hi.u);
+    const R i3 = fvna::source_increment(ddx2, ddy2, mid.v, vwe, lo.v, hi.v);
+    R r2 = fvna::apply_source(mid.q2, i2);
+    R r3 = fvna::apply_source(mid.q3, i3);
     r2 = s_int ? r2 : mid.q2;
     r3 = s_int ? r3 : mid.q3;
     pin(r2);
@@ -648,9 +583,9 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
     // registers from landing after this step's stores
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     bn.q1 = n1; bn.q2 = n2; bn.q3 = n3;
-    patch(bn, nec, nes, nh, r);
+    patch(bn, nh, r);
     if (r >= y0 + 4) source_store(r - 4, xn, xo, xm);
-    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3, nec, nes, nh);
+    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3, nh);
     if (r < y0 - 2) return;
     const int m = r - 2;
     // CFL lane masks of rows r (y interfaces) and m (x interfaces); the index
@@ -751,6 +686,48 @@ __global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict_
   CflSlot<R>::fold_in(sl->grp[local % kGroups], cflw);
 }
 
+// Marker for a step kernel that carries the march of every registered app
+// (apps/apps.h) and runs, per work item, that of its instance's boundary kind
+// (the item carries it, so the choice is made from the item alone): launches
+// over several instances.  A single-instance launch takes its app's policy
+// directly, a kernel with one march.
+struct AnyApp {};
+
+// The step kernel: one wave per work item.
+template <typename R, bool kOne, bool kExt, class BC>
+__global__ __launch_bounds__(64) void fv_step_kernel(const InstDesc* __restrict__ insts,
+                                                     const WorkItem* __restrict__ items,
+                                                     const uint64_t* __restrict__ rowmask,
+                                                     int n_items, int phase,
+                                                     const Ctl* __restrict__ ctl_in,
+                                                     const SyncSlots* __restrict__ sync_in) {
+  // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
+  // run of work items so vertically adjacent chunks of a strip (which share
+  // halo rows) hit the same L2.
+  int item;
+  {
+    const int b = blockIdx.x;
+    const int q = n_items >> 3, rem = n_items & 7, x = b & 7, k = b >> 3;
+    item = x * q + min(x, rem) + k;
+  }
+  const WorkItem it = items[item];
+  if constexpr (!std::is_same<BC, AnyApp>::value) {
+    step_impl<R, kOne, kExt, BC>(insts, it, rowmask, phase, ctl_in, sync_in);
+    return;
+  }
+  switch (uni(it.bc)) {
+#define FV_APP_CASE(P)                                                        \
+  case P::kId:                                                                \
+    if constexpr (kExt || !P::kExtOnly)                                       \
+      step_impl<R, kOne, kExt, P>(insts, it, rowmask, phase, ctl_in, sync_in); \
+    break;
+    FIVONAGI_APPS(FV_APP_CASE)
+#undef FV_APP_CASE
+    default:
+      break;
+  }
+}
+
 // Control block after the last launched attempt (phase = index of the next
 // attempt mod 6), stored where that attempt's launch will store it too; the
 // host reads it after a synchronisation.  One block per instance.
@@ -759,94 +736,6 @@ __global__ __launch_bounds__(64) void fv_settle_kernel(const InstDesc* __restric
   const InstDesc& D = insts[blockIdx.x];
   const Ctl P = current(D, phase, true);
   if (threadIdx.x == 0) D.ctl[(phase + 1) & 1] = P;
-}
-
-// Data collection of the hifu app (apps/hifu-beam/data_collect.h:51-106) for
-// one accepted attempt: row jcut of its new state, staged by the step before
-// the window shift, read with the pre-shift MDX.  Thread i = column i.
-template <typename R>
-__device__ void collect_hifu(const InstDesc& D, const Ctl& P, double T, int64_t nf,
-                             typename Mem<R>::g* __restrict__ m, int i) {
-  const int nx = D.nx, ny = D.ny;
-  if (i >= nx) return;
-  const R u1 = ((const R*)D.stage)[i];
-  const int mdx = P.mdx;
-  for (int k = 0; k < 4; ++k) {                     // data_collect.h:51-76
-    if (i + mdx == D.mic[k]) {
-      typename Mem<R>::g* row = m + (size_t)(2 + k) * nx;
-      if (nf == 0)
-        for (int ii = 0; ii < nx; ++ii) row[ii] = R(0);
-      row[nf] = u1;
-      break;
-    }
-  }
-  if (i == 0 && nf == 0)                            // microphones outside the window
-    for (int k = 0; k < 4; ++k)
-      if (D.mic[k] - mdx < 0 || D.mic[k] - mdx >= nx)
-        for (int ii = 0; ii < nx; ++ii) m[(size_t)(2 + k) * nx + ii] = R(0);
-  // central peak only (data_collect.h:100-106)
-  if (fabs((double)(i + mdx) / D.eta - T) > 0.5) return;
-  const int64_t o = (int64_t)i + mdx + 6 * (int64_t)nx;
-  if (o < (int64_t)nx * ny) m[o] = fmax_(u1, m[o]);
-}
-
-// Data collection of the taylor-angle-xy app (apps/taylor-angle-xy/
-// data_collect.h:43-96) for one accepted attempt: rho' of the new state along
-// the propagation direction through the front's centre, interpolated in y
-// between the two rows nearest to the line (entry iwrite of row 2), beside the
-// closed form at the nearer row (row 3).  The arithmetic follows the
-// reference's types: T, dx, dy and the constants in R, the line position ja
-// and the interpolation weights in float.  The state is read from the output
-// buffer, where the step has already applied the window shift (sx, sy) of this
-// attempt: the cell (i, j) of the unshifted frame is (i - sx, j - sy) there.
-// Thread tid = entry iwrite of the cut, 0 <= tid <= 2 cut_half.
-template <typename R>
-__device__ void collect_taylor(const InstDesc& D, const Ctl& P, double Td,
-                               typename Mem<R>::g* __restrict__ m, int tid) {
-  const int L = D.cut_half;
-  if (tid > 2 * L) return;
-  const int nx = D.nx, ny = D.ny;
-  const R T = R(Td), dx = R(D.dx), dy = R(D.dy);
-  const int icenter = (int)(T * R(D.md_vx) / dx) - P.mdx;                  // :43
-  const int jcenter = ny / 2 + (int)((T - R(D.md_t)) * R(D.md_vy) / dy) - P.mdy;  // :44-46
-  const int i = icenter - L + tid;                  // iwrite = tid (:87-89)
-  if (i < 0 || i > nx - 1) return;
-  const float ja = (float)fvna::fma_(R((float)(i - icenter)) * (dx / dy), R(D.tan_theta),
-                               R((float)jcenter));  // :52-55
-  int sx = 0, sy = 0;
-  if (D.md_on) {
-    sx = md_shift(Td, D.md_t, D.md_vx, D.dx, P.mdx);
-    sy = md_shift(Td, D.md_t, D.md_vy, D.dy, P.mdy);
-  }
-  const R* __restrict__ u = (const R*)D.q[P.cur ^ 1][0];
-  // tex2D reads clamp to the grid; the shifted frame starts at (sx, sy)
-  auto at = [&](int ii, int jj) -> R {
-    jj = min(max(jj, 0), ny - 1);
-    const int pi = max(ii - sx, 0), pj = max(jj - sy, 0);
-    return u[(size_t)pj * nx + pi];
-  };
-  // the thread (i, j) of the reference whose row is nearest to ja: j - ja in
-  // [0, 1/2] takes rows j, j-1 (:58-68), j - ja in (-1/2, 0) rows j+1, j (:69-79)
-  const int jf = (int)floorf(ja);
-  for (int j = max(jf - 1, 0); j <= min(jf + 2, ny - 1); ++j) {
-    const float dj = (float)j - ja;
-    R val;
-    if (dj >= 0.0f && dj <= 0.5f) {
-      const float djm = (float)(j - 1) - ja;
-      val = fvna::fma_(R(-djm), at(i, j), R(dj) * at(i, j - 1));
-    } else if (dj < 0.0f && -dj < 0.5f) {
-      const float djp = (float)(j + 1) - ja;
-      val = fvna::fma_(R(-dj), at(i, j + 1), R(djp) * at(i, j));
-    } else {
-      continue;
-    }
-    // the closed form at (i + MDX, j + MDY) (:90-95)
-    const R arg = R((float)(i + P.mdx)) * dx * R(D.tay_cos) +
-                  (R(j + P.mdy - ny / 2) * dy + R(D.md_t) * R(D.tay_sin)) * R(D.tay_sin) - T;
-    m[(size_t)tid + 2 * (size_t)nx] = val;
-    m[(size_t)tid + 3 * (size_t)nx] = R(D.tay_rhoa) * tanh(R(D.tay_c) * arg);
-    return;
-  }
 }
 
 // Data collection of attempt a, run after its step launch: if the attempt
@@ -883,9 +772,9 @@ __global__ __launch_bounds__(256) void fv_collect_kernel(const InstDesc* __restr
     m[nf + nx] = F.cfl;
   }
   if (D.bc == kBcTaylor)
-    collect_taylor<R>(D, P, T, m, tid);
+    fvbc::collect_taylor<R>(D, P, T, (R*)m, tid);
   else
-    collect_hifu<R>(D, P, T, nf, m, tid);
+    fvbc::collect_hifu<R>(D, P, T, nf, (R*)m, tid);
 }
 
 // Closed-form ghost ring of the Taylor-shock instances (apps/taylor-angle-xy/
@@ -919,19 +808,17 @@ __global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __
     // (apps/taylor-angle-xy/boundary.h:45-49: i + MDX, j + MDY)
     const size_t o = (size_t)j * (size_t)nx + (size_t)i;
     if (D.fp64) {
-      double a, b, c;
-      fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx, D.dy,
-                   T, i + P.mdx, j + P.mdy, a, b, c);
-      ((double*)D.q[cur][0])[o] = a;
-      ((double*)D.q[cur][1])[o] = b;
-      ((double*)D.q[cur][2])[o] = c;
+      fvbc::Cell<double> c;
+      fvbc::Taylor::prepass(D, T, i + P.mdx, j + P.mdy, c);
+      ((double*)D.q[cur][0])[o] = c.q1;
+      ((double*)D.q[cur][1])[o] = c.q2;
+      ((double*)D.q[cur][2])[o] = c.q3;
     } else {
-      float a, b, c;
-      fvbc::taylor(D.tay_rhoa, D.tay_c, D.tay_cos, D.tay_sin, D.tay_yoff, D.tay_jc, D.dx, D.dy,
-                   T, i + P.mdx, j + P.mdy, a, b, c);
-      D.q[cur][0][o] = a;
-      D.q[cur][1][o] = b;
-      D.q[cur][2][o] = c;
+      fvbc::Cell<float> c;
+      fvbc::Taylor::prepass(D, T, i + P.mdx, j + P.mdy, c);
+      D.q[cur][0][o] = c.q1;
+      D.q[cur][1][o] = c.q2;
+      D.q[cur][2][o] = c.q3;
     }
   }
 }
@@ -967,63 +854,88 @@ extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, in
 }
 
 // Resident step waves per CU (occupancy of the step kernel on this device),
-// of the FP64 kernel if fp64.  The lower of the plain and extended-boundary
-// variants, so the split fits whichever of them a context launches.
+// of the FP64 kernel if fp64.  The lowest over the plain and extended
+// kernels of several instances (every app's march: the most registers), so
+// the split fits whichever kernel a context launches.
 extern "C" hipError_t fv_step_waves_per_cu(int* waves, int fp64) {
   int a = 0, b = 0;
   hipError_t e = fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                            &a, fv::fv_step_kernel<double, false, false>, fv::kLanes, 0)
+                            &a, fv::fv_step_kernel<double, false, false, fv::AnyApp>, fv::kLanes, 0)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                            &a, fv::fv_step_kernel<float, false, false>, fv::kLanes, 0);
+                            &a, fv::fv_step_kernel<float, false, false, fv::AnyApp>, fv::kLanes, 0);
   if (e != hipSuccess) return e;
   e = fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                 &b, fv::fv_step_kernel<double, false, true>, fv::kLanes, 0)
+                 &b, fv::fv_step_kernel<double, false, true, fv::AnyApp>, fv::kLanes, 0)
            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                 &b, fv::fv_step_kernel<float, false, true>, fv::kLanes, 0);
+                 &b, fv::fv_step_kernel<float, false, true, fv::AnyApp>, fv::kLanes, 0);
   *waves = a < b ? a : b;
   return e;
 }
 
 namespace {
-template <typename R, bool kOne, bool kExt>
+template <typename R, bool kOne, bool kExt, class BC>
 void launch_step_t(const fv::InstDesc* insts, const fv::WorkItem* items, const uint64_t* rowmask,
                    int n_items, int phase, const fv::Ctl* c_in, const fv::SyncSlots* s_in,
                    hipStream_t stream) {
-  hipLaunchKernelGGL((fv::fv_step_kernel<R, kOne, kExt>), dim3(n_items), dim3(fv::kLanes), 0,
+  hipLaunchKernelGGL((fv::fv_step_kernel<R, kOne, kExt, BC>), dim3(n_items), dim3(fv::kLanes), 0,
                      stream, insts, items, rowmask, n_items, phase, c_in, s_in);
 }
+// a single instance: its app's own kernel (bc), else the dispatching one
+template <typename R, bool kExt>
+void launch_one(int bc, const fv::InstDesc* insts, const fv::WorkItem* items,
+                const uint64_t* rowmask, int n_items, int phase, const fv::Ctl* c_in,
+                const fv::SyncSlots* s_in, hipStream_t stream) {
+  switch (bc) {
+#define FV_APP_LAUNCH(P)                                                                 \
+  case P::kId:                                                                           \
+    if constexpr (kExt || !P::kExtOnly) {                                                \
+      launch_step_t<R, true, kExt, P>(insts, items, rowmask, n_items, phase, c_in, s_in, \
+                                      stream);                                           \
+      return;                                                                            \
+    }                                                                                    \
+    break;
+    FIVONAGI_APPS(FV_APP_LAUNCH)
+#undef FV_APP_LAUNCH
+    default:
+      break;
+  }
+  launch_step_t<R, true, kExt, fv::AnyApp>(insts, items, rowmask, n_items, phase, c_in, s_in,
+                                           stream);
+}
 template <typename R>
-void launch_step_r(bool one, bool ext, const fv::InstDesc* insts, const fv::WorkItem* items,
-                   const uint64_t* rowmask, int n_items, int phase, const fv::Ctl* c_in,
-                   const fv::SyncSlots* s_in, hipStream_t stream) {
+void launch_step_r(bool one, bool ext, int bc, const fv::InstDesc* insts,
+                   const fv::WorkItem* items, const uint64_t* rowmask, int n_items, int phase,
+                   const fv::Ctl* c_in, const fv::SyncSlots* s_in, hipStream_t stream) {
   if (one && !ext)
-    launch_step_t<R, true, false>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+    launch_one<R, false>(bc, insts, items, rowmask, n_items, phase, c_in, s_in, stream);
   else if (one)
-    launch_step_t<R, true, true>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+    launch_one<R, true>(bc, insts, items, rowmask, n_items, phase, c_in, s_in, stream);
   else if (!ext)
-    launch_step_t<R, false, false>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+    launch_step_t<R, false, false, fv::AnyApp>(insts, items, rowmask, n_items, phase, c_in, s_in,
+                                               stream);
   else
-    launch_step_t<R, false, true>(insts, items, rowmask, n_items, phase, c_in, s_in, stream);
+    launch_step_t<R, false, true, fv::AnyApp>(insts, items, rowmask, n_items, phase, c_in, s_in,
+                                              stream);
 }
 }  // namespace
 
 // One step attempt of every instance.  ctl0/sync0: the control blocks and
-// slot sets of the only instance when n_inst == 1 (null otherwise); ext: some
-// instance has a moving window, the HIFU boundary or data collection; fp64:
-// the instances are FP64.
+// slot sets of the only instance when n_inst == 1 (null otherwise), bc0 its
+// boundary kind; ext: some instance has a moving window, the HIFU boundary or
+// data collection; fp64: the instances are FP64.
 extern "C" hipError_t fv_launch_step(const fv::InstDesc* insts, const fv::WorkItem* items,
                                      const uint64_t* rowmask, int n_items, int phase,
-                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int ext,
-                                     int fp64, hipStream_t stream) {
+                                     int n_inst, fv::Ctl* ctl0, fv::SyncSlots* sync0, int bc0,
+                                     int ext, int fp64, hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
   const bool one = n_inst == 1 && ctl0 && sync0;
   const fv::Ctl* c_in = one ? ctl0 + (phase & 1) : nullptr;
   const fv::SyncSlots* s_in = one ? sync0 + (phase + 2) % 3 : nullptr;
   if (fp64)
-    launch_step_r<double>(one, ext != 0, insts, items, rowmask, n_items, phase, c_in, s_in,
+    launch_step_r<double>(one, ext != 0, bc0, insts, items, rowmask, n_items, phase, c_in, s_in,
                           stream);
   else
-    launch_step_r<float>(one, ext != 0, insts, items, rowmask, n_items, phase, c_in, s_in,
+    launch_step_r<float>(one, ext != 0, bc0, insts, items, rowmask, n_items, phase, c_in, s_in,
                          stream);
   return hipGetLastError();
 }

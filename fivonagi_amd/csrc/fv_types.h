@@ -118,7 +118,8 @@ struct WorkItem {
   int32_t y0, y1;
   int32_t mrow;
   int32_t local;
-  int32_t pad[2];
+  int32_t bc;           // the instance's boundary kind (picks the app's march)
+  int32_t pad;
 };
 
 }  // namespace fv

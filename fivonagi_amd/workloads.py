@@ -38,9 +38,11 @@ def gauss_pulse(nx=400, ny=400, amp=0.01, sigma_cells=10.0, beta=3.5, delta=1e-4
 
 
 def edge_profile(ny, amp=0.02, halfwidth_cells=200.0, centre=None):
+    """amp exp(-((j - jc)/w)^2) per row, with the C library's exp (as the
+    app's C header apps/edge-drive/parameters.h computes it)."""
     jc = (ny - 1) / 2.0 if centre is None else centre
-    j = np.arange(ny)
-    return amp * np.exp(-((j - jc) / halfwidth_cells) ** 2)
+    return np.array([amp * math.exp(-((j - jc) / halfwidth_cells) * ((j - jc) / halfwidth_cells))
+                     for j in range(ny)])
 
 
 def edge_drive(nx=4096, ny=1024, amp=0.02, halfwidth_cells=200.0, cells_per_wavelength=20.0,

@@ -87,7 +87,9 @@ typedef struct {
   int32_t error, cur;
   int32_t n_items, rows_per_wave; /* work items per step, mean rows per item */
   double skew;             /* height skew of the work items in use            */
-  int32_t rotate, pad;     /* 1: skew levels rotated per strip                */
+  int32_t rotate;          /* 1: skew levels rotated per strip                */
+  int32_t phase;           /* next step launch mod 6 (captured graphs replay
+                              from phase 0)                                   */
   int32_t md_x, md_y;      /* moving-window offset MDX, MDY                   */
   int32_t fp64, pad2;      /* 1: FP64 context (fivonagi_init_f64)             */
 } fivonagi_info_t;

@@ -58,7 +58,7 @@ class Box(C.Structure):
 
 
 def _load(name):
-    path = os.path.join(_HERE, name)
+    path = name if os.path.isabs(name) else os.path.join(_HERE, name)
     if not os.path.exists(path):
         raise FileNotFoundError(
             f"{path} missing: build it with `make -C oracle` (or __graft_entry__.build())")
@@ -99,7 +99,12 @@ _LIBS = {}
 
 
 def lib(fast=False):
-    name = "liboracle_fast.so" if fast else "liboracle.so"
+    """fast: the -O3 build; a path: a build of the same sources elsewhere
+    (bench.py's host-native CPU baseline)."""
+    if isinstance(fast, str):
+        name = fast
+    else:
+        name = "liboracle_fast.so" if fast else "liboracle.so"
     if name not in _LIBS:
         _LIBS[name] = _load(name)
     return _LIBS[name]
