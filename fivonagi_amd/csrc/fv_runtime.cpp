@@ -116,14 +116,20 @@ struct fivonagi_ctx {
   fv::Ctl* d_ctl = nullptr;                // 2 control blocks per instance
   fv::SyncSlots* d_sync = nullptr;         // 3 reduction slot sets per instance
   int64_t attempts = 0;                    // step attempts launched (the phase is this mod 6)
-  // captured step launches: graphs of kGraphLong and kGraphShort attempts
-  // (multiples of 6, so a graph starting at phase 0 ends at phase 0)
-  static constexpr int kGraphLong = 30, kGraphShort = 6;
+  // captured step launches, each starting at phase 0: requests of up to
+  // kGraphChunk attempts replay one graph of their exact length, longer ones
+  // graphs of kGraphChunk (a multiple of 6, so it ends at phase 0) and one of
+  // the rest; at most kGraphCache lengths are kept (the oldest goes first)
+  static constexpr int kGraphChunk = 300, kGraphCache = 8;
   int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
   int collect_nx = 0;                      // widest instance with data collection (0: none)
   bool unlimited = false;                  // targets set to "run every attempt"
-  hipGraph_t graph[2] = {nullptr, nullptr};          // long, short
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  struct Graph {
+    int len;
+    hipGraph_t g;
+    hipGraphExec_t e;
+  };
+  std::vector<Graph> graphs;                 // oldest first
   std::vector<float*> adj_lam;             // dJ/dq0 of the last adjoint call (per instance)
   struct CollectBufs { void* measure = nullptr; void* stage = nullptr; };
   std::vector<CollectBufs> collect_bufs;   // per instance, kept once allocated
@@ -140,12 +146,11 @@ struct fivonagi_ctx {
     return p;
   }
   void drop_graph() {
-    for (int k = 0; k < 2; ++k) {
-      if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
-      if (graph[k]) (void)hipGraphDestroy(graph[k]);
-      graph_exec[k] = nullptr;
-      graph[k] = nullptr;
+    for (auto& gr : graphs) {
+      (void)hipGraphExecDestroy(gr.e);
+      (void)hipGraphDestroy(gr.g);
     }
+    graphs.clear();
   }
   ~fivonagi_ctx() {
     drop_graph();
@@ -414,8 +419,19 @@ int collect_status(fivonagi_ctx* c) {
   return 0;
 }
 
-// Capture G attempts (phases 0..G-1 mod 6) into graph slot k.
-int capture_graph(fivonagi_ctx* c, int k, int G) {
+// The captured graph of G attempts from phase 0 (phases 0..G-1 mod 6),
+// captured on first use.
+int graph_for(fivonagi_ctx* c, int G, hipGraphExec_t* out) {
+  for (auto& gr : c->graphs)
+    if (gr.len == G) {
+      *out = gr.e;
+      return 0;
+    }
+  if ((int)c->graphs.size() >= fivonagi_ctx::kGraphCache) {
+    (void)hipGraphExecDestroy(c->graphs.front().e);
+    (void)hipGraphDestroy(c->graphs.front().g);
+    c->graphs.erase(c->graphs.begin());
+  }
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (int a = 0; a < G; ++a) {
     hipError_t e = launch_phase(c, a % 6);
@@ -426,13 +442,21 @@ int capture_graph(fivonagi_ctx* c, int k, int G) {
       return hip_fail(e, "fv_launch_step (capture)");
     }
   }
-  HIP_TRY(hipStreamEndCapture(c->stream, &c->graph[k]));
-  HIP_TRY(hipGraphInstantiate(&c->graph_exec[k], c->graph[k], nullptr, nullptr, 0));
+  fivonagi_ctx::Graph gr{G, nullptr, nullptr};
+  HIP_TRY(hipStreamEndCapture(c->stream, &gr.g));
+  hipError_t e = hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(gr.g);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  c->graphs.push_back(gr);
+  *out = gr.e;
   return 0;
 }
 
-// n attempts: single launches up to a phase-0 boundary, then replays of the
-// long graph, then of the short one, then at most 5 single launches.
+// n attempts: single launches up to a phase-0 boundary, then graph replays
+// (graph_for: the exact length of a request up to kGraphChunk, else chunks
+// and the rest); fewer than 6 attempts run as single launches.
 int launch_attempts(fivonagi_ctx* c, int64_t n) {
   c->ctl_cache_ok = false;
   while (n > 0 && c->attempts % 6 != 0) {
@@ -441,15 +465,19 @@ int launch_attempts(fivonagi_ctx* c, int64_t n) {
     ++c->attempts;
     --n;
   }
-  const int lens[2] = {fivonagi_ctx::kGraphLong, fivonagi_ctx::kGraphShort};
-  for (int k = 0; k < 2; ++k) {
-    const int G = lens[k];
-    if (n >= G && !c->graph_exec[k])
-      if (int rc = capture_graph(c, k, G)) return rc;
-    while (n >= G) {
-      HIP_TRY(hipGraphLaunch(c->graph_exec[k], c->stream));
-      c->attempts += G;
-      n -= G;
+  while (n >= 6) {
+    const int G = (int)std::min<int64_t>(n, fivonagi_ctx::kGraphChunk);
+    hipGraphExec_t ge = nullptr;
+    if (int rc = graph_for(c, G, &ge)) return rc;
+    HIP_TRY(hipGraphLaunch(ge, c->stream));
+    c->attempts += G;
+    n -= G;
+    // a graph of a length that is not a multiple of 6 ends mid-rotation
+    while (n > 0 && c->attempts % 6 != 0) {
+      hipError_t e = launch_phase(c, (int)(c->attempts % 6));
+      if (e != hipSuccess) return hip_fail(e, "fv_launch_step");
+      ++c->attempts;
+      --n;
     }
   }
   while (n > 0) {
