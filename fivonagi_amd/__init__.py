@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libfivonagi.so")
 
 BC_ZEROGRAD, BC_EDGE_DRIVE, BC_TAYLOR, BC_HIFU = 0, 1, 2, 3
 CFL_REFERENCE_FOLD, CFL_ALL = 0, 1
-OK, ERR_EIGEN, ERR_COLLECT, ERR_ARG, ERR_DTSEQ, ERR_STALL = 0, 2, 4, 8, 9, 10
+OK, ERR_EIGEN, ERR_COLLECT, ERR_ARG, ERR_DTSEQ, ERR_STALL, ERR_CHECK = 0, 2, 4, 8, 9, 10, 11
 
 EXPORTS = (
     "fivonagi_init", "fivonagi_init_f64", "fivonagi_set_stream", "fivonagi_set_dt_sequence",
@@ -84,7 +84,8 @@ class NoallocState(C.Structure):
 
 class AdjointOpts(C.Structure):
     """fivonagi_adjoint_opts (include/fivonagi.h)."""
-    _fields_ = [("checkpoint_every", C.c_int32), ("plain_forward", C.c_int32)]
+    _fields_ = [("checkpoint_every", C.c_int32), ("plain_forward", C.c_int32),
+                ("check_recompute", C.c_int32), ("pad", C.c_int32)]
 
 
 _lib = None
@@ -306,15 +307,19 @@ class Context:
         _check(load().fivonagi_info(self._h, inst, C.byref(i)))
         return {k: getattr(i, k) for k, _ in Info._fields_}
 
-    def step_adjoint(self, nsteps, boxes, checkpoint_every=0, plain_forward=False):
-        """Forward nsteps (recording dt), then the adjoint.  boxes: one
-        (x0, y0, nx, ny) per instance.  Returns (J, grad_amp, grad_phase) with
-        one entry per instance (arrays of ny)."""
+    def step_adjoint(self, nsteps, boxes, checkpoint_every=0, plain_forward=False,
+                     check_recompute=False):
+        """Forward nsteps (recording each step's control block), then the
+        adjoint.  boxes: one (x0, y0, nx, ny) per instance.  check_recompute:
+        the reverse pass also recomputes every state and raises
+        (ERR_CHECK) if one differs in any bit from the stored one.  Returns
+        (J, grad_amp, grad_phase) with one entry per instance (arrays of ny)."""
         n = len(self.problems)
         if isinstance(boxes, tuple):
             boxes = [boxes] * n
         objs = (Objective * n)(*[Objective(*b) for b in boxes])
-        opts = AdjointOpts(int(checkpoint_every), int(bool(plain_forward)))
+        opts = AdjointOpts(int(checkpoint_every), int(bool(plain_forward)),
+                           int(bool(check_recompute)), 0)
         J = np.zeros(n)
         tot = sum(p.ny for p in self.problems)
         ga = np.zeros(tot)

@@ -1,36 +1,36 @@
-// fv_adjoint.h -- per-instance table of the adjoint march kernels (device
-// memory), shared by fv_adjoint_march.hip and the host runtime.
+// fv_adjoint.h -- per-instance table of the adjoint's reverse march kernel
+// (device memory), shared by fv_adjoint_march.hip and the host runtime.
 #ifndef FIVONAGI_FV_ADJOINT_H
 #define FIVONAGI_FV_ADJOINT_H
 
 #include <stdint.h>
 
+#include "fv_types.h"
+
 namespace fv {
 
 struct AdjDesc {
   int32_t nx, ny, bc;
-  float beta;
-  double dx, dy, delta, omega;
-  double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff;
-  int32_t tay_jc;
+  float hbeta;                         // BETA - 1, formed as the step kernel forms it
   int32_t bx0, by0, bnx, bny;          // objective box
-  int32_t pad;
-  const float* edge_amp;               // ny drive amplitudes
-  const float* edge_phase;             // ny drive phases
-  float* seg;                          // (K+1) states of the current segment, 3 grids each
-  float* lam;                          // adjoint of the state (3 grids)
-  float* lamx;                         // adjoint of q_y between the two reverse kernels
-  double* grad_amp;                    // ny, accumulated
-  double* grad_phase;                  // ny, accumulated
-  double* jstep;                       // per step: dt_n sum_box rho'_{n+1}^2
-  const double* dt_log;                // dt of accepted step n of the run
-  const double* T_log;                 // T at the start of accepted step n
+  const void* edge_c;                  // the step's drive tables (float, ny each)
+  const void* edge_s;
+  float* seg;                          // states of the current segment: slot s = 3 grids
+  float* lam[2];                       // adjoint of the state (3 grids), ping-pong by step
+  const Ctl* ctl_log;                  // control block accepted step n of the run ran with
+  double* grad_ec;                     // dJ/d(amp_j cos phase_j), ny, accumulated
+  double* grad_es;                     // dJ/d(amp_j sin phase_j)
+  double* jpart;                       // per work item: sum over steps of dt sum_box rho'^2
+  uint32_t* mismatch;                  // cells whose recomputed state differs (checking)
 };
 
-// One wave's work in the adjoint march kernels: a strip of one instance,
-// output columns [x0, x0 + width), rows [y0, y1).
+// One wave's work in the reverse march: the strip whose lane kHalo holds
+// column x0, output columns [xo0, xo1), output rows [y0, y1).  Strip and row
+// boundaries never separate column 2 from columns 0, 1, column nx-3 from
+// nx-2, nx-1, row 2 from rows 0, 1 or row ny-3 from ny-2, ny-1 (the ghost
+// copies' transposes stay within one wave).
 struct AdjItem {
-  int32_t inst, x0, y0, y1;
+  int32_t inst, x0, xo0, xo1, y0, y1;
 };
 
 }  // namespace fv

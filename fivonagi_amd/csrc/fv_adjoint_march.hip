@@ -1,529 +1,517 @@
-// fv_adjoint_march.hip -- the adjoint step as three march kernels (gfx950).
+// fv_adjoint_march.hip -- the reverse step of the discrete adjoint as one
+// march kernel (gfx950).
 //
-// The reference has no adjoint.  The step (reference: common/main.cu:310-635,
-// apps/na/fv.h, apps/na/source.h, common/boundary.h) is
-//   q_b = B(q_n, T_n), q_y = Y(q_b), q_xy = X(q_y), q_{n+1} = S(q_xy)
-// and the objective J = sum_n dt_n sum_box rho'_{n+1}^2 (dt frozen, the
-// executed branch path differentiated).  Per step:
-//   adj_march_x<false>  q_n -> q_{n+1}                      (segment recompute)
-//   adj_march_x<true>   q_n, lam_{n+1} -> lam_y = X^T S^T (lam_{n+1} + dJ/dq_{n+1})
-//   adj_march_y         q_n, lam_y -> lam_n = B^T Y^T lam_y, drive gradient
-// Each is a wave64 marching up a strip of columns, like the forward step
-// kernel: the forward quantities the transposes need (waves, speeds,
-// limiter branches) are recomputed in registers from q_n instead of being
-// stored; x neighbours come from DPP lane shifts, y neighbours from
-// register windows.  adj_march_x uses a 4-column halo (56 output columns),
-// adj_march_y a 2-column halo (60).
+// The reference has no adjoint.  The step it differentiates (reference:
+// common/main.cu:310-635, apps/na/fv.h, apps/na/source.h, common/boundary.h)
+// is the one csrc/fv_step.hip runs:
+//   q_b = B(q_n), q_y = Y(q_b), q_xy = X(q_y), q_{n+1} = S(q_xy)
+// with J = sum_n dt_n sum_box rho'_{n+1}^2 (dt frozen).  One launch of
+// adj_reverse_kernel takes lam_{n+1} (the adjoint of q_{n+1}) to
+//   lam_n = B^T Y^T X^T S^T (lam_{n+1} + dJ_n/dq_{n+1})
+// and adds step n's share to the drive gradient and to J.
+//
+// Each wave marches up a strip of 56 output columns (4-column halo on each
+// side), like the step kernel: per row it recomputes the forward values of
+// q_n with the step kernel's own functions (the app's boundary patch, the
+// solver.h Riemann solve, limiter and fluxes, and the control block the
+// forward step ran with: its dt/dx, dt/dy and drive phase as the step used
+// them), so every branch it differentiates is the one the forward step took,
+// and applies the transposes in the same pass:
+//   row m = r - 2:  q_y(m), the x sweep of m, S^T and X^T of m -> lam_y(m)
+//   interface m:    the y limiter's transpose (lam_y of m and m-1)
+//   interface m-1:  the Riemann transpose (its wave adjoint is complete once
+//                   the limiters of m-2, m-1 and m are done) -> lam_b(m-2)
+//   row m-2:        B^T -> lam_n(m-2)
+// x neighbours come from DPP lane shifts, y neighbours from register windows.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../apps/apps.h"
 #include "../apps/na/adjoint.h"
+#include "../apps/na/source.h"
 #include "fv_adjoint.h"
+#include "fv_types.h"
 
 namespace fv {
 namespace adjm {
 
 constexpr int kLanes = 64;
+constexpr int kHalo = 4;                       // x halo of the reverse march, each side
+constexpr int kCols = kLanes - 2 * kHalo;      // 56 output columns per strip
 
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float cgfloat;
+typedef __attribute__((address_space(1))) char gchar;
+typedef __attribute__((address_space(1))) const char cgchar;
+typedef __attribute__((address_space(4))) const float kfloat;   // read-only: scalar loads
+
+// loads and stores at a 32-bit byte offset from a uniform base
+__device__ __forceinline__ float ldg(cgfloat* base, uint32_t off) {
+  return *(cgfloat*)((cgchar*)base + off);
+}
+__device__ __forceinline__ void stg(gfloat* base, uint32_t off, float v) {
+  *(gfloat*)((gchar*)base + off) = v;
+}
+
+// shr1: lane l receives lane l-1's value (lane 0 receives 0);
+// shl1: lane l receives lane l+1's value (lane 63 receives 0).
 __device__ __forceinline__ float shr1(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float shl1(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
-__device__ __forceinline__ int shr1i(int v) {
-  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
+
+struct Vec3 {
+  float a, b, c;       // rho', rho u, rho v
+};
+__device__ __forceinline__ Vec3 v3(float a, float b, float c) { return Vec3{a, b, c}; }
+__device__ __forceinline__ Vec3 operator+(const Vec3& x, const Vec3& y) {
+  return Vec3{x.a + y.a, x.b + y.b, x.c + y.c};
 }
-__device__ __forceinline__ int shl1i(int v) {
-  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
+__device__ __forceinline__ Vec3 sel(bool p, const Vec3& x) {
+  return Vec3{p ? x.a : 0.0f, p ? x.b : 0.0f, p ? x.c : 0.0f};
 }
+__device__ __forceinline__ Vec3 shr1(const Vec3& x) { return Vec3{shr1(x.a), shr1(x.b), shr1(x.c)}; }
+__device__ __forceinline__ Vec3 shl1(const Vec3& x) { return Vec3{shl1(x.a), shl1(x.b), shl1(x.c)}; }
 
-struct Cell { float r, u, v; };   // rho', rho u, rho v
-
-// Cell (i, r) of q_b = B(q_n, T) for the lane's column i (rows and columns
-// clamped into the grid; lanes and rows outside it only feed halo work).
-// Ghost rules as apps/common/boundary.h: left, then top, then bottom, then
-// right; the drive value p and the Taylor closed form are evaluated in double.
-struct Loader {
-  const float* q;
-  int nx, ny, n, bc, i, ic;
-  double T, omega;
-  const float* amp;
-  const float* phase;
-  double tay_rhoa, tay_c, tay_cos, tay_sin, tay_yoff, dx, dy;
-  int tay_jc;
-
-  __device__ Cell operator()(int r) const {
-    const int rc = min(max(r, 0), ny - 1);
-    Cell c;
-    if (bc == 2) {
-      if (ic <= 1 || rc <= 1 || ic >= nx - 2 || rc >= ny - 2) {
-        const double arg = tay_c * ((double)ic * dx * tay_cos +
-                                    ((double)(rc - tay_jc) * dy + tay_yoff) * tay_sin - T);
-        const double rr = tay_rhoa * tanh(arg);
-        c.r = (float)rr; c.u = (float)(rr * tay_cos); c.v = (float)(rr * tay_sin);
-        return c;
-      }
-      const int o = ic + rc * nx;
-      c.r = q[o]; c.u = q[n + o]; c.v = q[2 * n + o];
-      return c;
-    }
-    int si = ic, sj = rc;
-    if (ic <= 1) {
-      if (bc == 1) {
-        const double th = omega * T + (double)phase[rc];
-        const float p = (float)((double)amp[rc] * sin(th));
-        c.r = p; c.u = p * (p + 1.0f); c.v = 0.0f;
-        return c;
-      }
-      si = 2;
-    } else if (rc >= ny - 2) {
-      sj = ny - 3;
-    } else if (rc <= 1) {
-      sj = 2;
-    } else if (ic >= nx - 2) {
-      si = nx - 3;
-    }
-    const int o = si + sj * nx;
-    c.r = q[o]; c.u = q[n + o]; c.v = q[2 * n + o];
-    return c;
-  }
+struct Row {           // one cell of q_b as the y sweep sees it
+  float q1, q2, q3;    // rho', rho u, rho v
+  float un, ut;        // v = q3/rho (y normal), u = q2/rho (tangential)
 };
 
-__device__ __forceinline__ Loader make_loader(const AdjDesc& d, const float* q, double T, int i) {
-  Loader L;
-  L.q = q; L.nx = d.nx; L.ny = d.ny; L.n = d.nx * d.ny; L.bc = d.bc; L.i = i;
-  L.ic = min(max(i, 0), d.nx - 1);
-  L.T = T; L.omega = d.omega; L.amp = d.edge_amp; L.phase = d.edge_phase;
-  L.tay_rhoa = d.tay_rhoa; L.tay_c = d.tay_c; L.tay_cos = d.tay_cos; L.tay_sin = d.tay_sin;
-  L.tay_yoff = d.tay_yoff; L.dx = d.dx; L.dy = d.dy; L.tay_jc = d.tay_jc;
-  return L;
-}
+struct XRow {          // one cell of q_xy (the recomputed-state check)
+  float q1, q2, q3, u, v;
+};
 
-struct YIf { float W[7]; float s[3]; };   // y interface: waves (rho', rho v, rho u) and speeds
+__device__ __forceinline__ fvad::SumAdj zero_sum() { return fvad::SumAdj{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
 
-__device__ __forceinline__ bool interior(int i, int j, int nx, int ny) {
-  return i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3;
-}
+// The reverse step of one work item of an instance whose ghost cells follow
+// the boundary policy BC.  kCheck: also recompute q_{n+1} on the item's
+// cells and count those that differ from the stored one (D.mismatch).
+template <class BC, bool kCheck>
+__device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it, int item,
+                                             int k, int slot, int par) {
+  const int nx = D.nx, ny = D.ny;
+  const uint32_t n = (uint32_t)nx * (uint32_t)ny;
+  const Ctl* __restrict__ C = D.ctl_log + k;
+  const double dt = C->dt;
+  const float dtx = C->dtx, dty = C->dty, ddx2 = C->ddx2, ddy2 = C->ddy2;
+  const float sin_t = (float)C->drive_s, cos_t = (float)C->drive_c;
+  const float hbeta = D.hbeta;
+  const float* qs = D.seg + (size_t)slot * 3 * n;
+  cgfloat* __restrict__ in1 = (cgfloat*)qs;
+  cgfloat* __restrict__ in2 = (cgfloat*)(qs + n);
+  cgfloat* __restrict__ in3 = (cgfloat*)(qs + 2 * (size_t)n);
+  cgfloat* __restrict__ nq1 = (cgfloat*)(qs + 3 * (size_t)n);   // q_{n+1}
+  cgfloat* __restrict__ nq2 = (cgfloat*)(qs + 4 * (size_t)n);
+  cgfloat* __restrict__ nq3 = (cgfloat*)(qs + 5 * (size_t)n);
+  cgfloat* __restrict__ g1p = (cgfloat*)D.lam[par];
+  cgfloat* __restrict__ g2p = (cgfloat*)(D.lam[par] + n);
+  cgfloat* __restrict__ g3p = (cgfloat*)(D.lam[par] + 2 * (size_t)n);
+  gfloat* __restrict__ o1 = (gfloat*)D.lam[par ^ 1];
+  gfloat* __restrict__ o2 = (gfloat*)(D.lam[par ^ 1] + n);
+  gfloat* __restrict__ o3 = (gfloat*)(D.lam[par ^ 1] + 2 * (size_t)n);
 
-// y interface between rows a (below) and b (above), y component order
-__device__ __forceinline__ void y_waves(const Cell& a, const Cell& b, float beta, YIf& f) {
-  fvad::RW w;
-  fvad::riemann_fw(a.r, a.v, a.u, b.r, b.v, b.u, beta, w);
+  const int lane = threadIdx.x;
+  const int i = it.x0 - kHalo + lane;
+  const bool col_in = i >= 0 && i < nx;
+  const int ic = min(max(i, 0), nx - 1);
+  const bool col_int = i >= 2 && i <= nx - 3;
+  const bool col_x = col_in && i >= 1;           // lane holds an x interface of the grid
+  const bool out_lane = i >= it.xo0 && i < it.xo1;
+  const bool box_col = i >= D.bx0 && i < D.bx0 + D.bnx;
+  const uint32_t row_bytes = 4u * (uint32_t)nx;
+  const uint32_t cb = 4u * (uint32_t)ic;
+  const float two_dt = (float)(2.0 * dt);
+  const int y0 = it.y0, y1 = it.y1;
+
+  // the boundary policy's source columns and patch, as the step kernel
+  fvbc::LaneMap lm;
+  BC::lane(ic, nx, lm, 4u);
+  fvbc::Tables tb{D.edge_c, D.edge_s, nullptr};
+  fvbc::PatchCtx<float> pc;
+  pc.sin_t = sin_t;
+  pc.cos_t = cos_t;
+  pc.T = C->T;
+  pc.hk = 0.0f;
+  pc.mdx = 0;
+  pc.x0 = it.x0 - 1;     // the column of lane 3 (the step kernel's first output lane)
+  pc.nx = nx;
+  pc.ny = ny;
+  pc.ic = ic;
+  pc.left = lm.left;
+  auto load_raw = [&](int r, float& a, float& b, float& c, fvbc::RowData<float>& h) {
+    const int rc = min(max(r, 0), ny - 1);
+    const uint32_t o = BC::offset(rc, ny, lm, row_bytes);
+    a = ldg(in1, o);
+    b = ldg(in2, o);
+    c = ldg(in3, o);
+    if (BC::kRowData) BC::template load_row<float>(tb, rc, h);
+  };
+  auto patch = [&](Row& q, const fvbc::RowData<float>& h, int r) {
+    fvbc::Cell<float> c{q.q1, q.q2, q.q3};
+    BC::patch(pc, min(max(r, 0), ny - 1), c, h);
+    q.q1 = c.q1;
+    q.q2 = c.q2;
+    q.q3 = c.q3;
+    const float ir = fvna::rcp(1.0f + q.q1);
+    q.un = q.q3 * ir;
+    q.ut = q.q2 * ir;
+  };
+  // g(l) = lam_{n+1}(l) + dJ_n/dq_{n+1}(l), where the box term is
+  // 2 dt rho'_{n+1}; row l is requested one step before it is used
+  // (load_g), then completed (finish_g), which also adds J_n's share of the
+  // item's own rows
+  double jacc = 0.0;
+  const bool in_box_col = col_in && box_col;
+  auto box_row = [&](int l) { return l >= 0 && l < ny && l >= D.by0 && l < D.by0 + D.bny; };
+  auto load_g = [&](int l, Vec3& g, float& rn) {
+    g = v3(0.f, 0.f, 0.f);
+    rn = 0.0f;
+    if (l < 0 || l >= ny) return;
+    const uint32_t o = (uint32_t)l * row_bytes + cb;
+    g.a = ldg(g1p, o);
+    g.b = ldg(g2p, o);
+    g.c = ldg(g3p, o);
+    if (box_row(l)) rn = ldg(nq1, o);
+  };
+  auto finish_g = [&](int l, Vec3& g, float rn) {
+    g = sel(col_in, g);
+    if (box_row(l)) {
+      g.a = in_box_col ? fvna::fma_(two_dt, rn, g.a) : g.a;
+      if (l >= y0 && l < y1 && in_box_col && out_lane) jacc += (double)rn * (double)rn;
+    }
+  };
+
+  // ---- windows (every one zeroed: the first rows of the march only warm
+  // them up, and nothing they hold reaches an emitted row) ----
+  Row b1{}, b2{}, b3{}, b4{};               // q_b rows r-1 .. r-4
+  fvna::Iface f1{}, f2{}, f3{};             // y interfaces r-1, r-2, r-3
+  fvad::LimT lim2{};                        // limiter of interface r-2
+  float dl1[3] = {0.f, 0.f, 0.f};           // W(r-1) . W(r-2) per family
+  float Pp[3] = {0.f, 0.f, 0.f};            // P of interface r-2
+  fvad::WaveAdj acc3, acc2;                 // wave adjoints of interfaces r-3, r-2
+  fvad::zero(acc3);
+  fvad::zero(acc2);
+  Vec3 ly1{}, ly2{};                        // lam_y rows r-3, r-4
+  fvad::SumAdj gRp = zero_sum();            // transpose of interface r-4 (its cell r-4 part)
+  Vec3 gd{}, g0{}, gu{}, gnx{};             // g rows m-1, m, m+1 and the prefetched row
+  float rnx = 0.0f;                         // rho' of q_{n+1} on the prefetched row (box rows)
+  Vec3 hbot{}, htop{}, hold{};              // lam_b(0) + lam_b(1); lam_b(ny-2) + lam_b(ny-1);
+                                            // lam_n(ny-3) waiting for the top rule
+  XRow xc1{}, xc2{};                        // check: q_xy rows m-1, m-2
+  uint32_t bad = 0;
+
+  const int r0 = y0 - 3, r_end = y1 + 3;
+  float n1, n2, n3;
+  fvbc::RowData<float> nh{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  load_raw(r0 - 1, n1, n2, n3, nh);
+  b1.q1 = n1; b1.q2 = n2; b1.q3 = n3;
+  patch(b1, nh, r0 - 1);
+  load_raw(r0, n1, n2, n3, nh);
+  load_g(r0 - 1, gnx, rnx);
+
+  for (int r = r0; r <= r_end; ++r) {
+    Row bn;
+    bn.q1 = n1; bn.q2 = n2; bn.q3 = n3;
+    patch(bn, nh, r);
+    if (r + 1 <= r_end) load_raw(r + 1, n1, n2, n3, nh);
+    gd = g0; g0 = gu; gu = gnx;
+    finish_g(r - 1, gu, rnx);
+    load_g(r, gnx, rnx);
+
+    // ---- y interface r (rows r-1 | r): normal = q3, tangential = q2 ----
+    fvna::Iface fn;
+    fvna::riemann(b1.q1, b1.q3, b1.q2, b1.un, b1.ut, bn.q1, bn.q3, bn.q2, bn.un, bn.ut, hbeta, fn);
+    float dln[3];
+    dln[0] = fvna::dot1(fn.w, f1.w);
+    dln[1] = fvna::dot2(fn.w, f1.w);
+    dln[2] = fn.w.a3 * f1.w.a3;
+    // ---- y limiter at interface r-1 ----
+    float P1[3], M[3];
+    fvad::LimT lim1;
+    fvad::limiter_rec(f1, dl1[0], dln[0], dl1[1], dln[1], dl1[2], dln[2], dty, P1, M, lim1);
+
+    // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2) ----
+    const int m = r - 2;
+    const bool row_int = m >= 2 && m <= ny - 3;
+    const bool upd = col_int && row_int;
+    const float dtyu = upd ? dty : 0.0f, dtxu = upd ? dtx : 0.0f;
+    const float y1q = b2.q1 - dtyu * (Pp[0] + M[0]);
+    const float y3q = b2.q3 - dtyu * (Pp[1] + M[1]);
+    const float y2q = b2.q2 - dtyu * (Pp[2] + M[2]);
+
+    // ---- x sweep on row m: normal = q2, tangential = q3 ----
+    const float ir = fvna::rcp(1.0f + y1q);
+    const float un = y2q * ir, ut = y3q * ir;
+    const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
+    const float lun = shr1(un), lut = shr1(ut);
+    fvna::Iface fx;
+    fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, hbeta, fx);
+    fvna::Waves wl;
+    wl.a1 = shr1(fx.w.a1); wl.w21 = shr1(fx.w.w21); wl.w31 = shr1(fx.w.w31);
+    wl.a2 = shr1(fx.w.a2); wl.w22 = shr1(fx.w.w22); wl.w32 = shr1(fx.w.w32);
+    wl.a3 = shr1(fx.w.a3);
+    const float dlx1 = fvna::dot1(fx.w, wl), dlx2 = fvna::dot2(fx.w, wl);
+    const float dlx3 = fx.w.a3 * wl.a3;
+    const float drx1 = shl1(dlx1), drx2 = shl1(dlx2), drx3 = shl1(dlx3);
+    float PX[3], MX[3];
+    fvad::LimT limx;
+    fvad::limiter_rec(fx, dlx1, drx1, dlx2, drx2, dlx3, drx3, dtx, PX, MX, limx);
+    const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
+    XRow xc;
+    xc.q1 = y1q - dtxu * (mr0 + PX[0]);
+    xc.q2 = y2q - dtxu * (mr1 + PX[1]);
+    xc.q3 = y3q - dtxu * (mr2 + PX[2]);
+    const float ir2 = fvna::rcp(1.0f + xc.q1);
+    xc.u = xc.q2 * ir2;
+    xc.v = xc.q3 * ir2;
+
+    if (kCheck) {
+      // the source of row m-1 from q_xy rows m-2, m-1, m, against q_{n+1}
+      const int s = m - 1;
+      const float uwe = shr1(xc1.u) + shl1(xc1.u);
+      const float vwe = shr1(xc1.v) + shl1(xc1.v);
+      const float i2 = fvna::source_increment(ddx2, ddy2, xc1.u, uwe, xc2.u, xc.u);
+      const float i3 = fvna::source_increment(ddx2, ddy2, xc1.v, vwe, xc2.v, xc.v);
+      const bool s_int = col_int && s >= 2 && s <= ny - 3;
+      const float r2 = s_int ? fvna::apply_source(xc1.q2, i2) : xc1.q2;
+      const float r3 = s_int ? fvna::apply_source(xc1.q3, i3) : xc1.q3;
+      // a Taylor state's ghost ring is rewritten in place by the next step's
+      // prepass (its closed form at the next T): only the cells the step
+      // computes are compared there
+      const bool cmp = BC::kId != kBcTaylor || s_int;
+      if (s >= y0 && s < y1 && out_lane && cmp) {
+        const uint32_t o = (uint32_t)s * row_bytes + cb;
+        const bool same = __float_as_uint(ldg(nq1, o)) == __float_as_uint(xc1.q1) &&
+                          __float_as_uint(ldg(nq2, o)) == __float_as_uint(r2) &&
+                          __float_as_uint(ldg(nq3, o)) == __float_as_uint(r3);
+        bad += same ? 0u : 1u;
+      }
+      xc2 = xc1;
+      xc1 = xc;
+    }
+
+    // ---- S^T on row m: the source adds ddx2 (u_W - 2u + u_E) +
+    // ddy2 (u_S - 2u + u_N) to rho u, the same with v to rho v, on interior cells ----
+    const bool int_d = col_int && m - 1 >= 2 && m - 1 <= ny - 3;
+    const bool int_u = col_int && m + 1 >= 2 && m + 1 <= ny - 3;
+    const float G2c = upd ? g0.b : 0.0f, G3c = upd ? g0.c : 0.0f;
+    const float G2y = (int_d ? gd.b : 0.0f) + (int_u ? gu.b : 0.0f);
+    const float G3y = (int_d ? gd.c : 0.0f) + (int_u ? gu.c : 0.0f);
+    const float sU = fvna::fma_(ddx2, fvna::fma_(-2.0f, G2c, shr1(G2c) + shl1(G2c)),
+                                ddy2 * fvna::fma_(-2.0f, G2c, G2y));
+    const float sV = fvna::fma_(ddx2, fvna::fma_(-2.0f, G3c, shr1(G3c) + shl1(G3c)),
+                                ddy2 * fvna::fma_(-2.0f, G3c, G3y));
+    Vec3 lxy;
+    lxy.a = g0.a - fvna::fma_(sU, xc.u, sV * xc.v) * ir2;
+    lxy.b = fvna::fma_(sU, ir2, g0.b);
+    lxy.c = fvna::fma_(sV, ir2, g0.c);
+
+    // ---- X^T on row m ----
+    // q_xy(k) = q_y(k) - dtxu(k) (PX(k) + MX(k+1))
+    float gPX[3] = {-dtxu * lxy.a, -dtxu * lxy.b, -dtxu * lxy.c};
+    float gMX[3] = {shr1(gPX[0]), shr1(gPX[1]), shr1(gPX[2])};
+    fvad::WaveAdj gx;
+    fvad::zero(gx);
+    float gdx[3];
+    fvad::limiter_bw(fx, limx, dtx, gPX, gMX, gx, gdx);
+    {
+      // family p of interface k dotted with k-1 (L_p > 0) or with k+1
+      fvna::Waves wr;
+      wr.a1 = shl1(fx.w.a1); wr.w21 = shl1(fx.w.w21); wr.w31 = shl1(fx.w.w31);
+      wr.a2 = shl1(fx.w.a2); wr.w22 = shl1(fx.w.w22); wr.w32 = shl1(fx.w.w32);
+      wr.a3 = shl1(fx.w.a3);
+      const float Ls[3] = {fx.L1, fx.L2, fx.L3};
+      float kl[3], kr[3];
 #pragma unroll
-  for (int c = 0; c < 7; ++c) f.W[c] = w.W[c];
-  f.s[0] = w.s[0]; f.s[1] = w.s[1]; f.s[2] = w.s[2];
+      for (int p = 0; p < 3; ++p) {
+        const float gdl = Ls[p] > 0.0f ? gdx[p] : 0.0f;
+        const float gdr = gdx[p] - gdl;
+        kl[p] = gdl + shr1(gdr);      // times W(k-1): own left dot, and k-1's right dot
+        kr[p] = gdr + shl1(gdl);      // times W(k+1): own right dot, and k+1's left dot
+      }
+      fvad::add_scaled(gx, kl, wl);
+      fvad::add_scaled(gx, kr, wr);
+    }
+    fvad::SumAdj sx = fvad::riemann_bw(fvna::add_nc(l1, y1q), fvna::add_nc(lun, un),
+                                       fvna::add_nc(lut, ut), y1q - l1, y2q - l2, hbeta, fx, gx);
+    if (!col_x) sx = zero_sum();
+    Vec3 lyc;
+    {
+      // cell k: interface k's R part and interface k+1's L part (lane k+1)
+      const float ar1 = (sx.s1 + sx.dr) + shl1(sx.s1 - sx.dr);
+      const float an = sx.dn - shl1(sx.dn);
+      const float at = sx.dt - shl1(sx.dt);
+      const float aun = sx.s2 + shl1(sx.s2);
+      const float aut = sx.s3 + shl1(sx.s3);
+      lyc.a = lxy.a + ar1 - fvna::fma_(aun, un, aut * ut) * ir;
+      lyc.b = lxy.b + fvna::fma_(aun, ir, an);
+      lyc.c = lxy.c + fvna::fma_(aut, ir, at);
+    }
+
+    // ---- Y^T: limiter of interface m (P -> cell m, M -> cell m-1) ----
+    fvad::WaveAdj acc1;
+    fvad::zero(acc1);
+    {
+      const bool upd1 = col_int && m - 1 >= 2 && m - 1 <= ny - 3;
+      const float dtyu1 = upd1 ? dty : 0.0f;
+      // y component order (rho', normal = rho v, tangential = rho u)
+      float gPy[3] = {-dtyu * lyc.a, -dtyu * lyc.c, -dtyu * lyc.b};
+      float gMy[3] = {-dtyu1 * ly1.a, -dtyu1 * ly1.c, -dtyu1 * ly1.b};
+      float gdy[3];
+      fvad::limiter_bw(f2, lim2, dty, gPy, gMy, acc2, gdy);
+      const float Ls[3] = {f2.L1, f2.L2, f2.L3};
+      float kdn[3], kup[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        kdn[p] = Ls[p] > 0.0f ? gdy[p] : 0.0f;   // dotted with interface m-1
+        kup[p] = gdy[p] - kdn[p];                // ... with m+1
+      }
+      fvad::add_scaled(acc2, kdn, f3.w);
+      fvad::add_scaled(acc2, kup, f1.w);
+      fvad::add_scaled(acc3, kdn, f2.w);
+      fvad::add_scaled(acc1, kup, f2.w);
+    }
+    // ---- Riemann transpose of interface m-1 = r-3 (rows r-4 | r-3) ----
+    fvad::SumAdj sy = fvad::riemann_bw(fvna::add_nc(b4.q1, b3.q1), fvna::add_nc(b4.un, b3.un),
+                                       fvna::add_nc(b4.ut, b3.ut), b3.q1 - b4.q1, b3.q3 - b4.q3,
+                                       hbeta, f3, acc3);
+    if (!(col_in && r - 3 >= 1 && r - 3 <= ny - 1)) sy = zero_sum();
+    // ---- lam_b of row r-4: lam_y, interface r-4's R part, interface r-3's L part ----
+    Vec3 lb;
+    {
+      const float br1 = (gRp.s1 + gRp.dr) + (sy.s1 - sy.dr);
+      const float bn_ = gRp.dn - sy.dn;
+      const float bt = gRp.dt - sy.dt;
+      const float bun = gRp.s2 + sy.s2;
+      const float but = gRp.s3 + sy.s3;
+      const float ir4 = fvna::rcp(1.0f + b4.q1);
+      lb.a = ly2.a + br1 - fvna::fma_(bun, b4.un, but * b4.ut) * ir4;
+      lb.c = ly2.c + fvna::fma_(bun, ir4, bn_);     // normal = rho v
+      lb.b = ly2.b + fvna::fma_(but, ir4, bt);      // tangential = rho u
+    }
+    gRp = sy;
+
+    // ---- B^T and the drive gradient of row j = r-4 ----
+    const int j = r - 4;
+    if (j >= y0 && j < y1) {
+      const bool interior = col_int && j >= 2 && j <= ny - 3;
+      Vec3 a = sel(interior, lb);
+      bool defer = false;
+      if (BC::kId != kBcTaylor) {
+        // zero-gradient copies (apps/common/boundary.h): left columns <- 2
+        // (zero-gradient app only; the edge drive sets them), right columns
+        // <- nx-3 on rows 2..ny-3, rows 0, 1 <- 2 and ny-2, ny-1 <- ny-3 off
+        // the left columns
+        if (BC::kId == kBcZeroGrad) {
+          const Vec3 w1 = shr1(lb), w2 = shr1(w1);
+          if (i == 2) a = a + w1 + w2;
+        }
+        {
+          const Vec3 e1 = shl1(lb), e2 = shl1(e1);
+          if (i == nx - 3 && j >= 2 && j <= ny - 3) a = a + e1 + e2;
+        }
+        if (j == 0) hbot = lb;
+        if (j == 1) hbot = hbot + lb;
+        if (j == 2 && i >= 2) a = a + hbot;
+        if (j >= ny - 2) htop = htop + lb;
+        if (j == ny - 3) {
+          hold = a;
+          defer = true;
+        }
+      }
+      const uint32_t o = (uint32_t)j * row_bytes + cb;
+      if (out_lane && !defer) {
+        stg(o1, o, a.a);
+        stg(o2, o, a.b);
+        stg(o3, o, a.c);
+      }
+      if (BC::kId != kBcTaylor && j == ny - 1) {
+        const Vec3 t = i >= 2 ? hold + htop : hold;
+        const uint32_t ot = (uint32_t)(ny - 3) * row_bytes + cb;
+        if (out_lane) {
+          stg(o1, ot, t.a);
+          stg(o2, ot, t.b);
+          stg(o3, ot, t.c);
+        }
+      }
+      if (BC::kId == kBcEdgeDrive && it.xo0 == 0) {
+        // left columns hold p = ampc sin(theta) + amps cos(theta), rho u = p (p + 1)
+        const float ec = ((kfloat*)D.edge_c)[j], es = ((kfloat*)D.edge_s)[j];
+        fvbc::Cell<float> dc;
+        fvbc::drive(ec, es, sin_t, cos_t, dc);
+        const float p = dc.q1;
+        const float tp = (i == 0 || i == 1) ? fvna::fma_(lb.b, 2.0f * p + 1.0f, lb.a) : 0.0f;
+        // columns 0 and 1 are lanes kHalo and kHalo + 1 of the first strip
+        const int tb0 = __builtin_amdgcn_readlane(__float_as_int(tp), kHalo);
+        const int tb1 = __builtin_amdgcn_readlane(__float_as_int(tp), kHalo + 1);
+        const double g = (double)__int_as_float(tb0) + (double)__int_as_float(tb1);
+        if (lane == 0) {
+          D.grad_ec[j] += g * (double)sin_t;
+          D.grad_es[j] += g * (double)cos_t;
+        }
+      }
+    }
+
+    // ---- rotate the windows ----
+    b4 = b3; b3 = b2; b2 = b1; b1 = bn;
+    f3 = f2; f2 = f1; f1 = fn;
+    lim2 = lim1;
+    dl1[0] = dln[0]; dl1[1] = dln[1]; dl1[2] = dln[2];
+    Pp[0] = P1[0]; Pp[1] = P1[1]; Pp[2] = P1[2];
+    acc3 = acc2; acc2 = acc1;
+    ly2 = ly1; ly1 = lyc;
+  }
+
+  // J of this step over the item's box cells, one double per item
+  double v = jacc;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0 && v != 0.0) D.jpart[item] += dt * v;
+  if (kCheck) {
+    if (__builtin_amdgcn_ballot_w64(bad != 0) != 0) {
+      uint32_t s = bad;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane == 0) atomicAdd(D.mismatch, s);
+    }
+  }
 }
 
 }  // namespace adjm
 
 using namespace adjm;
 
-// ------------------------------------------------------------ x march --
-// kRev = false: q_{n+1} = S(X(Y(B(q_n)))) into state slot+1.
-// kRev = true:  lam_y = X^T S^T (lam + dJ/dq_{n+1}) into d.lamx, and
-//               jstep[k] = dt_k sum_box rho'_{n+1}^2.
-template <bool kRev>
-__global__ __launch_bounds__(64) void adj_march_x(const AdjDesc* __restrict__ ds,
-                                                  const AdjItem* __restrict__ items, int k,
-                                                  int slot) {
-  constexpr int kHalo = 4;
-  const AdjItem it = items[blockIdx.x];
-  const AdjDesc& d = ds[it.inst];
-  const int nx = d.nx, ny = d.ny, n = nx * ny;
-  const float* q = d.seg + (size_t)slot * 3 * n;
-  const double dt = d.dt_log[k];
-  const double T = d.T_log[k];
-  const float dtx = (float)(dt / d.dx), dty = (float)(dt / d.dy);
-  const float ddx2 = (float)(d.delta * dt / (d.dx * d.dx));
-  const float ddy2 = (float)(d.delta * dt / (d.dy * d.dy));
-  const float beta = d.beta;
-  const int lane = threadIdx.x;
-  const int i = it.x0 - kHalo + lane;
-  const bool col_int = i >= 2 && i <= nx - 3;
-  const bool out_lane = lane >= kHalo && lane < kLanes - kHalo && i < nx;
-  const bool iface_x = i >= 1 && i <= nx - 1;   // x interface (i-1 | i) exists
-  const int y0 = it.y0, y1 = it.y1;
-  const Loader load = make_loader(d, q, T, i);
-  const bool in_box_col = i >= d.bx0 && i < d.bx0 + d.bnx;
-
-  // windows: q_b rows r, r-1, r-2; y interfaces r, r-1, r-2; P of interface r-2
-  Cell b0{}, b1{}, b2{};
-  YIf f0{}, f1{}, f2{};
-  float Pp[3] = {0.f, 0.f, 0.f};
-  // forward-only: q_xy rows m-1, m (the source at m-1 needs m-2..m)
-  Cell xm2{}, xm1{};
-  float um2 = 0.f, vm2 = 0.f, um1 = 0.f, vm1 = 0.f;
-  // reverse-only: lam rows r-1 (next), r-2, r-3
-  float la[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-  double jacc = 0.0;
-
-  auto load_lam = [&](int row, float out[3]) {
-    if (row < 0 || row >= ny || i < 0 || i >= nx) {
-      out[0] = out[1] = out[2] = 0.0f;
-      return;
-    }
-    const int o = i + row * nx;
-    out[0] = d.lam[o]; out[1] = d.lam[n + o]; out[2] = d.lam[2 * n + o];
-  };
-
-  const int r_end = kRev ? y1 + 1 : y1 + 2;
-  if (kRev) {
-    load_lam(y0 - 1, la[1]);   // becomes row r-3 at r = y0 + 2
-    load_lam(y0, la[0]);
+// One reverse step of every instance: step index k of the run (its logged
+// control block), segment slot `slot` (q_n; slot+1 holds q_{n+1}), adjoint
+// ping-pong index par (lam[par] -> lam[par^1]).
+template <bool kCheck>
+__global__ __launch_bounds__(64) void adj_reverse_kernel(const AdjDesc* __restrict__ ds,
+                                                         const AdjItem* __restrict__ items,
+                                                         int n_items, int k, int slot, int par) {
+  // XCD-aware order, as the step kernel: consecutive items of a strip on one XCD
+  int item;
+  {
+    const int b = blockIdx.x;
+    const int q = n_items >> 3, rem = n_items & 7, x = b & 7, kk = b >> 3;
+    item = x * q + min(x, rem) + kk;
   }
-  Cell nx0 = load(y0 - 3);     // q_b rows are fetched one iteration ahead
-  for (int r = y0 - 3; r <= r_end; ++r) {
-    // ---- q_b row r, y interface r, y limiter at r-1 ----
-    b2 = b1; b1 = b0;
-    b0 = nx0;
-    nx0 = load(r + 1);
-    f2 = f1; f1 = f0;
-    if (r >= y0 - 2) y_waves(b1, b0, beta, f0);
-    if (r < y0) continue;
-    float Py[3], My[3];
-    {
-      fvad::Lim l;
-      fvad::limiter_fw(f2.W, f1.W, f1.s, f0.W, dty, l, Py, My);
-    }
-    // q_y(m = r-2) = q_b(m) - dty (P(m) + M(m+1)), y component order (rho', rho v, rho u)
-    const int m = r - 2;
-    float yq[3];
-    {
-      const bool upd = col_int && m >= 2 && m <= ny - 3;
-      const float du = upd ? dty : 0.0f;
-      yq[0] = b2.r - du * (Pp[0] + My[0]);
-      yq[2] = b2.v - du * (Pp[1] + My[1]);
-      yq[1] = b2.u - du * (Pp[2] + My[2]);
-    }
-    Pp[0] = Py[0]; Pp[1] = Py[1]; Pp[2] = Py[2];
-    if (r < y0 + 1) continue;
-    if (kRev && r < y0 + 2) {
-      float nl[3];
-      load_lam(r, nl);   // keep the lam window aligned (row r - 1 next)
-      la[2][0] = la[1][0]; la[2][1] = la[1][1]; la[2][2] = la[1][2];
-      la[1][0] = la[0][0]; la[1][1] = la[0][1]; la[1][2] = la[0][2];
-      la[0][0] = nl[0]; la[0][1] = nl[1]; la[0][2] = nl[2];
-      continue;
-    }
-
-    // ---- x sweep on row m (x component order: rho', rho u, rho v) ----
-    fvad::RW w;
-    fvad::riemann_fw(shr1(yq[0]), shr1(yq[1]), shr1(yq[2]), yq[0], yq[1], yq[2], beta, w);
-    float Wl[7], Wr[7];
-#pragma unroll
-    for (int c = 0; c < 7; ++c) { Wl[c] = shr1(w.W[c]); Wr[c] = shl1(w.W[c]); }
-    fvad::Lim lx;
-    float Px[3], Mx[3];
-    fvad::limiter_fw(Wl, w.W, w.s, Wr, dtx, lx, Px, Mx);
-    float xq[3];
-    {
-      const bool upd = col_int && m >= 2 && m <= ny - 3;
-      const float du = upd ? dtx : 0.0f;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) xq[c] = yq[c] - du * (Px[c] + shl1(Mx[c]));
-    }
-
-    if (!kRev) {
-      // ---- source on row m-1 (apps/na/source.h:84-104), store q_{n+1} ----
-      const float irho = fvad::rcp(1.0f + xq[0]);
-      const float u0 = xq[1] * irho, v0 = xq[2] * irho;
-      // cross-lane moves with every lane active (a disabled source lane reads 0)
-      const float uw = shr1(um1), ue = shl1(um1), vw = shr1(vm1), ve = shl1(vm1);
-      if (r >= y0 + 3 && out_lane) {
-        const int s = m - 1;
-        float o1 = xm1.u, o2 = xm1.v;
-        if (col_int && s >= 2 && s <= ny - 3) {
-          o1 = xm1.u + ddx2 * (uw - 2.0f * um1 + ue) + ddy2 * (um2 - 2.0f * um1 + u0);
-          o2 = xm1.v + ddx2 * (vw - 2.0f * vm1 + ve) + ddy2 * (vm2 - 2.0f * vm1 + v0);
-        }
-        float* out = d.seg + (size_t)(slot + 1) * 3 * n;
-        const int o = i + s * nx;
-        out[o] = xm1.r;
-        out[n + o] = o1;
-        out[2 * n + o] = o2;
-      }
-      xm2 = xm1; um2 = um1; vm2 = vm1;
-      xm1.r = xq[0]; xm1.u = xq[1]; xm1.v = xq[2];
-      um1 = u0; vm1 = v0;
-      continue;
-    } else {
-      // ---- lam window: rows m-1 (la[2]), m (la[1]), m+1 (la[0]); prefetch m+2 ----
-      float nl[3];
-      load_lam(r, nl);
-      // ---- S^T at row m ----
-      const bool in_box = in_box_col && m >= d.by0 && m < d.by0 + d.bny;
-      float g0 = la[1][0];
-      if (in_box) {
-        g0 += (float)(2.0 * dt) * xq[0];
-        jacc += (double)xq[0] * (double)xq[0];
-      }
-      const bool cint = col_int && m >= 2 && m <= ny - 3;
-      const bool ilo = col_int && m - 1 >= 2 && m - 1 <= ny - 3;
-      const bool ihi = col_int && m + 1 >= 2 && m + 1 <= ny - 3;
-      const float mu1 = cint ? la[1][1] : 0.0f, mu2 = cint ? la[1][2] : 0.0f;
-      const float cc = -2.0f * (ddx2 + ddy2);
-      const float gU = cc * mu1 + ddx2 * (shr1(mu1) + shl1(mu1)) +
-                       ddy2 * ((ilo ? la[2][1] : 0.0f) + (ihi ? la[0][1] : 0.0f));
-      const float gV = cc * mu2 + ddx2 * (shr1(mu2) + shl1(mu2)) +
-                       ddy2 * ((ilo ? la[2][2] : 0.0f) + (ihi ? la[0][2] : 0.0f));
-      const float irho = fvad::rcp(1.0f + xq[0]);
-      const float u = xq[1] * irho, v = xq[2] * irho;
-      float lxy[3];
-      lxy[0] = g0 - (gU * u + gV * v) * irho;
-      lxy[1] = la[1][1] + gU * irho;
-      lxy[2] = la[1][2] + gV * irho;
-
-      // ---- X^T at row m ----
-      // gP(k) = -dtx lam_xy(k) on interior k; gM(k) = -dtx lam_xy(k-1) on interior k-1
-      const bool rint = m >= 2 && m <= ny - 3;
-      const bool own = rint && col_int;
-      const bool lft = rint && (i - 1 >= 2) && (i - 1 <= nx - 3);
-      float gP[3], gM[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        gP[c] = own ? -dtx * lxy[c] : 0.0f;
-        const float ll = shr1(lxy[c]);
-        gM[c] = lft ? -dtx * ll : 0.0f;
-      }
-      float gW[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gs[3] = {0.f, 0.f, 0.f}, gd[3];
-      fvad::limiter_bw(w.W, w.s, lx, dtx, gP, gM, gW, gs, gd);
-      // the dot products W_p(k) . W_p(k +- 1) of the limiters of k-1, k, k+1
-      const int nb = lx.nb, nbl = shr1i(nb), nbr = shl1i(nb);
-      float gdl[3], gdr[3];
-#pragma unroll
-      for (int p = 0; p < 3; ++p) { gdl[p] = shr1(gd[p]); gdr[p] = shl1(gd[p]); }
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        const bool right = (nb >> p) & 1;
-        // own limiter: its upwind neighbour
-        const float go = gd[p];
-        // limiter of k-1 used W(k) when its upwind side was +1
-        const float gl = ((nbl >> p) & 1) ? gdl[p] : 0.0f;
-        // limiter of k+1 used W(k) when its upwind side was -1
-        const float gr = ((nbr >> p) & 1) ? 0.0f : gdr[p];
-        if (p < 2) {
-          const int b = 3 * p;
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            gW[b + c] += go * (right ? Wr[b + c] : Wl[b + c]) + gl * Wl[b + c] + gr * Wr[b + c];
-        } else {
-          gW[6] += go * (right ? Wr[6] : Wl[6]) + gl * Wl[6] + gr * Wr[6];
-        }
-      }
-      float gL[3] = {0.f, 0.f, 0.f}, gR[3] = {0.f, 0.f, 0.f};
-      fvad::riemann_bw(w, beta, gW, gs, gL, gR);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        gL[c] = iface_x ? gL[c] : 0.0f;
-        gR[c] = iface_x ? gR[c] : 0.0f;
-      }
-      float ly[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) ly[c] = lxy[c] + gR[c] + shl1(gL[c]);
-      if (out_lane) {
-        const int o = i + m * nx;
-        d.lamx[o] = ly[0];
-        d.lamx[n + o] = ly[1];
-        d.lamx[2 * n + o] = ly[2];
-      }
-      la[2][0] = la[1][0]; la[2][1] = la[1][1]; la[2][2] = la[1][2];
-      la[1][0] = la[0][0]; la[1][1] = la[0][1]; la[1][2] = la[0][2];
-      la[0][0] = nl[0]; la[0][1] = nl[1]; la[0][2] = nl[2];
-    }
-  }
-  if (kRev) {
-    // J of this step: the wave's box cells, one double atomic per wave
-    const bool any = __any(jacc != 0.0);
-    if (any) {
-      double v = out_lane ? jacc : 0.0;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      if (lane == 0) atomicAdd(&d.jstep[k], dt * v);
-    }
-  }
-}
-
-// ------------------------------------------------------------ y march --
-// lam = B^T Y^T lam_y (d.lamx -> d.lam), drive gradient of the step.
-__global__ __launch_bounds__(64) void adj_march_y(const AdjDesc* __restrict__ ds,
-                                                  const AdjItem* __restrict__ items, int k,
-                                                  int slot) {
-  constexpr int kHalo = 2;
-  const AdjItem it = items[blockIdx.x];
-  const AdjDesc& d = ds[it.inst];
-  const int nx = d.nx, ny = d.ny, n = nx * ny;
-  const float* q = d.seg + (size_t)slot * 3 * n;
-  const double dt = d.dt_log[k];
-  const double T = d.T_log[k];
-  const float dty = (float)(dt / d.dy);
-  const float beta = d.beta;
-  const int lane = threadIdx.x;
-  const int i = it.x0 - kHalo + lane;
-  const bool col_int = i >= 2 && i <= nx - 3;
-  const bool out_lane = lane >= kHalo && lane < kLanes - kHalo && i < nx;
-  const int y0 = it.y0, y1 = it.y1;
-  const Loader load = make_loader(d, q, T, i);
-  const int bc = d.bc;
-
-  // windows: q_b rows r..r-3; y interfaces r..r-3; limiter adjoint of r-2, r-3
-  Cell b0{}, b1{}, b2{}, b3{};
-  YIf f0{}, f1{}, f2{}, f3{};
-  float gWo2[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gs2[3] = {0.f, 0.f, 0.f};
-  float gd2[3] = {0.f, 0.f, 0.f}, gd3[3] = {0.f, 0.f, 0.f};
-  int nb2 = 0, nb3 = 0;
-  float gRp[3] = {0.f, 0.f, 0.f};   // gR of interface r-3 (cell r-3)
-  // lam_y rows r (prefetched), r-1, r-2, r-3
-  float ly[4][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-  // held lam_b rows: 0 and 1 (bottom rule, for row 2), ny-3 (waits for the
-  // top rule) and the sum of rows ny-2 and ny-1 (top rule, for row ny-3);
-  // only the bottom and top items of a strip ever fill them
-  float lb0[3] = {0.f, 0.f, 0.f}, lb1[3] = {0.f, 0.f, 0.f};
-  float lbt[3] = {0.f, 0.f, 0.f}, ltop[3] = {0.f, 0.f, 0.f};
-
-  auto load_ly = [&](int row, float out[3]) {
-    if (row < 0 || row >= ny || i < 0 || i >= nx) {
-      out[0] = out[1] = out[2] = 0.0f;
-      return;
-    }
-    const int o = i + row * nx;
-    out[0] = d.lamx[o]; out[1] = d.lamx[n + o]; out[2] = d.lamx[2 * n + o];
-  };
-
-  // B^T of row j from lam_b(j) (fields rho', rho u, rho v) and the held rows
-  auto emit = [&](int j, const float lbj[3]) {
-    const bool ghost = i <= 1 || j <= 1 || i >= nx - 2 || j >= ny - 2;
-    float v[3];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const float l1 = shr1(lbj[f]);                // lane i-1
-      const float l2 = shr1(l1);                    // lane i-2
-      const float r1 = shl1(lbj[f]);                // lane i+1
-      const float r2 = shl1(r1);                    // lane i+2
-      float a = ghost ? 0.0f : lbj[f];
-      if (bc != 2) {
-        if (i == 2 && bc == 0) a += l2 + l1;                               // left rule
-        if (j == 2 && i >= 2) a += lb0[f] + lb1[f];                        // bottom rule
-        if (j == ny - 3 && i >= 2) a += ltop[f];                           // top rule
-        if (i == nx - 3 && j >= 2 && j <= ny - 3) a += r1 + r2;            // right rule
-      }
-      v[f] = a;
-    }
-    if (out_lane) {
-      const int o = i + j * nx;
-      d.lam[o] = v[0];
-      d.lam[n + o] = v[1];
-      d.lam[2 * n + o] = v[2];
-    }
-    // drive parameters: the left ghost cells hold p, p(p+1), 0 with
-    // p = amp sin(omega T + phase)
-    if (bc == 1 && it.x0 == 0) {
-      const double th = d.omega * T + (double)d.edge_phase[j];
-      const double sn = sin(th), cs = cos(th);
-      const double p = (double)d.edge_amp[j] * sn;
-      const bool gl = i >= 0 && i <= 1;
-      const double t = gl ? (double)lbj[0] + (double)lbj[1] * (2.0 * p + 1.0) : 0.0;
-      const double g = __shfl(t, kHalo, 64) + __shfl(t, kHalo + 1, 64);
-      if (lane == 0) {
-        d.grad_amp[j] += g * sn;
-        d.grad_phase[j] += g * (double)d.edge_amp[j] * cs;
-      }
-    }
-  };
-
-  Cell nx0 = load(y0 - 3);     // q_b rows are fetched one iteration ahead
-  for (int r = y0 - 3; r <= y1 + 2; ++r) {
-    b3 = b2; b2 = b1; b1 = b0;
-    b0 = nx0;
-    nx0 = load(r + 1);
-    f3 = f2; f2 = f1; f1 = f0;
-    if (r >= y0 - 2) y_waves(b1, b0, beta, f0);
-    // lam_y window: ly[3] = row r-3, ly[2] = r-2, ly[1] = r-1, ly[0] = r
-    // (row r is first used one iteration later)
-    {
-      float nl[3];
-      load_ly(r, nl);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        ly[3][c] = ly[2][c]; ly[2][c] = ly[1][c]; ly[1][c] = ly[0][c]; ly[0][c] = nl[c];
-      }
-    }
-    if (r < y0) continue;
-
-    // ---- limiter and its adjoint at interface r-1 ----
-    float gWo1[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gs1[3] = {0.f, 0.f, 0.f}, gd1[3];
-    int nb1;
-    {
-      fvad::Lim l;
-      float P[3], M[3];
-      fvad::limiter_fw(f2.W, f1.W, f1.s, f0.W, dty, l, P, M);
-      const int j = r - 1;
-      const bool own = col_int && j >= 2 && j <= ny - 3;           // P(j) updates cell j
-      const bool blw = col_int && j - 1 >= 2 && j - 1 <= ny - 3;   // M(j) updates cell j-1
-      // y component order (rho', rho v, rho u) -> lam_y fields (0, 2, 1)
-      float gP[3], gM[3];
-      gP[0] = own ? -dty * ly[1][0] : 0.0f;
-      gP[1] = own ? -dty * ly[1][2] : 0.0f;
-      gP[2] = own ? -dty * ly[1][1] : 0.0f;
-      gM[0] = blw ? -dty * ly[2][0] : 0.0f;
-      gM[1] = blw ? -dty * ly[2][2] : 0.0f;
-      gM[2] = blw ? -dty * ly[2][1] : 0.0f;
-      fvad::limiter_bw(f1.W, f1.s, l, dty, gP, gM, gWo1, gs1, gd1);
-      nb1 = l.nb;
-    }
-    if (r >= y0 + 2) {
-      // ---- total adjoint of the waves of interface r-2, then its Riemann^T ----
-      float gW[7], gs[3];
-#pragma unroll
-      for (int c = 0; c < 7; ++c) gW[c] = gWo2[c];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) gs[c] = gs2[c];
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        const bool right = (nb2 >> p) & 1;
-        const float go = gd2[p];
-        const float gl = ((nb3 >> p) & 1) ? gd3[p] : 0.0f;   // limiter of r-3 looked up (+1)
-        const float gr = ((nb1 >> p) & 1) ? 0.0f : gd1[p];   // limiter of r-1 looked down (-1)
-        if (p < 2) {
-          const int b = 3 * p;
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            gW[b + c] += go * (right ? f1.W[b + c] : f3.W[b + c]) + gl * f3.W[b + c] +
-                         gr * f1.W[b + c];
-        } else {
-          gW[6] += go * (right ? f1.W[6] : f3.W[6]) + gl * f3.W[6] + gr * f1.W[6];
-        }
-      }
-      fvad::RW w;
-      fvad::riemann_fw(b3.r, b3.v, b3.u, b2.r, b2.v, b2.u, beta, w);
-      float gL[3] = {0.f, 0.f, 0.f}, gR[3] = {0.f, 0.f, 0.f};
-      const int j = r - 2;
-      if (j >= 1 && j <= ny - 1) fvad::riemann_bw(w, beta, gW, gs, gL, gR);
-      if (r >= y0 + 3) {
-        // lam_b(r-3) = lam_y(r-3) + gR(interface r-3) + gL(interface r-2)
-        const int jj = r - 3;
-        float lb[3];
-        lb[0] = ly[3][0] + gRp[0] + gL[0];
-        lb[2] = ly[3][2] + gRp[1] + gL[1];
-        lb[1] = ly[3][1] + gRp[2] + gL[2];
-        if (jj == 0) { lb0[0] = lb[0]; lb0[1] = lb[1]; lb0[2] = lb[2]; }
-        if (jj == 1) { lb1[0] = lb[0]; lb1[1] = lb[1]; lb1[2] = lb[2]; }
-        if (jj == ny - 3) { lbt[0] = lb[0]; lbt[1] = lb[1]; lbt[2] = lb[2]; }
-        if (jj >= ny - 2) { ltop[0] += lb[0]; ltop[1] += lb[1]; ltop[2] += lb[2]; }
-        if (jj != ny - 3) emit(jj, lb);
-        // row ny-3 waited for the top ghost rows ny-2 and ny-1
-        if (jj == ny - 1) emit(ny - 3, lbt);
-      }
-      gRp[0] = gR[0]; gRp[1] = gR[1]; gRp[2] = gR[2];
-    }
-    // shift the limiter-adjoint window
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { gd3[c] = gd2[c]; gd2[c] = gd1[c]; }
-    nb3 = nb2; nb2 = nb1;
-#pragma unroll
-    for (int c = 0; c < 7; ++c) gWo2[c] = gWo1[c];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) gs2[c] = gs1[c];
+  const AdjItem it = items[item];
+  const AdjDesc& D = ds[it.inst];
+  switch (__builtin_amdgcn_readfirstlane(D.bc)) {
+    case kBcZeroGrad:
+      reverse_item<fvbc::ZeroGradient, kCheck>(D, it, item, k, slot, par);
+      break;
+    case kBcEdgeDrive:
+      reverse_item<fvbc::EdgeDrive, kCheck>(D, it, item, k, slot, par);
+      break;
+    case kBcTaylor:
+      reverse_item<fvbc::Taylor, kCheck>(D, it, item, k, slot, par);
+      break;
+    default:
+      break;
   }
 }
 
@@ -531,28 +519,21 @@ __global__ __launch_bounds__(64) void adj_march_y(const AdjDesc* __restrict__ ds
 
 // ---------------------------------------------------------------- launches --
 
-extern "C" hipError_t fv_adj_march_forward(const fv::AdjDesc* ds, const fv::AdjItem* items,
-                                           int n_items, int k, int slot, hipStream_t s) {
+extern "C" hipError_t fv_adj_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items,
+                                     int n_items, int k, int slot, int par, int check,
+                                     hipStream_t s) {
   if (n_items <= 0) return hipSuccess;
-  hipLaunchKernelGGL(fv::adj_march_x<false>, dim3(n_items), dim3(64), 0, s, ds, items, k, slot);
+  if (check)
+    hipLaunchKernelGGL(fv::adj_reverse_kernel<true>, dim3(n_items), dim3(64), 0, s, ds, items,
+                       n_items, k, slot, par);
+  else
+    hipLaunchKernelGGL(fv::adj_reverse_kernel<false>, dim3(n_items), dim3(64), 0, s, ds, items,
+                       n_items, k, slot, par);
   return hipGetLastError();
 }
 
-extern "C" hipError_t fv_adj_march_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items_x,
-                                           int n_items_x, const fv::AdjItem* items_y,
-                                           int n_items_y, int k, int slot, hipStream_t s) {
-  if (n_items_x <= 0 || n_items_y <= 0) return hipSuccess;
-  hipLaunchKernelGGL(fv::adj_march_x<true>, dim3(n_items_x), dim3(64), 0, s, ds, items_x, k,
-                     slot);
-  hipLaunchKernelGGL(fv::adj_march_y, dim3(n_items_y), dim3(64), 0, s, ds, items_y, k, slot);
-  return hipGetLastError();
-}
-
-extern "C" hipError_t fv_adj_march_occupancy(int* x_fwd, int* x_rev, int* y_rev) {
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(x_fwd, fv::adj_march_x<false>, 64, 0);
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(x_rev, fv::adj_march_x<true>, 64, 0);
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(y_rev, fv::adj_march_y, 64, 0);
-  return e;
+// Resident reverse waves per CU.
+extern "C" hipError_t fv_adj_occupancy(int* waves_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(waves_per_cu, fv::adj_reverse_kernel<false>,
+                                                      64, 0);
 }

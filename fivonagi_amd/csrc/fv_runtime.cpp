@@ -30,12 +30,10 @@ extern "C" hipError_t fv_launch_settle(const fv::InstDesc* insts, int n_inst, in
 extern "C" hipError_t fv_launch_collect(const fv::InstDesc* insts, int n_inst, int max_nx,
                                         int phase, int fp64, hipStream_t stream);
 extern "C" hipError_t fv_step_waves_per_cu(int* waves, int fp64);
-extern "C" hipError_t fv_adj_march_forward(const fv::AdjDesc* ds, const fv::AdjItem* items,
-                                           int n_items, int k, int slot, hipStream_t s);
-extern "C" hipError_t fv_adj_march_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items_x,
-                                           int n_items_x, const fv::AdjItem* items_y,
-                                           int n_items_y, int k, int slot, hipStream_t s);
-extern "C" hipError_t fv_adj_march_occupancy(int* x_fwd, int* x_rev, int* y_rev);
+extern "C" hipError_t fv_adj_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items,
+                                     int n_items, int k, int slot, int par, int check,
+                                     hipStream_t s);
+extern "C" hipError_t fv_adj_occupancy(int* waves_per_cu);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, int phase, hipStream_t stream);
 extern "C" hipError_t fv_scratch_get(void** ptr, size_t* bytes);
@@ -1244,100 +1242,114 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   K = (int)std::min<int64_t>(K, nsteps);
   const int64_t ncp = (nsteps + K - 1) / K;
   const bool plain = opts && opts->plain_forward;
+  const bool check = opts && opts->check_recompute;
 
   std::vector<fv::Ctl> ctl;
   if (int rc = read_ctls(c, ctl)) return rc;
   for (int k = 0; k < ni; ++k)
     if (ctl[k].err) return fivonagi_sync(c);
 
+  // ---- work items of the reverse march: strips of 56 output columns, rows
+  // split so that every wave of a step is resident at once ----
+  std::vector<fv::AdjItem> items;
+  std::vector<size_t> first_item(ni + 1, 0);
+  {
+    int occ = 0, cus = 0;
+    if (fv_adj_occupancy(&occ) != hipSuccess || occ <= 0) occ = 2;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    const long target = (long)cus * occ;
+    constexpr int kCols = 56;
+    // strip starts; the last one never falls on nx-2 or nx-1 (column nx-3's
+    // wave also holds nx-2 and nx-1)
+    auto strips = [&](int nx) {
+      std::vector<int> b;
+      for (int x = 0; x < nx; x += kCols) b.push_back(x);
+      if (b.size() >= 2 && nx - b.back() <= 2) b.back() = nx - 3;
+      b.push_back(nx);
+      return b;
+    };
+    int H = 8;
+    for (; H < 4096; ++H) {
+      long waves = 0;
+      for (const auto& p : c->params)
+        waves += (long)(strips(p.nx).size() - 1) * std::max(1L, std::lround((double)p.ny / H));
+      if (waves <= target) break;
+    }
+    for (int k = 0; k < ni; ++k) {
+      const auto& p = c->params[k];
+      const int chunks = std::max(1, (int)std::lround((double)p.ny / H));
+      std::vector<int> rb(chunks + 1);
+      for (int ch = 0; ch <= chunks; ++ch) rb[ch] = (int)((long)p.ny * ch / chunks);
+      // row 2 stays with rows 0, 1 and row ny-3 with ny-2, ny-1
+      for (int ch = 1; ch < chunks; ++ch) {
+        if (rb[ch] <= 2) rb[ch] = 3;
+        if (rb[ch] >= p.ny - 2) rb[ch] = p.ny - 3;
+      }
+      const std::vector<int> sb = strips(p.nx);
+      first_item[k] = items.size();
+      for (size_t s = 0; s + 1 < sb.size(); ++s)
+        for (int ch = 0; ch < chunks; ++ch)
+          if (rb[ch + 1] > rb[ch]) items.push_back({k, sb[s], sb[s], sb[s + 1], rb[ch], rb[ch + 1]});
+    }
+    first_item[ni] = items.size();
+  }
+
+  // ---- device buffers of the call ----
   AdjScratch sc;
   std::vector<fv::AdjDesc> ad(ni);
   std::vector<float*> cps(ni);
-  std::vector<double*> dlog(ni), tlog(ni);
+  std::vector<fv::Ctl*> logd(ni);
+  double* jpart = sc.get<double>(items.size());
+  uint32_t* d_mis = sc.get<uint32_t>(1);
   for (int k = 0; k < ni; ++k) {
     const auto& p = c->params[k];
     const size_t n = (size_t)p.nx * p.ny;
     auto& a = ad[k];
     std::memset(&a, 0, sizeof(a));
-    a.nx = p.nx; a.ny = p.ny; a.bc = p.bc; a.beta = (float)p.beta;
-    a.dx = p.dx; a.dy = p.dy; a.delta = p.delta; a.omega = p.omega;
-    a.tay_rhoa = p.tay_rhoa; a.tay_c = p.tay_c;
-    a.tay_cos = std::cos(p.tay_theta); a.tay_sin = std::sin(p.tay_theta);
-    a.tay_yoff = p.tay_yoff; a.tay_jc = p.tay_jc;
+    a.nx = p.nx; a.ny = p.ny; a.bc = p.bc;
+    a.hbeta = 0.5f * c->desc[k].beta2m2;          // the step kernel's BETA - 1
     a.bx0 = obj[k].box_x0; a.by0 = obj[k].box_y0; a.bnx = obj[k].box_nx; a.bny = obj[k].box_ny;
-    float* ea = sc.get<float>(2 * (size_t)p.ny);
+    a.edge_c = c->desc[k].edge_c;
+    a.edge_s = c->desc[k].edge_s;
     a.seg = sc.get<float>((size_t)(K + 1) * 3 * n);
-    a.lam = sc.get<float>(3 * n);
-    a.lamx = sc.get<float>(3 * n);
-    a.grad_amp = sc.get<double>(p.ny);
-    a.grad_phase = sc.get<double>(p.ny);
-    a.jstep = sc.get<double>(nsteps);
-    dlog[k] = sc.get<double>(nsteps);
-    tlog[k] = sc.get<double>(nsteps);
+    a.lam[0] = sc.get<float>(3 * n);
+    a.lam[1] = sc.get<float>(3 * n);
+    logd[k] = sc.get<fv::Ctl>(nsteps);
+    a.ctl_log = logd[k];
+    a.grad_ec = sc.get<double>(p.ny);
+    a.grad_es = sc.get<double>(p.ny);
+    a.jpart = jpart;
+    a.mismatch = d_mis;
     cps[k] = sc.get<float>((size_t)ncp * 3 * n);
     if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint scratch)");
-    std::vector<float> hv(2 * (size_t)p.ny, 0.f);
-    for (int j = 0; j < p.ny; ++j) {
-      hv[j] = (float)(p.edge_amp ? p.edge_amp[j] : 0.0);
-      hv[p.ny + j] = (float)(p.edge_phase ? p.edge_phase[j] : 0.0);
-    }
-    HIP_TRY(hipMemcpy(ea, hv.data(), sizeof(float) * 2 * p.ny, hipMemcpyHostToDevice));
-    a.edge_amp = ea;
-    a.edge_phase = ea + p.ny;
-    a.dt_log = dlog[k];
-    a.T_log = tlog[k];
-    HIP_TRY(hipMemsetAsync(a.lam, 0, sizeof(float) * 3 * n, c->stream));
-    HIP_TRY(hipMemsetAsync(a.grad_amp, 0, sizeof(double) * p.ny, c->stream));
-    HIP_TRY(hipMemsetAsync(a.grad_phase, 0, sizeof(double) * p.ny, c->stream));
-    HIP_TRY(hipMemsetAsync(a.jstep, 0, sizeof(double) * nsteps, c->stream));
+    HIP_TRY(hipMemsetAsync(a.lam[0], 0, sizeof(float) * 3 * n, c->stream));
+    HIP_TRY(hipMemsetAsync(a.grad_ec, 0, sizeof(double) * p.ny, c->stream));
+    HIP_TRY(hipMemsetAsync(a.grad_es, 0, sizeof(double) * p.ny, c->stream));
+    HIP_TRY(hipMemsetAsync(logd[k], 0, sizeof(fv::Ctl) * nsteps, c->stream));
   }
-  // work items of the march kernels: strips of 56 (x march, 4-column halo)
-  // and 60 (y march, 2-column halo) output columns, rows split so every
-  // kernel's waves fit one resident round of the device
-  std::vector<fv::AdjItem> ix, iy;
-  {
-    int ox = 0, orx = 0, ory = 0, cus = 0;
-    if (fv_adj_march_occupancy(&ox, &orx, &ory) != hipSuccess || orx <= 0 || ory <= 0) orx = ory = 2;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-    auto build = [&](int width, int occ, std::vector<fv::AdjItem>& out) {
-      const long target = (long)cus * occ;
-      int H = 8;
-      for (; H < 4096; ++H) {
-        long waves = 0;
-        for (const auto& p : c->params)
-          waves += (long)((p.nx + width - 1) / width) * std::max(1L, std::lround((double)p.ny / H));
-        if (waves <= target) break;
-      }
-      for (int k = 0; k < ni; ++k) {
-        const auto& p = c->params[k];
-        const int chunks = std::max(1, (int)std::lround((double)p.ny / H));
-        for (int x0 = 0; x0 < p.nx; x0 += width)
-          for (int ch = 0; ch < chunks; ++ch)
-            out.push_back({k, x0, (int)((long)p.ny * ch / chunks),
-                           (int)((long)p.ny * (ch + 1) / chunks)});
-      }
-    };
-    build(56, orx, ix);
-    build(60, ory, iy);
-  }
-  fv::AdjItem* d_ix = sc.get<fv::AdjItem>(ix.size());
-  fv::AdjItem* d_iy = sc.get<fv::AdjItem>(iy.size());
-  if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint work items)");
-  HIP_TRY(hipMemcpyAsync(d_ix, ix.data(), sizeof(fv::AdjItem) * ix.size(), hipMemcpyHostToDevice,
-                         c->stream));
-  HIP_TRY(hipMemcpyAsync(d_iy, iy.data(), sizeof(fv::AdjItem) * iy.size(), hipMemcpyHostToDevice,
-                         c->stream));
+  fv::AdjItem* d_items = sc.get<fv::AdjItem>(items.size());
   fv::AdjDesc* d_ad = sc.get<fv::AdjDesc>(ni);
-  if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint table)");
+  // the step kernel's descriptors and control blocks for the segment
+  // recompute: slot s of every instance, and its logged control block
+  fv::InstDesc* d_rdesc = sc.get<fv::InstDesc>((size_t)K * ni);
+  fv::Ctl* d_rctl = sc.get<fv::Ctl>((size_t)K * ni * 2);
+  fv::SyncSlots* d_rsync = sc.get<fv::SyncSlots>((size_t)3 * ni);
+  if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint tables)");
+  HIP_TRY(hipMemsetAsync(jpart, 0, sizeof(double) * items.size(), c->stream));
+  HIP_TRY(hipMemsetAsync(d_mis, 0, sizeof(uint32_t), c->stream));
+  HIP_TRY(hipMemsetAsync(d_rsync, 0, sizeof(fv::SyncSlots) * 3 * ni, c->stream));
+  HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(fv::AdjItem) * items.size(),
+                         hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(d_ad, ad.data(), sizeof(fv::AdjDesc) * ni, hipMemcpyHostToDevice,
                          c->stream));
 
-  // ---- forward: record dt/T of every accepted step, keep checkpoints ----
+  // ---- forward: log the control block of every accepted step, keep a
+  // checkpoint every K steps ----
   for (int k = 0; k < ni; ++k) {
-    c->desc[k].dt_log = dlog[k];
-    c->desc[k].T_log = tlog[k];
+    c->desc[k].ctl_log = logd[k];
     c->desc[k].log_base = ctl[k].n_acc;
     c->desc[k].log_len = nsteps;
   }
@@ -1362,50 +1374,125 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
       rc = fivonagi_step(c, len);
     }
   }
-  for (int k = 0; k < ni; ++k) {
-    c->desc[k].dt_log = nullptr;
-    c->desc[k].T_log = nullptr;
-  }
+  for (int k = 0; k < ni; ++k) c->desc[k].ctl_log = nullptr;
   int rc2 = upload_tables(c);
   if (rc) return rc;
   if (rc2) return rc2;
+  if (int r = read_ctls(c, ctl)) return r;      // the final state's buffer (cur)
+  std::vector<fv::Ctl> hlog((size_t)ni * nsteps);
+  for (int k = 0; k < ni; ++k)
+    HIP_TRY(hipMemcpyAsync(hlog.data() + (size_t)k * nsteps, logd[k], sizeof(fv::Ctl) * nsteps,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (size_t e = 0; e < hlog.size(); ++e)
+    if (!(hlog[e].dt > 0.0)) return fail(FIVONAGI_ERR_ARG, "adjoint: step log incomplete");
 
-  // ---- reverse, segment by segment ----
+  // ---- reverse, segment by segment: recompute the segment's states with
+  // the step kernel (each step from its logged control block), then march
+  // the adjoint back over them ----
+  std::vector<fv::InstDesc> rdh((size_t)K * ni);
+  std::vector<fv::Ctl> rch((size_t)K * ni * 2);
+  const bool one = ni == 1;
+  int par = 0;
   for (int64_t seg = ncp - 1; seg >= 0; --seg) {
     const int64_t s0 = seg * K;
     const int len = (int)std::min<int64_t>(K, nsteps - s0);
+    // the previous segment's kernels read the descriptors rewritten below
+    HIP_TRY(hipStreamSynchronize(c->stream));
     for (int k = 0; k < ni; ++k) {
       const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
-      HIP_TRY(hipMemcpyAsync(ad[k].seg, cps[k] + (size_t)seg * 3 * n, sizeof(float) * 3 * n,
+      float* sg = ad[k].seg;
+      HIP_TRY(hipMemcpyAsync(sg, cps[k] + (size_t)seg * 3 * n, sizeof(float) * 3 * n,
                              hipMemcpyDeviceToDevice, c->stream));
+      // the state after the segment into slot len (rho' for the objective;
+      // every field when checking the recompute against it)
+      for (int f = 0; f < (check ? 3 : 1); ++f) {
+        const float* src = seg + 1 < ncp ? cps[k] + ((size_t)(seg + 1) * 3 + f) * n
+                                         : c->desc[k].q[ctl[k].cur][f];
+        HIP_TRY(hipMemcpyAsync(sg + ((size_t)len * 3 + f) * n, src, sizeof(float) * n,
+                               hipMemcpyDeviceToDevice, c->stream));
+      }
+      for (int s = 0; s < len; ++s) {
+        fv::InstDesc d = c->desc[k];
+        for (int f = 0; f < 3; ++f) {
+          d.q[0][f] = sg + ((size_t)s * 3 + f) * n;
+          d.q[1][f] = sg + ((size_t)(s + 1) * 3 + f) * n;
+        }
+        d.ctl = d_rctl + 2 * ((size_t)s * ni + k);
+        d.sync = d_rsync + 3 * (size_t)k;
+        d.dt_seq = nullptr;
+        d.dt_seq_len = 0;
+        d.ctl_log = nullptr;
+        d.measure = nullptr;
+        d.stage = nullptr;
+        d.md_on = 0;
+        rdh[(size_t)s * ni + k] = d;
+        // the logged block as the attempt's own (an idle fold keeps it as is)
+        fv::Ctl t = hlog[(size_t)k * nsteps + s0 + s];
+        t.cur = 0;
+        t.err = 0;
+        t.target_acc = INT64_MAX;
+        t.T_stop = INFINITY;
+        rch[2 * ((size_t)s * ni + k)] = t;
+        rch[2 * ((size_t)s * ni + k) + 1] = t;
+      }
     }
-    // recompute the segment's states (the last one is not needed)
-    for (int s = 0; s + 1 < len; ++s)
-      HIP_TRY(fv_adj_march_forward(d_ad, d_ix, (int)ix.size(), (int)(s0 + s), s, c->stream));
-    for (int s = len - 1; s >= 0; --s)
-      HIP_TRY(fv_adj_march_reverse(d_ad, d_ix, (int)ix.size(), d_iy, (int)iy.size(),
-                                   (int)(s0 + s), s, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_rdesc, rdh.data(), sizeof(fv::InstDesc) * len * ni,
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_rctl, rch.data(), sizeof(fv::Ctl) * 2 * len * ni,
+                           hipMemcpyHostToDevice, c->stream));
+    // recompute states 1 .. len-1 of the segment (phase 0: every launch
+    // reads control block 0 and the never-written slot set 2, an idle fold)
+    for (int s = 0; s + 1 < len; ++s) {
+      const fv::InstDesc* di = d_rdesc + (size_t)s * ni;
+      if (c->taylor_ring > 0)
+        HIP_TRY(fv_launch_taylor_ghosts(di, ni, c->taylor_ring, 0, c->stream));
+      HIP_TRY(fv_launch_step(di, c->d_items, c->d_rowmask, (int)c->items.size(), 0, ni,
+                             one ? d_rctl + 2 * (size_t)s : nullptr, one ? d_rsync : nullptr,
+                             c->params[0].bc, c->ext_any ? 1 : 0, 0, c->stream));
+    }
+    // the last state of the segment was not stepped: its Taylor ghost ring
+    if (c->taylor_ring > 0)
+      HIP_TRY(fv_launch_taylor_ghosts(d_rdesc + (size_t)(len - 1) * ni, ni, c->taylor_ring, 0,
+                                      c->stream));
+    for (int s = len - 1; s >= 0; --s) {
+      HIP_TRY(fv_adj_reverse(d_ad, d_items, (int)items.size(), (int)(s0 + s), s, par,
+                             check ? 1 : 0, c->stream));
+      par ^= 1;
+    }
   }
 
   // ---- results ----
+  std::vector<double> jp(items.size());
+  HIP_TRY(hipMemcpyAsync(jp.data(), jpart, sizeof(double) * items.size(), hipMemcpyDeviceToHost,
+                         c->stream));
+  uint32_t mism = 0;
+  HIP_TRY(hipMemcpyAsync(&mism, d_mis, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   size_t goff = 0;
   for (int k = 0; k < ni; ++k) {
     const auto& p = c->params[k];
-    std::vector<double> js(nsteps), ga(p.ny), gp(p.ny);
-    HIP_TRY(hipMemcpyAsync(js.data(), ad[k].jstep, sizeof(double) * nsteps,
+    std::vector<double> gc(p.ny), gs(p.ny);
+    HIP_TRY(hipMemcpyAsync(gc.data(), ad[k].grad_ec, sizeof(double) * p.ny,
                            hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(ga.data(), ad[k].grad_amp, sizeof(double) * p.ny,
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(gp.data(), ad[k].grad_phase, sizeof(double) * p.ny,
+    HIP_TRY(hipMemcpyAsync(gs.data(), ad[k].grad_es, sizeof(double) * p.ny,
                            hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    double acc = 0.0;
-    for (double v : js) acc += v;
+    double acc = 0.0;                       // item order: the same sum every call
+    for (size_t it = first_item[k]; it < first_item[k + 1]; ++it) acc += jp[it];
     J[k] = acc;
-    if (grad_amp) std::memcpy(grad_amp + goff, ga.data(), sizeof(double) * p.ny);
-    if (grad_phase) std::memcpy(grad_phase + goff, gp.data(), sizeof(double) * p.ny);
+    // ampc = amp cos(phase), amps = amp sin(phase) per row
+    for (int j = 0; j < p.ny; ++j) {
+      const double a = c->edge_amp[k].empty() ? 0.0 : c->edge_amp[k][j];
+      const double ph = c->edge_phase[k].empty() ? 0.0 : c->edge_phase[k][j];
+      const double cs = std::cos(ph), sn = std::sin(ph);
+      if (grad_amp) grad_amp[goff + j] = gc[j] * cs + gs[j] * sn;
+      if (grad_phase) grad_phase[goff + j] = a * (gs[j] * cs - gc[j] * sn);
+    }
     goff += p.ny;
   }
+  if (check && mism != 0)
+    return fail(FIVONAGI_ERR_CHECK, "adjoint: " + std::to_string(mism) +
+                                         " recomputed cells differ from the forward's");
   // keep dJ/dq0 for fivonagi_adjoint_state
   for (float* q : c->adj_lam) (void)hipFree(q);
   c->adj_lam.assign(ni, nullptr);
@@ -1413,7 +1500,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
     hipError_t e = hipMalloc(&c->adj_lam[k], sizeof(float) * 3 * n);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint state)");
-    HIP_TRY(hipMemcpyAsync(c->adj_lam[k], ad[k].lam, sizeof(float) * 3 * n,
+    HIP_TRY(hipMemcpyAsync(c->adj_lam[k], ad[k].lam[par], sizeof(float) * 3 * n,
                            hipMemcpyDeviceToDevice, c->stream));
   }
   HIP_TRY(hipStreamSynchronize(c->stream));

@@ -258,8 +258,9 @@ __device__ __forceinline__ int next_cur(const Fold<R>& f) {
 using fvbc::md_shift;
 
 // Control block of attempt a+1 from attempt a's fold: common/main.cu:199-230
-// with a clean retry on rejection.  log: record dt and T of an accepted step
-// (adjoint runs); every writer records the same values.
+// with a clean retry on rejection.  log: record the control block an accepted
+// step ran with (adjoint runs: its recompute and reverse pass replay it); every
+// writer records the same values.
 template <typename R>
 __device__ Ctl advance(const InstDesc& D, const Fold<R>& f, bool log) {
   const Ctl& p = f.p;
@@ -277,12 +278,9 @@ __device__ Ctl advance(const InstDesc& D, const Fold<R>& f, bool log) {
   o.last_cfl = cfl;
   double T = p.T;
   if (acc) {
-    if (log && D.dt_log) {
+    if (log && D.ctl_log && __lane_id() == 0) {
       const int64_t idx = p.n_acc - D.log_base;
-      if (idx >= 0 && idx < D.log_len) {
-        D.dt_log[idx] = p.dt;
-        D.T_log[idx] = p.T;
-      }
+      if (idx >= 0 && idx < D.log_len) D.ctl_log[idx] = p;
     }
     T = p.T + p.dt;
     o.T = T;

@@ -64,8 +64,7 @@ struct InstDesc {
   const void* edge_s;   // amp_j * sin(phase_j), length ny         double (fp64)
   const double* dt_seq; // prescribed dt per attempt, or null
   int64_t dt_seq_len;
-  double* dt_log;       // when set: dt and T of accepted step n_acc - log_base
-  double* T_log;
+  Ctl* ctl_log;         // when set: the control block accepted step n_acc - log_base ran with
   int64_t log_base, log_len;
   double dx, dy, delta, cfl_max, cfl_wish, omega;
   double inv_dx, inv_dy;     // 1/dx, 1/dy

@@ -44,7 +44,9 @@ enum {
 };
 enum {
   FIVONAGI_OK = 0, FIVONAGI_ERR_EIGEN = 2, FIVONAGI_ERR_ARG = 8,
-  FIVONAGI_ERR_DTSEQ = 9, FIVONAGI_ERR_STALL = 10, FIVONAGI_ERR_COLLECT = 4
+  FIVONAGI_ERR_DTSEQ = 9, FIVONAGI_ERR_STALL = 10, FIVONAGI_ERR_COLLECT = 4,
+  FIVONAGI_ERR_CHECK = 11  /* fivonagi_step_adjoint with check_recompute: a
+                              recomputed state differs from the forward's */
 };
 
 /* Per-instance parameters: the runtime form of apps/<app>/parameters.h
@@ -174,9 +176,16 @@ typedef struct {
 
 typedef struct {
   int32_t checkpoint_every;  /* states kept every K steps (0 = ceil(sqrt(nsteps)));
-                                segments of K steps are recomputed backwards  */
+                                segments of K steps are recomputed backwards by
+                                the step kernel, each step from the control
+                                block the forward step ran with            */
   int32_t plain_forward;     /* 1 = forward with one launch per step and a host
                                 readback of dt after each (no graph replay)   */
+  int32_t check_recompute;   /* 1 = the reverse pass also recomputes every
+                                q_{n+1} and compares it bit for bit with the
+                                stored state; any difference returns
+                                FIVONAGI_ERR_CHECK                            */
+  int32_t pad;
 } fivonagi_adjoint_opts;
 
 /* Advance every instance nsteps accepted steps (as fivonagi_step, recording

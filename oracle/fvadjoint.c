@@ -534,36 +534,19 @@ int orc_line_step_f64(const orc_params *p, double *q1, double *q2, double *q3, d
   return rc;
 }
 
-int orc_adjoint_f64(const orc_params *p, const double *q0, int64_t nsteps, const orc_box *bx,
-                    double *J, double *grad_amp, double *grad_phase, double *grad_q0) {
-  if (check(p, nsteps, bx) || !q0 || !J) return ORC_ERR_ARG;
-#ifdef _OPENMP
-  if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
-#endif
-  step_ws s;
-  if (ws_alloc(&s, p->nx, p->ny)) { ws_free(&s); return ORC_ERR_ARG; }
-  size_t n = s.n;
-  /* every state of the run (small test grids only) */
-  double *states = malloc(sizeof(double) * 3 * n * (size_t)(nsteps + 1));
-  double *Ts = malloc(sizeof(double) * (size_t)(nsteps + 1));
+/* The reverse sweep over a stored trajectory: states holds q_0 .. q_nsteps
+   (3 grids each), Ts the start time of each step. */
+static int adjoint_reverse(step_ws *s, const orc_params *p, double *states, const double *Ts,
+                           int64_t nsteps, const orc_box *bx, double *J, double *grad_amp,
+                           double *grad_phase, double *grad_q0) {
+  size_t n = s->n;
   double *lam = calloc(3 * n, sizeof(double)), *lam2 = calloc(3 * n, sizeof(double));
   double *lx = calloc(3 * n, sizeof(double)), *gu = calloc(n, sizeof(double));
   int rc = 0;
-  if (!states || !Ts || !lam || !lam2 || !lx || !gu) { rc = ORC_ERR_ARG; goto out; }
-  memcpy(states, q0, sizeof(double) * 3 * n);
-  Ts[0] = p->T0;
+  if (!lam || !lam2 || !lx || !gu) { rc = ORC_ERR_ARG; goto out; }
   *J = 0.0;
-  for (int64_t k = 0; k < nsteps; ++k) {
-    double *qi[3], *qo[3];
-    for (int f = 0; f < 3; ++f) {
-      qi[f] = states + (size_t)(3 * k + f) * n;
-      qo[f] = states + (size_t)(3 * (k + 1) + f) * n;
-    }
-    rc = step_fw(&s, p, qi, qo, Ts[k], p->dt_seq[k]);
-    if (rc) goto out;
-    Ts[k + 1] = Ts[k] + p->dt_seq[k];
-    *J += p->dt_seq[k] * box_sum(bx, p->nx, qo[0]);
-  }
+  for (int64_t k = 0; k < nsteps; ++k)
+    *J += p->dt_seq[k] * box_sum(bx, p->nx, states + (size_t)(3 * (k + 1)) * n);
   if (grad_amp) memset(grad_amp, 0, sizeof(double) * p->ny);
   if (grad_phase) memset(grad_phase, 0, sizeof(double) * p->ny);
   for (int64_t k = nsteps - 1; k >= 0; --k) {
@@ -579,18 +562,78 @@ int orc_adjoint_f64(const orc_params *p, const double *q0, int64_t nsteps, const
         L[0][o] += 2.0 * dt * qo1[o];
       }
     /* recompute the step's intermediates */
-    boundary_fw(p, qi, s.b, Ts[k]);
-    sweep_lines(&s, p, 1, s.b, s.y, dt / p->dy, 0, NULL, NULL);
-    sweep_lines(&s, p, 0, s.y, s.xy, dt / p->dx, 0, NULL, NULL);
+    boundary_fw(p, qi, s->b, Ts[k]);
+    sweep_lines(s, p, 1, s->b, s->y, dt / p->dy, 0, NULL, NULL);
+    sweep_lines(s, p, 0, s->y, s->xy, dt / p->dx, 0, NULL, NULL);
     /* S^T, X^T, Y^T, B^T */
-    source_bw(p, s.xy, L, L2, dt, gu);                           /* lam2 = S^T lam   */
-    sweep_lines(&s, p, 0, s.y, NULL, dt / p->dx, 1, L2, LX);     /* lx   = X^T lam2  */
-    sweep_lines(&s, p, 1, s.b, NULL, dt / p->dy, 1, LX, L2);     /* lam2 = Y^T lx    */
-    boundary_bw(p, L2, L, Ts[k], grad_amp, grad_phase);          /* lam  = B^T lam2  */
+    source_bw(p, s->xy, L, L2, dt, gu);                           /* lam2 = S^T lam   */
+    sweep_lines(s, p, 0, s->y, NULL, dt / p->dx, 1, L2, LX);      /* lx   = X^T lam2  */
+    sweep_lines(s, p, 1, s->b, NULL, dt / p->dy, 1, LX, L2);      /* lam2 = Y^T lx    */
+    boundary_bw(p, L2, L, Ts[k], grad_amp, grad_phase);           /* lam  = B^T lam2  */
   }
   if (grad_q0) memcpy(grad_q0, lam, sizeof(double) * 3 * n);
 out:
-  free(states); free(Ts); free(lam); free(lam2); free(lx); free(gu);
+  free(lam); free(lam2); free(lx); free(gu);
+  return rc;
+}
+
+int orc_adjoint_f64(const orc_params *p, const double *q0, int64_t nsteps, const orc_box *bx,
+                    double *J, double *grad_amp, double *grad_phase, double *grad_q0) {
+  if (check(p, nsteps, bx) || !q0 || !J) return ORC_ERR_ARG;
+#ifdef _OPENMP
+  if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+  step_ws s;
+  if (ws_alloc(&s, p->nx, p->ny)) { ws_free(&s); return ORC_ERR_ARG; }
+  size_t n = s.n;
+  /* every state of the run (small test grids only) */
+  double *states = malloc(sizeof(double) * 3 * n * (size_t)(nsteps + 1));
+  double *Ts = malloc(sizeof(double) * (size_t)(nsteps + 1));
+  int rc = 0;
+  if (!states || !Ts) { rc = ORC_ERR_ARG; goto out; }
+  memcpy(states, q0, sizeof(double) * 3 * n);
+  Ts[0] = p->T0;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    double *qi[3], *qo[3];
+    for (int f = 0; f < 3; ++f) {
+      qi[f] = states + (size_t)(3 * k + f) * n;
+      qo[f] = states + (size_t)(3 * (k + 1) + f) * n;
+    }
+    rc = step_fw(&s, p, qi, qo, Ts[k], p->dt_seq[k]);
+    if (rc) goto out;
+    Ts[k + 1] = Ts[k] + p->dt_seq[k];
+  }
+  rc = adjoint_reverse(&s, p, states, Ts, nsteps, bx, J, grad_amp, grad_phase, grad_q0);
+out:
+  free(states); free(Ts);
+  ws_free(&s);
+  return rc;
+}
+
+/* The same adjoint linearised along a given trajectory q_0 .. q_nsteps
+   (states: nsteps + 1 states of 3 grids, e.g. another implementation's run
+   with the same dt sequence): the branches it differentiates are those of
+   that trajectory's steps, recomputed here in FP64. */
+int orc_adjoint_path_f64(const orc_params *p, const double *states, int64_t nsteps,
+                         const orc_box *bx, double *J, double *grad_amp, double *grad_phase,
+                         double *grad_q0) {
+  if (check(p, nsteps, bx) || !states || !J) return ORC_ERR_ARG;
+#ifdef _OPENMP
+  if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+  step_ws s;
+  if (ws_alloc(&s, p->nx, p->ny)) { ws_free(&s); return ORC_ERR_ARG; }
+  size_t n = s.n;
+  double *st = malloc(sizeof(double) * 3 * n * (size_t)(nsteps + 1));
+  double *Ts = malloc(sizeof(double) * (size_t)(nsteps + 1));
+  int rc = 0;
+  if (!st || !Ts) { rc = ORC_ERR_ARG; goto out; }
+  memcpy(st, states, sizeof(double) * 3 * n * (size_t)(nsteps + 1));
+  Ts[0] = p->T0;
+  for (int64_t k = 0; k < nsteps; ++k) Ts[k + 1] = Ts[k] + p->dt_seq[k];
+  rc = adjoint_reverse(&s, p, st, Ts, nsteps, bx, J, grad_amp, grad_phase, grad_q0);
+out:
+  free(st); free(Ts);
   ws_free(&s);
   return rc;
 }

@@ -90,6 +90,8 @@ def _load(name):
     lib.orc_line_step_f64.restype = C.c_int
     lib.orc_adjoint_f64.argtypes = [P, dp, C.c_int64, B, dp, dp, dp, dp]
     lib.orc_adjoint_f64.restype = C.c_int
+    lib.orc_adjoint_path_f64.argtypes = [P, dp, C.c_int64, B, dp, dp, dp, dp]
+    lib.orc_adjoint_path_f64.restype = C.c_int
     lib.orc_objective_f64.argtypes = [P, dp, C.c_int64, B, dp]
     lib.orc_objective_f64.restype = C.c_int
     return lib
@@ -286,6 +288,22 @@ def adjoint(prob, q0, nsteps, box):
     rc = lib().orc_adjoint_f64(C.byref(p), _ptr(q, C.c_double), int(nsteps), C.byref(Box(*box)),
                                C.byref(J), _ptr(ga, C.c_double), _ptr(gp, C.c_double),
                                _ptr(gq, C.c_double))
+    return J.value, ga, gp, gq, rc
+
+
+def adjoint_path(prob, states, box):
+    """adjoint() linearised along a given trajectory: states (nsteps + 1, 3,
+    ny, nx), q_0 .. q_nsteps of a run with prob's dt sequence."""
+    p = prob.cparams()
+    st = np.ascontiguousarray(states, np.float64)
+    nsteps = st.shape[0] - 1
+    J = C.c_double()
+    ga = np.zeros(prob.ny)
+    gp = np.zeros(prob.ny)
+    gq = np.zeros((3, prob.ny, prob.nx))
+    rc = lib().orc_adjoint_path_f64(C.byref(p), _ptr(st, C.c_double), int(nsteps),
+                                    C.byref(Box(*box)), C.byref(J), _ptr(ga, C.c_double),
+                                    _ptr(gp, C.c_double), _ptr(gq, C.c_double))
     return J.value, ga, gp, gq, rc
 
 

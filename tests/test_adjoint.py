@@ -152,3 +152,31 @@ def test_state_adjoint_matches_finite_differences(oracle, kind):
         ghost = np.zeros((prob.ny, prob.nx), bool)
         ghost[:2, :] = ghost[-2:, :] = ghost[:, :2] = ghost[:, -2:] = True
         assert not gq[:, ghost].any()
+
+
+def test_adjoint_along_a_given_trajectory_is_the_adjoint(oracle):
+    """orc_adjoint_path_f64 on the trajectory the host adjoint's own forward
+    computes (line_step) is orc_adjoint_f64, bit for bit: the two share the
+    reverse sweep, and the path form only takes the states as given."""
+    rng = np.random.default_rng(3)
+    amp, ph, q0 = setup_case(rng)
+    n = 9
+    dts = np.full(n + 2, 0.04)
+    op = to_oracle(mk(amp, ph), dt_seq=dts)
+    states = [q0.copy()]
+    T = op.T0
+    for k in range(n):
+        q, rc = oracle.line_step(op, states[-1], T, dts[k])
+        assert rc == 0
+        states.append(q)
+        T += dts[k]
+    J, ga, gp, gq, rc = oracle.adjoint(op, q0, n, BOX)
+    J2, ga2, gp2, gq2, rc2 = oracle.adjoint_path(op, np.array(states), BOX)
+    assert rc == 0 and rc2 == 0
+    assert J == J2 and np.array_equal(ga, ga2) and np.array_equal(gp, gp2)
+    assert np.array_equal(gq, gq2)
+    # a different trajectory (a perturbed start, stepped the same way) gives a
+    # different linearisation: the states are used, not recomputed
+    states[0] = states[0] * (1 + 1e-3)
+    J3, ga3, _, gq3, _ = oracle.adjoint_path(op, np.array(states), BOX)
+    assert not np.array_equal(gq3, gq)
