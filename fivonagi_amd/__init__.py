@@ -85,7 +85,7 @@ class NoallocState(C.Structure):
 class AdjointOpts(C.Structure):
     """fivonagi_adjoint_opts (include/fivonagi.h)."""
     _fields_ = [("checkpoint_every", C.c_int32), ("plain_forward", C.c_int32),
-                ("check_recompute", C.c_int32), ("pad", C.c_int32)]
+                ("check_recompute", C.c_int32), ("rows_per_wave", C.c_int32)]
 
 
 _lib = None
@@ -308,7 +308,7 @@ class Context:
         return {k: getattr(i, k) for k, _ in Info._fields_}
 
     def step_adjoint(self, nsteps, boxes, checkpoint_every=0, plain_forward=False,
-                     check_recompute=False):
+                     check_recompute=False, rows_per_wave=0):
         """Forward nsteps (recording each step's control block), then the
         adjoint.  boxes: one (x0, y0, nx, ny) per instance.  check_recompute:
         the reverse pass also recomputes every state and raises
@@ -319,7 +319,7 @@ class Context:
             boxes = [boxes] * n
         objs = (Objective * n)(*[Objective(*b) for b in boxes])
         opts = AdjointOpts(int(checkpoint_every), int(bool(plain_forward)),
-                           int(bool(check_recompute)), 0)
+                           int(bool(check_recompute)), int(rows_per_wave))
         J = np.zeros(n)
         tot = sum(p.ny for p in self.problems)
         ga = np.zeros(tot)

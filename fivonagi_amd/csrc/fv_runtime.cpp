@@ -34,6 +34,8 @@ extern "C" hipError_t fv_adj_reverse(const fv::AdjDesc* ds, const fv::AdjItem* i
                                      int n_items, int k, int slot, int par, int check,
                                      hipStream_t s);
 extern "C" hipError_t fv_adj_occupancy(int* waves_per_cu);
+extern "C" hipError_t fv_launch_hist(const fv::InstDesc* insts, int n_inst, long max_cells,
+                                     int phase, hipStream_t stream);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, int phase, hipStream_t stream);
 extern "C" hipError_t fv_scratch_get(void** ptr, size_t* bytes);
@@ -121,6 +123,8 @@ struct fivonagi_ctx {
   static constexpr int kGraphChunk = 300, kGraphCache = 8;
   int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
   int collect_nx = 0;                      // widest instance with data collection (0: none)
+  long hist_cells = 0;                     // largest instance whose accepted states are kept
+                                           // (adjoint forward pass; 0: none)
   bool unlimited = false;                  // targets set to "run every attempt"
   struct Graph {
     int len;
@@ -399,9 +403,12 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
   hipError_t e = fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(),
                                 phase, c->n_inst, c->d_ctl, c->d_sync, c->params[0].bc,
                                 c->ext_any ? 1 : 0, c->fp64 ? 1 : 0, c->stream);
-  if (e != hipSuccess || c->collect_nx == 0) return e;
-  return fv_launch_collect(c->d_desc, c->n_inst, c->collect_nx, phase, c->fp64 ? 1 : 0,
-                           c->stream);
+  if (e == hipSuccess && c->collect_nx > 0)
+    e = fv_launch_collect(c->d_desc, c->n_inst, c->collect_nx, phase, c->fp64 ? 1 : 0,
+                          c->stream);
+  if (e == hipSuccess && c->hist_cells > 0)
+    e = fv_launch_hist(c->d_desc, c->n_inst, c->hist_cells, phase, c->stream);
+  return e;
 }
 
 // data collection overflow flags (any instance collecting): FIVONAGI_ERR_COLLECT or 0
@@ -1237,10 +1244,19 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
         b.box_x0 + b.box_nx > p.nx || b.box_y0 + b.box_ny > p.ny)
       return fail(FIVONAGI_ERR_ARG, "objective box outside the grid");
   }
-  int K = opts && opts->checkpoint_every > 0 ? opts->checkpoint_every
-                                             : (int)std::ceil(std::sqrt((double)nsteps));
+  // keep every state of the forward run when asked (checkpoint_every = 1),
+  // or by default when they fit in half the free device memory; else keep
+  // one every K steps and recompute the segments between them
+  const int req = opts ? opts->checkpoint_every : 0;
+  size_t hist_bytes = 0;
+  for (const auto& p : c->params)
+    hist_bytes += sizeof(float) * 3 * (size_t)p.nx * p.ny * (size_t)(nsteps + 1);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const bool keep_all = req == 1 || (req <= 0 && hist_bytes <= free_b / 2);
+  int K = keep_all ? 64 : (req > 1 ? req : (int)std::ceil(std::sqrt((double)nsteps)));
   K = (int)std::min<int64_t>(K, nsteps);
-  const int64_t ncp = (nsteps + K - 1) / K;
+  const int64_t ncp = (nsteps + K - 1) / K;   // segments (descriptor batches when keep_all)
   const bool plain = opts && opts->plain_forward;
   const bool check = opts && opts->check_recompute;
 
@@ -1271,8 +1287,8 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
       b.push_back(nx);
       return b;
     };
-    int H = 8;
-    for (; H < 4096; ++H) {
+    int H = opts && opts->rows_per_wave > 0 ? std::max(8, (int)opts->rows_per_wave) : 8;
+    for (; H < 4096 && !(opts && opts->rows_per_wave > 0); ++H) {
       long waves = 0;
       for (const auto& p : c->params)
         waves += (long)(strips(p.nx).size() - 1) * std::max(1L, std::lround((double)p.ny / H));
@@ -1302,6 +1318,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   std::vector<fv::AdjDesc> ad(ni);
   std::vector<float*> cps(ni);
   std::vector<fv::Ctl*> logd(ni);
+  std::vector<float*> hist(ni, nullptr);
   double* jpart = sc.get<double>(items.size());
   uint32_t* d_mis = sc.get<uint32_t>(1);
   for (int k = 0; k < ni; ++k) {
@@ -1314,7 +1331,10 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     a.bx0 = obj[k].box_x0; a.by0 = obj[k].box_y0; a.bnx = obj[k].box_nx; a.bny = obj[k].box_ny;
     a.edge_c = c->desc[k].edge_c;
     a.edge_s = c->desc[k].edge_s;
-    a.seg = sc.get<float>((size_t)(K + 1) * 3 * n);
+    if (keep_all)
+      hist[k] = a.seg = sc.get<float>((size_t)(nsteps + 1) * 3 * n);
+    else
+      a.seg = sc.get<float>((size_t)(K + 1) * 3 * n);
     a.lam[0] = sc.get<float>(3 * n);
     a.lam[1] = sc.get<float>(3 * n);
     logd[k] = sc.get<fv::Ctl>(nsteps);
@@ -1323,7 +1343,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     a.grad_es = sc.get<double>(p.ny);
     a.jpart = jpart;
     a.mismatch = d_mis;
-    cps[k] = sc.get<float>((size_t)ncp * 3 * n);
+    if (!keep_all) cps[k] = sc.get<float>((size_t)ncp * 3 * n);
     if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint scratch)");
     HIP_TRY(hipMemsetAsync(a.lam[0], 0, sizeof(float) * 3 * n, c->stream));
     HIP_TRY(hipMemsetAsync(a.grad_ec, 0, sizeof(double) * p.ny, c->stream));
@@ -1346,15 +1366,40 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   HIP_TRY(hipMemcpyAsync(d_ad, ad.data(), sizeof(fv::AdjDesc) * ni, hipMemcpyHostToDevice,
                          c->stream));
 
-  // ---- forward: log the control block of every accepted step, keep a
-  // checkpoint every K steps ----
+  // ---- forward: log the control block of every accepted step; keep every
+  // state (the step launches are followed by fv_hist_kernel) or a checkpoint
+  // every K steps ----
+  long max_cells = 0;
   for (int k = 0; k < ni; ++k) {
     c->desc[k].ctl_log = logd[k];
     c->desc[k].log_base = ctl[k].n_acc;
     c->desc[k].log_len = nsteps;
+    c->desc[k].hist = hist[k];
+    max_cells = std::max(max_cells, (long)c->params[k].nx * c->params[k].ny);
   }
   int rc = upload_tables(c);
-  for (int64_t cp = 0; cp < ncp && rc == 0; ++cp) {
+  if (keep_all && rc == 0) {
+    for (int k = 0; k < ni && rc == 0; ++k) {
+      const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
+      for (int f = 0; f < 3 && rc == 0; ++f) {
+        hipError_t e = hipMemcpyAsync(hist[k] + (size_t)f * n, c->desc[k].q[ctl[k].cur][f],
+                                      sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "state history copy");
+      }
+    }
+    c->hist_cells = max_cells;
+    c->drop_graph();            // the captured launches gain the history copy
+    if (rc == 0) {
+      if (plain) {
+        for (int64_t s = 0; s < nsteps && rc == 0; ++s) rc = fivonagi_step(c, 1);
+      } else {
+        rc = fivonagi_step(c, nsteps);
+      }
+    }
+    c->hist_cells = 0;
+    c->drop_graph();
+  }
+  for (int64_t cp = 0; cp < ncp && rc == 0 && !keep_all; ++cp) {
     if ((rc = read_ctls(c, ctl))) break;
     for (int k = 0; k < ni; ++k) {
       const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
@@ -1374,7 +1419,10 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
       rc = fivonagi_step(c, len);
     }
   }
-  for (int k = 0; k < ni; ++k) c->desc[k].ctl_log = nullptr;
+  for (int k = 0; k < ni; ++k) {
+    c->desc[k].ctl_log = nullptr;
+    c->desc[k].hist = nullptr;
+  }
   int rc2 = upload_tables(c);
   if (rc) return rc;
   if (rc2) return rc2;
@@ -1388,8 +1436,8 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     if (!(hlog[e].dt > 0.0)) return fail(FIVONAGI_ERR_ARG, "adjoint: step log incomplete");
 
   // ---- reverse, segment by segment: recompute the segment's states with
-  // the step kernel (each step from its logged control block), then march
-  // the adjoint back over them ----
+  // the step kernel (each step from its logged control block) unless every
+  // state was kept, then march the adjoint back over them ----
   std::vector<fv::InstDesc> rdh((size_t)K * ni);
   std::vector<fv::Ctl> rch((size_t)K * ni * 2);
   const bool one = ni == 1;
@@ -1399,18 +1447,23 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     const int len = (int)std::min<int64_t>(K, nsteps - s0);
     // the previous segment's kernels read the descriptors rewritten below
     HIP_TRY(hipStreamSynchronize(c->stream));
+    // the reverse kernel's slot of local step s: its own slot of the
+    // segment buffer, or step s0 + s of the kept states
+    const int64_t slot0 = keep_all ? s0 : 0;
     for (int k = 0; k < ni; ++k) {
       const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
-      float* sg = ad[k].seg;
-      HIP_TRY(hipMemcpyAsync(sg, cps[k] + (size_t)seg * 3 * n, sizeof(float) * 3 * n,
-                             hipMemcpyDeviceToDevice, c->stream));
-      // the state after the segment into slot len (rho' for the objective;
-      // every field when checking the recompute against it)
-      for (int f = 0; f < (check ? 3 : 1); ++f) {
-        const float* src = seg + 1 < ncp ? cps[k] + ((size_t)(seg + 1) * 3 + f) * n
-                                         : c->desc[k].q[ctl[k].cur][f];
-        HIP_TRY(hipMemcpyAsync(sg + ((size_t)len * 3 + f) * n, src, sizeof(float) * n,
+      float* sg = ad[k].seg + (size_t)slot0 * 3 * n;
+      if (!keep_all) {
+        HIP_TRY(hipMemcpyAsync(sg, cps[k] + (size_t)seg * 3 * n, sizeof(float) * 3 * n,
                                hipMemcpyDeviceToDevice, c->stream));
+        // the state after the segment into slot len (rho' for the objective;
+        // every field when checking the recompute against it)
+        for (int f = 0; f < (check ? 3 : 1); ++f) {
+          const float* src = seg + 1 < ncp ? cps[k] + ((size_t)(seg + 1) * 3 + f) * n
+                                           : c->desc[k].q[ctl[k].cur][f];
+          HIP_TRY(hipMemcpyAsync(sg + ((size_t)len * 3 + f) * n, src, sizeof(float) * n,
+                                 hipMemcpyDeviceToDevice, c->stream));
+        }
       }
       for (int s = 0; s < len; ++s) {
         fv::InstDesc d = c->desc[k];
@@ -1443,7 +1496,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
                            hipMemcpyHostToDevice, c->stream));
     // recompute states 1 .. len-1 of the segment (phase 0: every launch
     // reads control block 0 and the never-written slot set 2, an idle fold)
-    for (int s = 0; s + 1 < len; ++s) {
+    for (int s = 0; s + 1 < len && !keep_all; ++s) {
       const fv::InstDesc* di = d_rdesc + (size_t)s * ni;
       if (c->taylor_ring > 0)
         HIP_TRY(fv_launch_taylor_ghosts(di, ni, c->taylor_ring, 0, c->stream));
@@ -1451,13 +1504,14 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
                              one ? d_rctl + 2 * (size_t)s : nullptr, one ? d_rsync : nullptr,
                              c->params[0].bc, c->ext_any ? 1 : 0, 0, c->stream));
     }
-    // the last state of the segment was not stepped: its Taylor ghost ring
-    if (c->taylor_ring > 0)
-      HIP_TRY(fv_launch_taylor_ghosts(d_rdesc + (size_t)(len - 1) * ni, ni, c->taylor_ring, 0,
-                                      c->stream));
     for (int s = len - 1; s >= 0; --s) {
-      HIP_TRY(fv_adj_reverse(d_ad, d_items, (int)items.size(), (int)(s0 + s), s, par,
-                             check ? 1 : 0, c->stream));
+      // q_n as the step read it: the Taylor ghost ring of its T (a state the
+      // recompute stepped has it already; writing it again changes nothing)
+      if (c->taylor_ring > 0)
+        HIP_TRY(fv_launch_taylor_ghosts(d_rdesc + (size_t)s * ni, ni, c->taylor_ring, 0,
+                                        c->stream));
+      HIP_TRY(fv_adj_reverse(d_ad, d_items, (int)items.size(), (int)(s0 + s),
+                             (int)(slot0 + s), par, check ? 1 : 0, c->stream));
       par ^= 1;
     }
   }

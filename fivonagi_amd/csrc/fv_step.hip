@@ -821,7 +821,48 @@ __global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __
   }
 }
 
+// The state after attempt a, when the attempt was accepted, into the
+// instance's state history (adjoint runs that keep every state instead of
+// recomputing segments): slot n_acc + 1 - log_base.  Run after the step
+// launch of attempt a; blockIdx.y = instance, threads cover the cells.
+__global__ __launch_bounds__(256) void fv_hist_kernel(const InstDesc* __restrict__ insts,
+                                                      int phase) {
+  const InstDesc& D = insts[blockIdx.y];
+  if (D.hist == nullptr || D.fp64) return;
+  const Fold<float> F = fold_decide(fold_load<float>(D.ctl + ((phase + 1) & 1), D.sync + phase % 3),
+                                    D.cfl_max, D.dt_seq != nullptr);
+  if (F.state != kFoldAccept) return;
+  const int64_t t = F.p.n_acc + 1 - D.log_base;
+  if (t < 1 || t > D.log_len) return;
+  const size_t n = (size_t)D.nx * (size_t)D.ny;
+  float* dst = D.hist + (size_t)t * 3 * n;
+  const int out = F.p.cur ^ 1;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const float* src = D.q[out][f];
+    float* d = dst + (size_t)f * n;
+    // 16-byte moves where both planes allow them (every buffer is 256-byte
+    // aligned; plane f of the history starts at f * n floats), then the rest
+    const size_t n4 = (n % 4 == 0) ? n / 4 : 0;
+    for (size_t c = first; c < n4; c += stride)
+      reinterpret_cast<float4*>(d)[c] = reinterpret_cast<const float4*>(src)[c];
+    for (size_t c = 4 * n4 + first; c < n; c += stride) d[c] = src[c];
+  }
+}
+
 }  // namespace fv
+
+extern "C" hipError_t fv_launch_hist(const fv::InstDesc* insts, int n_inst, long max_cells,
+                                     int phase, hipStream_t stream) {
+  if (n_inst <= 0) return hipSuccess;
+  const long want = (max_cells + 255) / 256;
+  const long blocks = want < 4096 ? want : 4096;
+  hipLaunchKernelGGL(fv::fv_hist_kernel, dim3((unsigned)blocks, n_inst), dim3(256), 0, stream,
+                     insts, phase);
+  return hipGetLastError();
+}
 
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, int phase, hipStream_t stream) {

@@ -175,17 +175,21 @@ typedef struct {
 } fivonagi_objective;
 
 typedef struct {
-  int32_t checkpoint_every;  /* states kept every K steps (0 = ceil(sqrt(nsteps)));
-                                segments of K steps are recomputed backwards by
-                                the step kernel, each step from the control
-                                block the forward step ran with            */
+  int32_t checkpoint_every;  /* 1 = keep every state of the forward run (no
+                                recompute); K > 1 = keep one every K steps and
+                                recompute the segments between them with the
+                                step kernel, each step from the control block
+                                the forward step ran with; 0 = every state
+                                when they fit in half the free device memory,
+                                else K = ceil(sqrt(nsteps))                   */
   int32_t plain_forward;     /* 1 = forward with one launch per step and a host
                                 readback of dt after each (no graph replay)   */
   int32_t check_recompute;   /* 1 = the reverse pass also recomputes every
                                 q_{n+1} and compares it bit for bit with the
                                 stored state; any difference returns
                                 FIVONAGI_ERR_CHECK                            */
-  int32_t pad;
+  int32_t rows_per_wave;     /* rows per wave of the reverse march (0 = the
+                                least that starts every wave at once)         */
 } fivonagi_adjoint_opts;
 
 /* Advance every instance nsteps accepted steps (as fivonagi_step, recording
