@@ -427,19 +427,29 @@ def bench_batched_adjoint(args, hip):
     ctx.close()
     n_plain = min(args.plain_instances, len(probs))
 
-    def solves(plain):
+    def solve(k, plain):
+        c = fa.Context(probs[k])
         t0 = time.perf_counter()
-        out = []
-        for k in range(n_plain):
-            c = fa.Context(probs[k])
-            jk, _, _ = c.step_adjoint(nsteps, boxes[k], plain_forward=plain)
-            out.append(jk[0])
-            c.close()
-            print(f"{'plain' if plain else 'graph-forward'} solve {k + 1}/{n_plain} done",
-                  file=sys.stderr, flush=True)
-        return (time.perf_counter() - t0) * len(probs) / n_plain, out
-    tp, Jp = solves(True)
-    tg, _ = solves(False)
+        jk, _, _ = c.step_adjoint(nsteps, boxes[k], plain_forward=plain)
+        el = time.perf_counter() - t0
+        c.close()
+        return el, jk[0]
+    solve(0, False)                      # untimed: first use of the single-instance kernels
+    # plain and graph-forward solves interleaved; the fastest of each kind,
+    # scaled to the 32 instances (after the batched run the device can clock
+    # down under the sustained load, which slows the later solves of both
+    # kinds alike)
+    tps, tgs, Jp = [], [], []
+    for k in range(n_plain):
+        el, jk = solve(k, True)
+        tps.append(el)
+        Jp.append(jk)
+        el, _ = solve(k, False)
+        tgs.append(el)
+        print(f"solve {k + 1}/{n_plain}: plain {tps[-1]:.3f} s, graph-forward {tgs[-1]:.3f} s",
+              file=sys.stderr, flush=True)
+    tp = float(np.min(tps)) * len(probs)
+    tg = float(np.min(tgs)) * len(probs)
     rel = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(J[:n_plain], Jp))
     cells = sum(p.nx * p.ny for p in probs)
     achieved = ADJ_BYTES_PER_CELL_STEP * cells * nsteps / tb / 1e9

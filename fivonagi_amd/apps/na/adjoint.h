@@ -75,10 +75,13 @@ struct WaveAdj {
   float gs[3];
 };
 
+// An empty accumulator.  -0 is the exact identity of IEEE addition
+// (-0 + x = x for every x, +0 included; fma(a, b, -0) = a b), so the first
+// term added to it needs no instruction of its own; +0 is not (+0 + -0 = +0).
 __device__ __forceinline__ void zero(WaveAdj& a) {
 #pragma unroll
-  for (int c = 0; c < 7; ++c) a.g[c] = 0.0f;
-  a.gs[0] = a.gs[1] = a.gs[2] = 0.0f;
+  for (int c = 0; c < 7; ++c) a.g[c] = -0.0f;
+  a.gs[0] = a.gs[1] = a.gs[2] = -0.0f;
 }
 
 // Transpose of limiter_rec: from the adjoints gP, gM of the interface's P

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfivonagi.so")
-SOURCES = [os.path.join(HERE, "csrc", n) for n in ("fv_step.hip", "fv_adjoint_march.hip", "fv_scratch.hip", "fv_runtime.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("fv_step.hip", "fv_adjoint_march.hip", "fv_scratch.hip", "fv_tables.cpp",
+                                                     "fv_runtime.cpp")]
 ARCH = os.environ.get("FIVONAGI_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -85,6 +85,12 @@ struct XRow {          // one cell of q_xy (the recomputed-state check)
 };
 
 __device__ __forceinline__ fvad::SumAdj zero_sum() { return fvad::SumAdj{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
+// a where keep holds, else 0: one select per field (no branch, so no merge
+// of two versions of every field)
+__device__ __forceinline__ fvad::SumAdj keep_if(bool keep, const fvad::SumAdj& a) {
+  return fvad::SumAdj{keep ? a.s1 : 0.f, keep ? a.s2 : 0.f, keep ? a.s3 : 0.f,
+                      keep ? a.dr : 0.f, keep ? a.dn : 0.f, keep ? a.dt : 0.f};
+}
 
 // The reverse step of one work item of an instance whose ghost cells follow
 // the boundary policy BC.  kCheck: also recompute q_{n+1} on the item's
@@ -337,7 +343,7 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
     }
     fvad::SumAdj sx = fvad::riemann_bw(fvna::add_nc(l1, y1q), fvna::add_nc(lun, un),
                                        fvna::add_nc(lut, ut), y1q - l1, y2q - l2, hbeta, fx, gx);
-    if (!col_x) sx = zero_sum();
+    sx = keep_if(col_x, sx);
     Vec3 lyc;
     {
       // cell k: interface k's R part and interface k+1's L part (lane k+1)
@@ -378,7 +384,7 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
     fvad::SumAdj sy = fvad::riemann_bw(fvna::add_nc(b4.q1, b3.q1), fvna::add_nc(b4.un, b3.un),
                                        fvna::add_nc(b4.ut, b3.ut), b3.q1 - b4.q1, b3.q3 - b4.q3,
                                        hbeta, f3, acc3);
-    if (!(col_in && r - 3 >= 1 && r - 3 <= ny - 1)) sy = zero_sum();
+    sy = keep_if(col_in && r - 3 >= 1 && r - 3 <= ny - 1, sy);
     // ---- lam_b of row r-4: lam_y, interface r-4's R part, interface r-3's L part ----
     Vec3 lb;
     {

@@ -160,12 +160,12 @@ int orc_line_step_f64(const orc_params *p, double *q1, double *q2, double *q3, d
    Any gradient pointer may be NULL. */
 int orc_adjoint_f64(const orc_params *p, const double *q0, int64_t nsteps, const orc_box *box,
                     double *J, double *grad_amp, double *grad_phase, double *grad_q0);
-/* J of the oracle forward (orc_run_f64 step by step) for finite differences */
 /* The same along a given trajectory: states = q_0 .. q_nsteps (3 grids each,
    nsteps + 1 states), the branches differentiated being those of its steps. */
 int orc_adjoint_path_f64(const orc_params *p, const double *states, int64_t nsteps,
                          const orc_box *box, double *J, double *grad_amp, double *grad_phase,
                          double *grad_q0);
+/* J of the oracle forward (orc_run_f64 step by step) for finite differences */
 int orc_objective_f64(const orc_params *p, const double *q0, int64_t nsteps,
                       const orc_box *box, double *J);
 
