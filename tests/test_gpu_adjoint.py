@@ -154,8 +154,9 @@ def test_batched_adjoint_matches_single():
         J, ga, gp = ctx.step_adjoint(NSTEPS, boxes)
     for k, (p, b) in enumerate(zip([prob, prob2], boxes)):
         Js, gas, gps, _ = gpu_adjoint(p, NSTEPS, dts, b)
-        assert abs(J[k] - Js) <= 1e-6 * Js
-        assert rel_max(ga[k], gas) < 1e-5 and rel_max(gp[k], gps) < 1e-5
+        # J sums the work items' shares, which differ between the two splits
+        assert abs(J[k] - Js) <= 1e-12 * Js
+        assert np.array_equal(ga[k], gas) and np.array_equal(gp[k], gps)
 
 
 def test_feedback_dt_adjoint_runs_and_plain_forward_agrees():
@@ -194,3 +195,75 @@ def test_hip_state_adjoint_other_boundaries_matches_fp64(oracle, kind):
           " ".join(f"{v:.2e}" for v in e[1:]) + f" along the GPU path ({eo:.2e} against the "
           "FP64 run's own path)", flush=True)
     assert e[0] < 1e-5 and max(e[1:]) < 1e-5 and eo < 1e-3
+
+
+def test_mixed_batch_adjoint_matches_single_instances():
+    """One batch of three apps (edge drive, zero gradient, Taylor; three grid
+    sizes): the reverse kernel dispatches per work item on the boundary kind.
+    Every state kept (K = 1) and segments recomputed (K = 5) agree bit for
+    bit, the recompute check passes, and each instance's gradient and dJ/dq0
+    are those of the instance run alone (the batch steps it with the kernel
+    that carries every app, alone it gets its app's own kernel)."""
+    rng = np.random.default_rng(31)
+    amp, ph, q0 = setup_case(rng)
+    p_ed = mk(amp, ph)
+    p_ed.q0 = np.ascontiguousarray(q0, np.float32)
+    rng2 = np.random.default_rng(32)
+    _, _, q0b = setup_case(rng2)
+    p_zg = mk_zero_gradient(q0b)
+    p_zg.q0 = np.ascontiguousarray(p_zg.q0, np.float32)
+    p_ty = mk_taylor()
+    p_ty.q0 = np.ascontiguousarray(p_ty.q0, np.float32)
+    probs = [p_ed, p_zg, p_ty]
+    boxes = [BOX, (1, 30, 12, 9), (20, 40, 16, 12)]
+    dts = [0.04, 0.04, 0.05]
+    n = 14
+    out = {}
+    for K in (1, 5):
+        with fa.Context(probs) as ctx:
+            for k, dt in enumerate(dts):
+                ctx.set_dt_sequence(k, np.full(n + 3, dt))
+            J, ga, gp = ctx.step_adjoint(n, boxes, checkpoint_every=K, check_recompute=True)
+            lam = [ctx.adjoint_state(k) for k in range(3)]
+        out[K] = (J, ga, gp, lam)
+    for k in range(3):
+        assert out[1][0][k] == out[5][0][k], k
+        assert np.array_equal(out[1][1][k], out[5][1][k]) and np.array_equal(out[1][3][k], out[5][3][k])
+    J, ga, gp, lam = out[1]
+    for k, (p, b, dt) in enumerate(zip(probs, boxes, dts)):
+        Js, gas, gps, lams = gpu_adjoint(p, n, np.full(n + 3, dt), b)
+        assert J[k] > 0 and abs(J[k] - Js) <= 1e-12 * Js, (k, J[k], Js)
+        assert np.array_equal(ga[k], gas) and np.array_equal(gp[k], gps), k
+        assert np.array_equal(lam[k], lams), k
+
+
+def test_adjoint_with_rejected_attempts_in_the_forward():
+    """CFL feedback from a first dt several times too long: the forward rejects
+    attempts, and the control-block log indexes accepted steps only.  Every
+    state kept and segments recomputed give the same result bit for bit, the
+    recompute check passes, and J equals the sum the forward's own states give."""
+    rng, amp, ph, prob = case(17)
+    prob.dt0 = 0.2
+    n = 20
+    res = {}
+    for K in (1, 3, n):
+        with fa.Context(prob) as ctx:
+            J, ga, gp = ctx.step_adjoint(n, BOX, checkpoint_every=K, check_recompute=True)
+            info = ctx.info(0)
+            lam = ctx.adjoint_state(0)
+        res[K] = (J[0], ga[0], gp[0], lam, info["n_rejected"], info["n_accepted"])
+    assert res[1][4] > 0 and res[1][5] == n
+    for K in (3, n):
+        assert res[K][0] == res[1][0] and np.array_equal(res[K][1], res[1][1])
+        assert np.array_equal(res[K][3], res[1][3])
+    # J from the forward's own states, step by step, with the dt it chose
+    x0, y0, bx, by = BOX
+    with fa.Context(prob) as ctx:
+        Jf, T_prev = 0.0, ctx.info(0)["T"]
+        for _ in range(n):
+            ctx.step(1)
+            q, st = ctx.collect(0)
+            dt = st["T"] - T_prev
+            T_prev = st["T"]
+            Jf += dt * float(np.sum(q[0, y0:y0 + by, x0:x0 + bx].astype(np.float64) ** 2))
+    assert abs(res[1][0] - Jf) <= 1e-9 * Jf, (res[1][0], Jf)
