@@ -35,8 +35,6 @@ extern "C" hipError_t fv_adj_reverse(const fv::AdjDesc* ds, const fv::AdjItem* i
                                      int n_items, int k, int slot, int par, int check,
                                      hipStream_t s);
 extern "C" hipError_t fv_adj_occupancy(int* waves_per_cu);
-extern "C" hipError_t fv_launch_hist(const fv::InstDesc* insts, int n_inst, long max_cells,
-                                     int phase, hipStream_t stream);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, int phase, hipStream_t stream);
 extern "C" hipError_t fv_scratch_get(void** ptr, size_t* bytes);
@@ -106,8 +104,8 @@ struct fivonagi_ctx {
   static constexpr int kGraphChunk = 300, kGraphCache = 8;
   int taylor_ring = 0;                     // largest ghost ring of a Taylor instance
   int collect_nx = 0;                      // widest instance with data collection (0: none)
-  long hist_cells = 0;                     // largest instance whose accepted states are kept
-                                           // (adjoint forward pass; 0: none)
+  bool keep_hist = false;                  // the step stores accepted states into the
+                                           // instances' histories (adjoint forward pass)
   bool unlimited = false;                  // targets set to "run every attempt"
   struct Graph {
     int len;
@@ -266,12 +264,11 @@ hipError_t launch_phase(fivonagi_ctx* c, int phase) {
   }
   hipError_t e = fv_launch_step(c->d_desc, c->d_items, c->d_rowmask, (int)c->items.size(),
                                 phase, c->n_inst, c->d_ctl, c->d_sync, c->params[0].bc,
-                                c->ext_any ? 1 : 0, c->fp64 ? 1 : 0, c->stream);
+                                c->ext_any || c->keep_hist ? 1 : 0, c->fp64 ? 1 : 0,
+                                c->stream);
   if (e == hipSuccess && c->collect_nx > 0)
     e = fv_launch_collect(c->d_desc, c->n_inst, c->collect_nx, phase, c->fp64 ? 1 : 0,
                           c->stream);
-  if (e == hipSuccess && c->hist_cells > 0)
-    e = fv_launch_hist(c->d_desc, c->n_inst, c->hist_cells, phase, c->stream);
   return e;
 }
 
@@ -1087,15 +1084,13 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
                          c->stream));
 
   // ---- forward: log the control block of every accepted step; keep every
-  // state (the step launches are followed by fv_hist_kernel) or a checkpoint
-  // every K steps ----
-  long max_cells = 0;
+  // state (the extended step kernel also stores its output into the history
+  // slot of its step) or a checkpoint every K steps ----
   for (int k = 0; k < ni; ++k) {
     c->desc[k].ctl_log = logd[k];
     c->desc[k].log_base = ctl[k].n_acc;
     c->desc[k].log_len = nsteps;
     c->desc[k].hist = hist[k];
-    max_cells = std::max(max_cells, (long)c->params[k].nx * c->params[k].ny);
   }
   int rc = upload_tables(c);
   if (keep_all && rc == 0) {
@@ -1107,8 +1102,8 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
         if (e != hipSuccess) rc = hip_fail(e, "state history copy");
       }
     }
-    c->hist_cells = max_cells;
-    c->drop_graph();            // the captured launches gain the history copy
+    c->keep_hist = true;
+    c->drop_graph();            // the captured launches become the extended kernel's
     if (rc == 0) {
       if (plain) {
         for (int64_t s = 0; s < nsteps && rc == 0; ++s) rc = fivonagi_step(c, 1);
@@ -1116,7 +1111,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
         rc = fivonagi_step(c, nsteps);
       }
     }
-    c->hist_cells = 0;
+    c->keep_hist = false;
     c->drop_graph();
   }
   for (int64_t cp = 0; cp < ncp && rc == 0 && !keep_all; ++cp) {

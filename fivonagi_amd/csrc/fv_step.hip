@@ -497,6 +497,24 @@ __device__ __forceinline__ void step_impl(const InstDesc* __restrict__ insts, co
   }
   typename Mem<R>::g* __restrict__ stage = (typename Mem<R>::g*)D.stage;
   const bool stage_on = kExt && stage != nullptr;
+  // adjoint forward runs that keep every state: this attempt's output also
+  // goes to history slot n_acc + 1 - log_base, the slot of the count it
+  // makes if accepted.  A rejected attempt's copy is overwritten by the
+  // retry, which has the same count, so the slot ends with the accepted
+  // state (FP32 instances, extended instantiation only).
+  gfloat* __restrict__ hs1 = nullptr;
+  gfloat* __restrict__ hs2 = nullptr;
+  gfloat* __restrict__ hs3 = nullptr;
+  if constexpr (kExt && sizeof(R) == 4) {
+    const int64_t t = P.n_acc + 1 - D.log_base;
+    if (D.hist != nullptr && t >= 1 && t <= D.log_len) {
+      const size_t n = (size_t)nx * (size_t)ny;
+      hs1 = (gfloat*)(D.hist + (size_t)t * 3 * n);
+      hs2 = hs1 + n;
+      hs3 = hs2 + n;
+    }
+  }
+  const bool hist_on = kExt && sizeof(R) == 4 && hs1 != nullptr;
   const int jcut = D.jcut;
   // what the boundary policy sees of this attempt; the moving window's MDX
   // only exists in the extended instantiation (else it is 0)
@@ -545,18 +563,23 @@ __device__ __forceinline__ void step_impl(const InstDesc* __restrict__ insts, co
         stg(o3, o, r3);
       }
     } else {
-      if (out_lane && i >= sx && s >= sy) {
-        const uint32_t o = (uint32_t)(s - sy) * row_bytes + kSz * (uint32_t)(i - sx);
+      // one cell of the output, at byte offset o, and of the history slot
+      auto put = [&](uint32_t o) {
         stg(o1, o, mid.q1);
         stg(o2, o, r2);
         stg(o3, o, r3);
-      }
-      if (out_lane && (i + sx > nx - 1 || s + sy > ny - 1)) {
-        const uint32_t o = (uint32_t)s * row_bytes + kSz * (uint32_t)i;
-        stg(o1, o, mid.q1);
-        stg(o2, o, r2);
-        stg(o3, o, r3);
-      }
+        if constexpr (sizeof(R) == 4) {
+          if (hist_on) {
+            stg(hs1, o, mid.q1);
+            stg(hs2, o, r2);
+            stg(hs3, o, r3);
+          }
+        }
+      };
+      if (out_lane && i >= sx && s >= sy)
+        put((uint32_t)(s - sy) * row_bytes + kSz * (uint32_t)(i - sx));
+      if (out_lane && (i + sx > nx - 1 || s + sy > ny - 1))
+        put((uint32_t)s * row_bytes + kSz * (uint32_t)i);
       // data collection reads row jcut as it is before the shift
       if (stage_on && s == jcut && out_lane) stg(stage, kSz * (uint32_t)i, mid.q1);
     }
@@ -821,48 +844,7 @@ __global__ __launch_bounds__(256) void fv_taylor_ghost_kernel(const InstDesc* __
   }
 }
 
-// The state after attempt a, when the attempt was accepted, into the
-// instance's state history (adjoint runs that keep every state instead of
-// recomputing segments): slot n_acc + 1 - log_base.  Run after the step
-// launch of attempt a; blockIdx.y = instance, threads cover the cells.
-__global__ __launch_bounds__(256) void fv_hist_kernel(const InstDesc* __restrict__ insts,
-                                                      int phase) {
-  const InstDesc& D = insts[blockIdx.y];
-  if (D.hist == nullptr || D.fp64) return;
-  const Fold<float> F = fold_decide(fold_load<float>(D.ctl + ((phase + 1) & 1), D.sync + phase % 3),
-                                    D.cfl_max, D.dt_seq != nullptr);
-  if (F.state != kFoldAccept) return;
-  const int64_t t = F.p.n_acc + 1 - D.log_base;
-  if (t < 1 || t > D.log_len) return;
-  const size_t n = (size_t)D.nx * (size_t)D.ny;
-  float* dst = D.hist + (size_t)t * 3 * n;
-  const int out = F.p.cur ^ 1;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    const float* src = D.q[out][f];
-    float* d = dst + (size_t)f * n;
-    // 16-byte moves where both planes allow them (every buffer is 256-byte
-    // aligned; plane f of the history starts at f * n floats), then the rest
-    const size_t n4 = (n % 4 == 0) ? n / 4 : 0;
-    for (size_t c = first; c < n4; c += stride)
-      reinterpret_cast<float4*>(d)[c] = reinterpret_cast<const float4*>(src)[c];
-    for (size_t c = 4 * n4 + first; c < n; c += stride) d[c] = src[c];
-  }
-}
-
 }  // namespace fv
-
-extern "C" hipError_t fv_launch_hist(const fv::InstDesc* insts, int n_inst, long max_cells,
-                                     int phase, hipStream_t stream) {
-  if (n_inst <= 0) return hipSuccess;
-  const long want = (max_cells + 255) / 256;
-  const long blocks = want < 4096 ? want : 4096;
-  hipLaunchKernelGGL(fv::fv_hist_kernel, dim3((unsigned)blocks, n_inst), dim3(256), 0, stream,
-                     insts, phase);
-  return hipGetLastError();
-}
 
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, int phase, hipStream_t stream) {

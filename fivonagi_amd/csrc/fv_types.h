@@ -67,7 +67,7 @@ struct InstDesc {
   Ctl* ctl_log;         // when set: the control block accepted step n_acc - log_base ran with
   int64_t log_base, log_len;
   float* hist;          // when set (FP32): state n_acc - log_base after each accepted
-                        // step, slot t = 3 grids, t in [1, log_len] (fv_hist_kernel)
+                        // step, slot t = 3 grids, t in [1, log_len] (the extended step kernel)
   double dx, dy, delta, cfl_max, cfl_wish, omega;
   double inv_dx, inv_dy;     // 1/dx, 1/dy
   double dif_x, dif_y;       // delta/dx^2, delta/dy^2
