@@ -71,6 +71,7 @@ struct PatchCtx {
   int nx, ny;
   int ic;            // the lane's clamped column
   bool left;         // the lane's column is a left ghost column
+  bool any_left;     // some lane of the wave has a left ghost column (uniform)
 };
 
 template <typename R>

@@ -42,7 +42,12 @@ struct EdgeDrive {
   }
   template <typename R>
   __device__ static void patch(const PatchCtx<R>& c, int, Cell<R>& q, const RowData<R>& h) {
-    if (c.mdx != 0) return;   // wave-uniform: the drive stops once the window moves
+    // wave-uniform: the drive stops once the window moves, and only the
+    // waves holding the left ghost columns compute it (the empty volatile asm
+    // keeps this a branch: the compiler would otherwise compute the drive on
+    // every wave and select)
+    if (c.mdx != 0 || !c.any_left) return;
+    asm volatile("");
     Cell<R> d;
     drive(h.ec, h.es, c.sin_t, c.cos_t, d);
     q.q1 = c.left ? d.q1 : q.q1;

@@ -126,6 +126,8 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
   const bool col_int = i >= 2 && i <= nx - 3;
   const bool col_x = col_in && i >= 1;           // lane holds an x interface of the grid
   const bool out_lane = i >= it.xo0 && i < it.xo1;
+  // dt/d on the lanes of interior columns, 0 on ghost columns
+  const float dty_c = col_int ? dty : 0.0f, dtx_c = col_int ? dtx : 0.0f;
   const bool box_col = i >= D.bx0 && i < D.bx0 + D.bnx;
   const uint32_t row_bytes = 4u * (uint32_t)nx;
   const uint32_t cb = 4u * (uint32_t)ic;
@@ -147,6 +149,7 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
   pc.ny = ny;
   pc.ic = ic;
   pc.left = lm.left;
+  pc.any_left = __builtin_amdgcn_ballot_w64(lm.left) != 0;
   auto load_raw = [&](int r, float& a, float& b, float& c, fvbc::RowData<float>& h) {
     const int rc = min(max(r, 0), ny - 1);
     const uint32_t o = BC::offset(rc, ny, lm, row_bytes);
@@ -222,7 +225,7 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
     Row bn;
     bn.q1 = n1; bn.q2 = n2; bn.q3 = n3;
     patch(bn, nh, r);
-    if (r + 1 <= r_end) load_raw(r + 1, n1, n2, n3, nh);
+    load_raw(r + 1, n1, n2, n3, nh);     // row index clamped (past r_end: unused)
     gd = g0; g0 = gu; gu = gnx;
     finish_g(r - 1, gu, rnx);
     load_g(r, gnx, rnx);
@@ -243,7 +246,7 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
     const int m = r - 2;
     const bool row_int = m >= 2 && m <= ny - 3;
     const bool upd = col_int && row_int;
-    const float dtyu = upd ? dty : 0.0f, dtxu = upd ? dtx : 0.0f;
+    const float dtyu = row_int ? dty_c : 0.0f, dtxu = row_int ? dtx_c : 0.0f;
     const float y1q = b2.q1 - dtyu * (Pp[0] + M[0]);
     const float y3q = b2.q3 - dtyu * (Pp[1] + M[1]);
     const float y2q = b2.q2 - dtyu * (Pp[2] + M[2]);
@@ -361,8 +364,7 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
     fvad::WaveAdj acc1;
     fvad::zero(acc1);
     {
-      const bool upd1 = col_int && m - 1 >= 2 && m - 1 <= ny - 3;
-      const float dtyu1 = upd1 ? dty : 0.0f;
+      const float dtyu1 = (m - 1 >= 2 && m - 1 <= ny - 3) ? dty_c : 0.0f;
       // y component order (rho', normal = rho v, tangential = rho u)
       float gPy[3] = {-dtyu * lyc.a, -dtyu * lyc.c, -dtyu * lyc.b};
       float gMy[3] = {-dtyu1 * ly1.a, -dtyu1 * ly1.c, -dtyu1 * ly1.b};

@@ -529,6 +529,7 @@ __device__ __forceinline__ void step_impl(const InstDesc* __restrict__ insts, co
   pc.ny = ny;
   pc.ic = ic;
   pc.left = lm.left;
+  pc.any_left = __builtin_amdgcn_ballot_w64(lm.left) != 0;
   auto patch = [&](Row<R>& q, const fvbc::RowData<R>& h, int r) {
     fvbc::Cell<R> c{q.q1, q.q2, q.q3};
     BC::patch(pc, min(max(r, 0), ny - 1), c, h);
@@ -585,6 +586,8 @@ __device__ __forceinline__ void step_impl(const InstDesc* __restrict__ insts, co
     }
   };
 
+  // dt/d on the lanes of interior columns, 0 on ghost columns
+  const R dty_c = col_int ? dty : R(0), dtx_c = col_int ? dtx : R(0);
   R Sx = R(0), Sy = R(0);
   uint64_t err = 0;                          // lanes that met tmp3 < 0 (or NaN)
   const uint64_t in_mask = __builtin_amdgcn_ballot_w64(col_in);
@@ -606,7 +609,10 @@ __device__ __forceinline__ void step_impl(const InstDesc* __restrict__ insts, co
     bn.q1 = n1; bn.q2 = n2; bn.q3 = n3;
     patch(bn, nh, r);
     if (r >= y0 + 4) source_store(r - 4, xn, xo, xm);
-    if (r + 1 <= y1 + 2) load_raw(r + 1, n1, n2, n3, nh);
+    // the next row, also past the item's last row: the row index is clamped
+    // to the grid, and an unconditional load needs no copy of the registers
+    // it replaces
+    load_raw(r + 1, n1, n2, n3, nh);
     if (r < y0 - 2) return;
     const int m = r - 2;
     // CFL lane masks of rows r (y interfaces) and m (x interfaces); the index
@@ -638,10 +644,10 @@ __device__ __forceinline__ void step_impl(const InstDesc* __restrict__ insts, co
 
     // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2)
     const bool row_int = (m >= 2) && (m <= ny - 3);
-    const bool upd = col_int && row_int;
     // ghost cells keep their value: dt/d is 0 there (the fluxes are finite
-    // wherever the attempt is valid, so q - 0*flux == q exactly)
-    const R dtyu = upd ? dty : R(0), dtxu = upd ? dtx : R(0);
+    // wherever the attempt is valid, so q - 0*flux == q exactly); the column
+    // test is folded in once per item (dty_c, dtx_c), the row test here
+    const R dtyu = row_int ? dty_c : R(0), dtxu = row_int ? dtx_c : R(0);
     const R y1q = bo.q1 - dtyu * (fl.P[0] + M[0]);
     const R y3q = bo.q3 - dtyu * (fl.P[1] + M[1]);
     const R y2q = bo.q2 - dtyu * (fl.P[2] + M[2]);
