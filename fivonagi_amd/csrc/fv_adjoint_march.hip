@@ -38,6 +38,7 @@ namespace adjm {
 constexpr int kLanes = 64;
 constexpr int kHalo = 4;                       // x halo of the reverse march, each side
 constexpr int kCols = kLanes - 2 * kHalo;      // 56 output columns per strip
+static_assert(kCols == 56, "the host cuts the reverse march's strips at 56 columns (fv_tables.cpp)");
 
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) const float cgfloat;

@@ -114,6 +114,8 @@ struct fivonagi_ctx {
   };
   std::vector<Graph> graphs;                 // oldest first
   std::vector<float*> adj_lam;             // dJ/dq0 of the last adjoint call (per instance)
+  void* adj_pool = nullptr;                // the adjoint calls' device buffers, kept for
+  size_t adj_pool_bytes = 0;               // the next call (grown when it needs more)
   struct CollectBufs { void* measure = nullptr; void* stage = nullptr; };
   std::vector<CollectBufs> collect_bufs;   // per instance, kept once allocated
   uint32_t* d_collect_err = nullptr;       // collection overflow flag per instance
@@ -137,9 +139,10 @@ struct fivonagi_ctx {
   }
   ~fivonagi_ctx() {
     drop_graph();
-    for (float* q : adj_lam) (void)hipFree(q);
-    for (double* d : dt_seq_dev) (void)hipFree(d);
     if (stream) (void)hipStreamSynchronize(stream);
+    for (float* q : adj_lam) (void)hipFree(q);
+    if (adj_pool) (void)hipFree(adj_pool);
+    for (double* d : dt_seq_dev) (void)hipFree(d);
     if (d_items) (void)hipFree(d_items);
     if (d_rowmask) (void)hipFree(d_rowmask);
     for (void* p : allocs) (void)hipFree(p);
@@ -956,19 +959,20 @@ int fivonagi_info(fivonagi_ctx* c, int inst, fivonagi_info_t* info) {
 namespace {
 
 // device allocations that live for one adjoint call
-struct AdjScratch {
-  std::vector<void*> p;
-  hipError_t e = hipSuccess;
+// The device buffers of an adjoint call, carved from one block that the
+// context keeps (fivonagi_ctx::adj_pool) and reuses on later calls: repeated
+// gradients allocate nothing, and a large block freed and allocated again can
+// wait for the driver to clear the freed memory.  The carving runs once with
+// no base to size the block, then again to hand out the pointers; every
+// buffer starts on a 256-byte boundary.
+struct AdjArena {
+  char* base = nullptr;
+  size_t off = 0;
   template <class T>
   T* get(size_t count) {
-    if (e != hipSuccess) return nullptr;
-    void* q = nullptr;
-    e = hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1));
-    if (e == hipSuccess) p.push_back(q);
-    return (T*)q;
-  }
-  ~AdjScratch() {
-    for (void* q : p) (void)hipFree(q);
+    const size_t o = off;
+    off += (sizeof(T) * std::max<size_t>(count, 1) + 255) & ~(size_t)255;
+    return base ? (T*)(base + o) : nullptr;
   }
 };
 
@@ -1003,6 +1007,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     hist_bytes += sizeof(float) * 3 * (size_t)p.nx * p.ny * (size_t)(nsteps + 1);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  free_b += c->adj_pool_bytes;                 // the kept block is reused
   const bool keep_all = req == 1 || (req <= 0 && hist_bytes <= free_b / 2);
   int K = keep_all ? 64 : (req > 1 ? req : (int)std::ceil(std::sqrt((double)nsteps)));
   K = (int)std::min<int64_t>(K, nsteps);
@@ -1031,16 +1036,45 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   }
 
   // ---- device buffers of the call ----
-  AdjScratch sc;
   std::vector<fv::AdjDesc> ad(ni);
   std::vector<float*> cps(ni);
   std::vector<fv::Ctl*> logd(ni);
   std::vector<float*> hist(ni, nullptr);
-  double* jpart = sc.get<double>(items.size());
-  uint32_t* d_mis = sc.get<uint32_t>(1);
+  double* jpart = nullptr;
+  uint32_t* d_mis = nullptr;
+  fv::AdjItem* d_items = nullptr;
+  fv::AdjDesc* d_ad = nullptr;
+  fv::InstDesc* d_rdesc = nullptr;    // the step kernel's descriptors and control
+  fv::Ctl* d_rctl = nullptr;          // blocks for the segment recompute: slot s of
+  fv::SyncSlots* d_rsync = nullptr;   // every instance, and its logged control block
+  auto carve = [&](AdjArena& A) {
+    jpart = A.get<double>(items.size());
+    d_mis = A.get<uint32_t>(1);
+    for (int k = 0; k < ni; ++k) {
+      const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
+      auto& a = ad[k];
+      if (keep_all)
+        hist[k] = a.seg = A.get<float>((size_t)(nsteps + 1) * 3 * n);
+      else
+        a.seg = A.get<float>((size_t)(K + 1) * 3 * n);
+      a.lam[0] = A.get<float>(3 * n);
+      a.lam[1] = A.get<float>(3 * n);
+      logd[k] = A.get<fv::Ctl>(nsteps);
+      a.ctl_log = logd[k];
+      a.grad_ec = A.get<double>(c->params[k].ny);
+      a.grad_es = A.get<double>(c->params[k].ny);
+      a.jpart = jpart;
+      a.mismatch = d_mis;
+      cps[k] = keep_all ? nullptr : A.get<float>((size_t)ncp * 3 * n);
+    }
+    d_items = A.get<fv::AdjItem>(items.size());
+    d_ad = A.get<fv::AdjDesc>(ni);
+    d_rdesc = A.get<fv::InstDesc>((size_t)K * ni);
+    d_rctl = A.get<fv::Ctl>((size_t)K * ni * 2);
+    d_rsync = A.get<fv::SyncSlots>((size_t)3 * ni);
+  };
   for (int k = 0; k < ni; ++k) {
     const auto& p = c->params[k];
-    const size_t n = (size_t)p.nx * p.ny;
     auto& a = ad[k];
     std::memset(&a, 0, sizeof(a));
     a.nx = p.nx; a.ny = p.ny; a.bc = p.bc;
@@ -1048,33 +1082,34 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
     a.bx0 = obj[k].box_x0; a.by0 = obj[k].box_y0; a.bnx = obj[k].box_nx; a.bny = obj[k].box_ny;
     a.edge_c = c->desc[k].edge_c;
     a.edge_s = c->desc[k].edge_s;
-    if (keep_all)
-      hist[k] = a.seg = sc.get<float>((size_t)(nsteps + 1) * 3 * n);
-    else
-      a.seg = sc.get<float>((size_t)(K + 1) * 3 * n);
-    a.lam[0] = sc.get<float>(3 * n);
-    a.lam[1] = sc.get<float>(3 * n);
-    logd[k] = sc.get<fv::Ctl>(nsteps);
-    a.ctl_log = logd[k];
-    a.grad_ec = sc.get<double>(p.ny);
-    a.grad_es = sc.get<double>(p.ny);
-    a.jpart = jpart;
-    a.mismatch = d_mis;
-    if (!keep_all) cps[k] = sc.get<float>((size_t)ncp * 3 * n);
-    if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint scratch)");
-    HIP_TRY(hipMemsetAsync(a.lam[0], 0, sizeof(float) * 3 * n, c->stream));
-    HIP_TRY(hipMemsetAsync(a.grad_ec, 0, sizeof(double) * p.ny, c->stream));
-    HIP_TRY(hipMemsetAsync(a.grad_es, 0, sizeof(double) * p.ny, c->stream));
+  }
+  {
+    AdjArena size_only;
+    carve(size_only);
+    if (c->adj_pool_bytes < size_only.off) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if (c->adj_pool) (void)hipFree(c->adj_pool);
+      c->adj_pool = nullptr;
+      c->adj_pool_bytes = 0;
+      hipError_t e = hipMalloc(&c->adj_pool, size_only.off);
+      if (e != hipSuccess) {
+        c->adj_pool = nullptr;
+        return hip_fail(e, "hipMalloc (adjoint buffers)");
+      }
+      c->adj_pool_bytes = size_only.off;
+    }
+    AdjArena A;
+    A.base = (char*)c->adj_pool;
+    carve(A);
+  }
+  for (int k = 0; k < ni; ++k) {
+    const auto& p = c->params[k];
+    const size_t n = (size_t)p.nx * p.ny;
+    HIP_TRY(hipMemsetAsync(ad[k].lam[0], 0, sizeof(float) * 3 * n, c->stream));
+    HIP_TRY(hipMemsetAsync(ad[k].grad_ec, 0, sizeof(double) * p.ny, c->stream));
+    HIP_TRY(hipMemsetAsync(ad[k].grad_es, 0, sizeof(double) * p.ny, c->stream));
     HIP_TRY(hipMemsetAsync(logd[k], 0, sizeof(fv::Ctl) * nsteps, c->stream));
   }
-  fv::AdjItem* d_items = sc.get<fv::AdjItem>(items.size());
-  fv::AdjDesc* d_ad = sc.get<fv::AdjDesc>(ni);
-  // the step kernel's descriptors and control blocks for the segment
-  // recompute: slot s of every instance, and its logged control block
-  fv::InstDesc* d_rdesc = sc.get<fv::InstDesc>((size_t)K * ni);
-  fv::Ctl* d_rctl = sc.get<fv::Ctl>((size_t)K * ni * 2);
-  fv::SyncSlots* d_rsync = sc.get<fv::SyncSlots>((size_t)3 * ni);
-  if (sc.e != hipSuccess) return hip_fail(sc.e, "hipMalloc (adjoint tables)");
   HIP_TRY(hipMemsetAsync(jpart, 0, sizeof(double) * items.size(), c->stream));
   HIP_TRY(hipMemsetAsync(d_mis, 0, sizeof(uint32_t), c->stream));
   HIP_TRY(hipMemsetAsync(d_rsync, 0, sizeof(fv::SyncSlots) * 3 * ni, c->stream));

@@ -196,7 +196,10 @@ typedef struct {
    dt and T), then run the adjoint backwards.  obj: n_inst entries.  J: n_inst
    values.  grad_amp / grad_phase: dJ/d edge_amp[j], dJ/d edge_phase[j], the
    instances' ny values concatenated (may be NULL).  The state is left at the
-   end of the forward run. */
+   end of the forward run.  The call's device buffers (kept states or
+   checkpoints, adjoint grids, logs) are one block that the context keeps for
+   later calls, grown when a call needs more; fivonagi_destroy frees it.  The
+   free memory that the checkpoint_every = 0 rule weighs counts that block. */
 int fivonagi_step_adjoint(fivonagi_ctx *ctx, int64_t nsteps, const fivonagi_objective *obj,
                           const fivonagi_adjoint_opts *opts, double *J, double *grad_amp,
                           double *grad_phase);

@@ -267,3 +267,32 @@ def test_adjoint_with_rejected_attempts_in_the_forward():
             T_prev = st["T"]
             Jf += dt * float(np.sum(q[0, y0:y0 + by, x0:x0 + bx].astype(np.float64) ** 2))
     assert abs(res[1][0] - Jf) <= 1e-9 * Jf, (res[1][0], Jf)
+
+
+def test_repeated_calls_reuse_the_context_block():
+    """Adjoint calls one after another on one context (every state kept,
+    checkpoints, a longer run that grows the kept block, a shorter one inside
+    it) against each call on a fresh context brought to the same state by
+    plain steps first: J, the drive gradients and dJ/dq0 bit for bit, so
+    nothing of an earlier call's buffers reaches a later one.  CFL feedback
+    (no prescribed dt), so the plain steps and the adjoint's forward take the
+    same dt."""
+    _, _, _, prob = case()
+    n = 12
+    calls = [(n, 1), (n, 4), (2 * n, 1), (n, 1)]
+    got = []
+    with fa.Context(prob) as ctx:
+        for steps, K in calls:
+            J, ga, gp = ctx.step_adjoint(steps, BOX, checkpoint_every=K)
+            got.append((J[0], ga[0], gp[0], ctx.adjoint_state(0)))
+    before = 0
+    for (steps, K), (J, ga, gp, lam) in zip(calls, got):
+        with fa.Context(prob) as ctx:
+            if before:
+                ctx.step(before)
+            Jr, gar, gpr = ctx.step_adjoint(steps, BOX, checkpoint_every=K)
+            lamr = ctx.adjoint_state(0)
+        assert J == Jr[0], (steps, K)
+        assert np.array_equal(ga, gar[0]) and np.array_equal(gp, gpr[0]), (steps, K)
+        assert all(np.array_equal(a, b) for a, b in zip(lam, lamr)), (steps, K)
+        before += steps
