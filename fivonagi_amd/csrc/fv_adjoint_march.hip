@@ -238,246 +238,239 @@ __device__ __forceinline__ void reverse_item(const AdjDesc& D, const AdjItem& it
     dln[0] = fvna::dot1(fn.w, f1.w);
     dln[1] = fvna::dot2(fn.w, f1.w);
     dln[2] = fn.w.a3 * f1.w.a3;
-    // Stages by row.  Only the rows the item emits (lam_n rows y0 .. y1-1)
-    // need a stage, and those reach back: the Riemann transpose runs for
-    // interfaces y0 .. y1 (rows r >= y0+3), the limiter transposes for
-    // interfaces y0-1 .. y1+1 (r >= y0+1), lam_y and the recomputed x sweep
-    // for rows y0-2 .. y1+1 (r >= y0), the y limiter for interfaces from
-    // y0-2 (r >= y0-1), the y Riemann solve on every row.  A skipped stage
-    // leaves its window entries at zero, and no emitted row reads them.
-    float P1[3] = {0.f, 0.f, 0.f};            // P of interface r-1
-    fvad::LimT lim1{};                        // limiter record of interface r-1
+    // ---- y limiter at interface r-1 ----
+    float P1[3], M[3];
+    fvad::LimT lim1;
+    fvad::limiter_rec(f1, dl1[0], dln[0], dl1[1], dln[1], dl1[2], dln[2], dty, P1, M, lim1);
+
+    // The rows before y0 only warm up the windows of the forward recompute
+    // (the y interfaces and limiters the first emitted rows need); everything
+    // from q_y on is skipped there, and its window entries stay at zero (no
+    // emitted row reads them).  One guard: finer ones split the row into
+    // blocks the scheduler cannot interleave, which costs tall items more
+    // than the rows they skip.
     Vec3 lyc{};                               // lam_y of row m = r-2
     fvad::WaveAdj acc1;                       // wave adjoint of interface r-1
     fvad::zero(acc1);
-    if (r >= y0 - 1) {
-      // ---- y limiter at interface r-1 ----
-      float M[3];
-      fvad::limiter_rec(f1, dl1[0], dln[0], dl1[1], dln[1], dl1[2], dln[2], dty, P1, M, lim1);
-      if (r >= y0) {
-        // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2) ----
-        const int m = r - 2;
-        const bool row_int = m >= 2 && m <= ny - 3;
-        const bool upd = col_int && row_int;
-        const float dtyu = row_int ? dty_c : 0.0f, dtxu = row_int ? dtx_c : 0.0f;
-        const float y1q = b2.q1 - dtyu * (Pp[0] + M[0]);
-        const float y3q = b2.q3 - dtyu * (Pp[1] + M[1]);
-        const float y2q = b2.q2 - dtyu * (Pp[2] + M[2]);
+    if (r >= y0) {
+      // ---- q_y on row m = r-2 (components: density, normal=q3, tangential=q2) ----
+      const int m = r - 2;
+      const bool row_int = m >= 2 && m <= ny - 3;
+      const bool upd = col_int && row_int;
+      const float dtyu = row_int ? dty_c : 0.0f, dtxu = row_int ? dtx_c : 0.0f;
+      const float y1q = b2.q1 - dtyu * (Pp[0] + M[0]);
+      const float y3q = b2.q3 - dtyu * (Pp[1] + M[1]);
+      const float y2q = b2.q2 - dtyu * (Pp[2] + M[2]);
 
-        // ---- x sweep on row m: normal = q2, tangential = q3 ----
-        const float ir = fvna::rcp(1.0f + y1q);
-        const float un = y2q * ir, ut = y3q * ir;
-        const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
-        const float lun = shr1(un), lut = shr1(ut);
-        fvna::Iface fx;
-        fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, hbeta, fx);
-        fvna::Waves wl;
-        wl.a1 = shr1(fx.w.a1); wl.w21 = shr1(fx.w.w21); wl.w31 = shr1(fx.w.w31);
-        wl.a2 = shr1(fx.w.a2); wl.w22 = shr1(fx.w.w22); wl.w32 = shr1(fx.w.w32);
-        wl.a3 = shr1(fx.w.a3);
-        const float dlx1 = fvna::dot1(fx.w, wl), dlx2 = fvna::dot2(fx.w, wl);
-        const float dlx3 = fx.w.a3 * wl.a3;
-        const float drx1 = shl1(dlx1), drx2 = shl1(dlx2), drx3 = shl1(dlx3);
-        float PX[3], MX[3];
-        fvad::LimT limx;
-        fvad::limiter_rec(fx, dlx1, drx1, dlx2, drx2, dlx3, drx3, dtx, PX, MX, limx);
-        const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
-        XRow xc;
-        xc.q1 = y1q - dtxu * (mr0 + PX[0]);
-        xc.q2 = y2q - dtxu * (mr1 + PX[1]);
-        xc.q3 = y3q - dtxu * (mr2 + PX[2]);
-        const float ir2 = fvna::rcp(1.0f + xc.q1);
-        xc.u = xc.q2 * ir2;
-        xc.v = xc.q3 * ir2;
+      // ---- x sweep on row m: normal = q2, tangential = q3 ----
+      const float ir = fvna::rcp(1.0f + y1q);
+      const float un = y2q * ir, ut = y3q * ir;
+      const float l1 = shr1(y1q), l2 = shr1(y2q), l3 = shr1(y3q);
+      const float lun = shr1(un), lut = shr1(ut);
+      fvna::Iface fx;
+      fvna::riemann(l1, l2, l3, lun, lut, y1q, y2q, y3q, un, ut, hbeta, fx);
+      fvna::Waves wl;
+      wl.a1 = shr1(fx.w.a1); wl.w21 = shr1(fx.w.w21); wl.w31 = shr1(fx.w.w31);
+      wl.a2 = shr1(fx.w.a2); wl.w22 = shr1(fx.w.w22); wl.w32 = shr1(fx.w.w32);
+      wl.a3 = shr1(fx.w.a3);
+      const float dlx1 = fvna::dot1(fx.w, wl), dlx2 = fvna::dot2(fx.w, wl);
+      const float dlx3 = fx.w.a3 * wl.a3;
+      const float drx1 = shl1(dlx1), drx2 = shl1(dlx2), drx3 = shl1(dlx3);
+      float PX[3], MX[3];
+      fvad::LimT limx;
+      fvad::limiter_rec(fx, dlx1, drx1, dlx2, drx2, dlx3, drx3, dtx, PX, MX, limx);
+      const float mr0 = shl1(MX[0]), mr1 = shl1(MX[1]), mr2 = shl1(MX[2]);
+      XRow xc;
+      xc.q1 = y1q - dtxu * (mr0 + PX[0]);
+      xc.q2 = y2q - dtxu * (mr1 + PX[1]);
+      xc.q3 = y3q - dtxu * (mr2 + PX[2]);
+      const float ir2 = fvna::rcp(1.0f + xc.q1);
+      xc.u = xc.q2 * ir2;
+      xc.v = xc.q3 * ir2;
 
-        if (kCheck) {
-          // the source of row m-1 from q_xy rows m-2, m-1, m, against q_{n+1}
-          const int s = m - 1;
-          const float uwe = shr1(xc1.u) + shl1(xc1.u);
-          const float vwe = shr1(xc1.v) + shl1(xc1.v);
-          const float i2 = fvna::source_increment(ddx2, ddy2, xc1.u, uwe, xc2.u, xc.u);
-          const float i3 = fvna::source_increment(ddx2, ddy2, xc1.v, vwe, xc2.v, xc.v);
-          const bool s_int = col_int && s >= 2 && s <= ny - 3;
-          const float r2 = s_int ? fvna::apply_source(xc1.q2, i2) : xc1.q2;
-          const float r3 = s_int ? fvna::apply_source(xc1.q3, i3) : xc1.q3;
-          // a Taylor state's ghost ring is rewritten in place by the next step's
-          // prepass (its closed form at the next T): only the cells the step
-          // computes are compared there
-          const bool cmp = BC::kId != kBcTaylor || s_int;
-          if (s >= y0 && s < y1 && out_lane && cmp) {
-            const uint32_t o = (uint32_t)s * row_bytes + cb;
-            const bool same = __float_as_uint(ldg(nq1, o)) == __float_as_uint(xc1.q1) &&
-                              __float_as_uint(ldg(nq2, o)) == __float_as_uint(r2) &&
-                              __float_as_uint(ldg(nq3, o)) == __float_as_uint(r3);
-            bad += same ? 0u : 1u;
+      if (kCheck) {
+        // the source of row m-1 from q_xy rows m-2, m-1, m, against q_{n+1}
+        const int s = m - 1;
+        const float uwe = shr1(xc1.u) + shl1(xc1.u);
+        const float vwe = shr1(xc1.v) + shl1(xc1.v);
+        const float i2 = fvna::source_increment(ddx2, ddy2, xc1.u, uwe, xc2.u, xc.u);
+        const float i3 = fvna::source_increment(ddx2, ddy2, xc1.v, vwe, xc2.v, xc.v);
+        const bool s_int = col_int && s >= 2 && s <= ny - 3;
+        const float r2 = s_int ? fvna::apply_source(xc1.q2, i2) : xc1.q2;
+        const float r3 = s_int ? fvna::apply_source(xc1.q3, i3) : xc1.q3;
+        // a Taylor state's ghost ring is rewritten in place by the next step's
+        // prepass (its closed form at the next T): only the cells the step
+        // computes are compared there
+        const bool cmp = BC::kId != kBcTaylor || s_int;
+        if (s >= y0 && s < y1 && out_lane && cmp) {
+          const uint32_t o = (uint32_t)s * row_bytes + cb;
+          const bool same = __float_as_uint(ldg(nq1, o)) == __float_as_uint(xc1.q1) &&
+                            __float_as_uint(ldg(nq2, o)) == __float_as_uint(r2) &&
+                            __float_as_uint(ldg(nq3, o)) == __float_as_uint(r3);
+          bad += same ? 0u : 1u;
+        }
+        xc2 = xc1;
+        xc1 = xc;
+      }
+
+      // ---- S^T on row m: the source adds ddx2 (u_W - 2u + u_E) +
+      // ddy2 (u_S - 2u + u_N) to rho u, the same with v to rho v, on interior cells ----
+      const bool int_d = col_int && m - 1 >= 2 && m - 1 <= ny - 3;
+      const bool int_u = col_int && m + 1 >= 2 && m + 1 <= ny - 3;
+      const float G2c = upd ? g0.b : 0.0f, G3c = upd ? g0.c : 0.0f;
+      const float G2y = (int_d ? gd.b : 0.0f) + (int_u ? gu.b : 0.0f);
+      const float G3y = (int_d ? gd.c : 0.0f) + (int_u ? gu.c : 0.0f);
+      const float sU = fvna::fma_(ddx2, fvna::fma_(-2.0f, G2c, shr1(G2c) + shl1(G2c)),
+                                  ddy2 * fvna::fma_(-2.0f, G2c, G2y));
+      const float sV = fvna::fma_(ddx2, fvna::fma_(-2.0f, G3c, shr1(G3c) + shl1(G3c)),
+                                  ddy2 * fvna::fma_(-2.0f, G3c, G3y));
+      Vec3 lxy;
+      lxy.a = g0.a - fvna::fma_(sU, xc.u, sV * xc.v) * ir2;
+      lxy.b = fvna::fma_(sU, ir2, g0.b);
+      lxy.c = fvna::fma_(sV, ir2, g0.c);
+
+      // ---- X^T on row m ----
+      // q_xy(k) = q_y(k) - dtxu(k) (PX(k) + MX(k+1))
+      float gPX[3] = {-dtxu * lxy.a, -dtxu * lxy.b, -dtxu * lxy.c};
+      float gMX[3] = {shr1(gPX[0]), shr1(gPX[1]), shr1(gPX[2])};
+      fvad::WaveAdj gx;
+      fvad::zero(gx);
+      float gdx[3];
+      fvad::limiter_bw(fx, limx, dtx, gPX, gMX, gx, gdx);
+      {
+        // family p of interface k dotted with k-1 (L_p > 0) or with k+1
+        fvna::Waves wr;
+        wr.a1 = shl1(fx.w.a1); wr.w21 = shl1(fx.w.w21); wr.w31 = shl1(fx.w.w31);
+        wr.a2 = shl1(fx.w.a2); wr.w22 = shl1(fx.w.w22); wr.w32 = shl1(fx.w.w32);
+        wr.a3 = shl1(fx.w.a3);
+        const float Ls[3] = {fx.L1, fx.L2, fx.L3};
+        float kl[3], kr[3];
+  #pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const float gdl = Ls[p] > 0.0f ? gdx[p] : 0.0f;
+          const float gdr = gdx[p] - gdl;
+          kl[p] = gdl + shr1(gdr);      // times W(k-1): own left dot, and k-1's right dot
+          kr[p] = gdr + shl1(gdl);      // times W(k+1): own right dot, and k+1's left dot
+        }
+        fvad::add_scaled(gx, kl, wl);
+        fvad::add_scaled(gx, kr, wr);
+      }
+      fvad::SumAdj sx = fvad::riemann_bw(fvna::add_nc(l1, y1q), fvna::add_nc(lun, un),
+                                         fvna::add_nc(lut, ut), y1q - l1, y2q - l2, hbeta, fx, gx);
+      sx = keep_if(col_x, sx);
+      {
+        // cell k: interface k's R part and interface k+1's L part (lane k+1)
+        const float ar1 = (sx.s1 + sx.dr) + shl1(sx.s1 - sx.dr);
+        const float an = sx.dn - shl1(sx.dn);
+        const float at = sx.dt - shl1(sx.dt);
+        const float aun = sx.s2 + shl1(sx.s2);
+        const float aut = sx.s3 + shl1(sx.s3);
+        lyc.a = lxy.a + ar1 - fvna::fma_(aun, un, aut * ut) * ir;
+        lyc.b = lxy.b + fvna::fma_(aun, ir, an);
+        lyc.c = lxy.c + fvna::fma_(aut, ir, at);
+      }
+
+      // ---- Y^T: limiter of interface m (P -> cell m, M -> cell m-1) ----
+      {
+        const float dtyu1 = (m - 1 >= 2 && m - 1 <= ny - 3) ? dty_c : 0.0f;
+        // y component order (rho', normal = rho v, tangential = rho u)
+        float gPy[3] = {-dtyu * lyc.a, -dtyu * lyc.c, -dtyu * lyc.b};
+        float gMy[3] = {-dtyu1 * ly1.a, -dtyu1 * ly1.c, -dtyu1 * ly1.b};
+        float gdy[3];
+        fvad::limiter_bw(f2, lim2, dty, gPy, gMy, acc2, gdy);
+        const float Ls[3] = {f2.L1, f2.L2, f2.L3};
+        float kdn[3], kup[3];
+  #pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          kdn[p] = Ls[p] > 0.0f ? gdy[p] : 0.0f;   // dotted with interface m-1
+          kup[p] = gdy[p] - kdn[p];                // ... with m+1
+        }
+        fvad::add_scaled(acc2, kdn, f3.w);
+        fvad::add_scaled(acc2, kup, f1.w);
+        fvad::add_scaled(acc3, kdn, f2.w);
+        fvad::add_scaled(acc1, kup, f2.w);
+      }
+      // ---- Riemann transpose of interface m-1 = r-3 (rows r-4 | r-3) ----
+      fvad::SumAdj sy = fvad::riemann_bw(fvna::add_nc(b4.q1, b3.q1), fvna::add_nc(b4.un, b3.un),
+                                         fvna::add_nc(b4.ut, b3.ut), b3.q1 - b4.q1, b3.q3 - b4.q3,
+                                         hbeta, f3, acc3);
+      sy = keep_if(col_in && r - 3 >= 1 && r - 3 <= ny - 1, sy);
+      // ---- lam_b of row r-4: lam_y, interface r-4's R part, interface r-3's L part ----
+      Vec3 lb;
+      {
+        const float br1 = (gRp.s1 + gRp.dr) + (sy.s1 - sy.dr);
+        const float bn_ = gRp.dn - sy.dn;
+        const float bt = gRp.dt - sy.dt;
+        const float bun = gRp.s2 + sy.s2;
+        const float but = gRp.s3 + sy.s3;
+        const float ir4 = fvna::rcp(1.0f + b4.q1);
+        lb.a = ly2.a + br1 - fvna::fma_(bun, b4.un, but * b4.ut) * ir4;
+        lb.c = ly2.c + fvna::fma_(bun, ir4, bn_);     // normal = rho v
+        lb.b = ly2.b + fvna::fma_(but, ir4, bt);      // tangential = rho u
+      }
+      gRp = sy;
+
+      // ---- B^T and the drive gradient of row j = r-4 ----
+      const int j = r - 4;
+      if (j >= y0 && j < y1) {
+        const bool interior = col_int && j >= 2 && j <= ny - 3;
+        Vec3 a = sel(interior, lb);
+        bool defer = false;
+        if (BC::kId != kBcTaylor) {
+          // zero-gradient copies (apps/common/boundary.h): left columns <- 2
+          // (zero-gradient app only; the edge drive sets them), right columns
+          // <- nx-3 on rows 2..ny-3, rows 0, 1 <- 2 and ny-2, ny-1 <- ny-3 off
+          // the left columns
+          if (BC::kId == kBcZeroGrad) {
+            const Vec3 w1 = shr1(lb), w2 = shr1(w1);
+            if (i == 2) a = a + w1 + w2;
           }
-          xc2 = xc1;
-          xc1 = xc;
-        }
-
-        // ---- S^T on row m: the source adds ddx2 (u_W - 2u + u_E) +
-        // ddy2 (u_S - 2u + u_N) to rho u, the same with v to rho v, on interior cells ----
-        const bool int_d = col_int && m - 1 >= 2 && m - 1 <= ny - 3;
-        const bool int_u = col_int && m + 1 >= 2 && m + 1 <= ny - 3;
-        const float G2c = upd ? g0.b : 0.0f, G3c = upd ? g0.c : 0.0f;
-        const float G2y = (int_d ? gd.b : 0.0f) + (int_u ? gu.b : 0.0f);
-        const float G3y = (int_d ? gd.c : 0.0f) + (int_u ? gu.c : 0.0f);
-        const float sU = fvna::fma_(ddx2, fvna::fma_(-2.0f, G2c, shr1(G2c) + shl1(G2c)),
-                                    ddy2 * fvna::fma_(-2.0f, G2c, G2y));
-        const float sV = fvna::fma_(ddx2, fvna::fma_(-2.0f, G3c, shr1(G3c) + shl1(G3c)),
-                                    ddy2 * fvna::fma_(-2.0f, G3c, G3y));
-        Vec3 lxy;
-        lxy.a = g0.a - fvna::fma_(sU, xc.u, sV * xc.v) * ir2;
-        lxy.b = fvna::fma_(sU, ir2, g0.b);
-        lxy.c = fvna::fma_(sV, ir2, g0.c);
-
-        // ---- X^T on row m ----
-        // q_xy(k) = q_y(k) - dtxu(k) (PX(k) + MX(k+1))
-        float gPX[3] = {-dtxu * lxy.a, -dtxu * lxy.b, -dtxu * lxy.c};
-        float gMX[3] = {shr1(gPX[0]), shr1(gPX[1]), shr1(gPX[2])};
-        fvad::WaveAdj gx;
-        fvad::zero(gx);
-        float gdx[3];
-        fvad::limiter_bw(fx, limx, dtx, gPX, gMX, gx, gdx);
-        {
-          // family p of interface k dotted with k-1 (L_p > 0) or with k+1
-          fvna::Waves wr;
-          wr.a1 = shl1(fx.w.a1); wr.w21 = shl1(fx.w.w21); wr.w31 = shl1(fx.w.w31);
-          wr.a2 = shl1(fx.w.a2); wr.w22 = shl1(fx.w.w22); wr.w32 = shl1(fx.w.w32);
-          wr.a3 = shl1(fx.w.a3);
-          const float Ls[3] = {fx.L1, fx.L2, fx.L3};
-          float kl[3], kr[3];
-    #pragma unroll
-          for (int p = 0; p < 3; ++p) {
-            const float gdl = Ls[p] > 0.0f ? gdx[p] : 0.0f;
-            const float gdr = gdx[p] - gdl;
-            kl[p] = gdl + shr1(gdr);      // times W(k-1): own left dot, and k-1's right dot
-            kr[p] = gdr + shl1(gdl);      // times W(k+1): own right dot, and k+1's left dot
-          }
-          fvad::add_scaled(gx, kl, wl);
-          fvad::add_scaled(gx, kr, wr);
-        }
-        fvad::SumAdj sx = fvad::riemann_bw(fvna::add_nc(l1, y1q), fvna::add_nc(lun, un),
-                                           fvna::add_nc(lut, ut), y1q - l1, y2q - l2, hbeta, fx, gx);
-        sx = keep_if(col_x, sx);
-        {
-          // cell k: interface k's R part and interface k+1's L part (lane k+1)
-          const float ar1 = (sx.s1 + sx.dr) + shl1(sx.s1 - sx.dr);
-          const float an = sx.dn - shl1(sx.dn);
-          const float at = sx.dt - shl1(sx.dt);
-          const float aun = sx.s2 + shl1(sx.s2);
-          const float aut = sx.s3 + shl1(sx.s3);
-          lyc.a = lxy.a + ar1 - fvna::fma_(aun, un, aut * ut) * ir;
-          lyc.b = lxy.b + fvna::fma_(aun, ir, an);
-          lyc.c = lxy.c + fvna::fma_(aut, ir, at);
-        }
-
-        if (r >= y0 + 1) {
-          // ---- Y^T: limiter of interface m (P -> cell m, M -> cell m-1) ----
           {
-            const float dtyu1 = (m - 1 >= 2 && m - 1 <= ny - 3) ? dty_c : 0.0f;
-            // y component order (rho', normal = rho v, tangential = rho u)
-            float gPy[3] = {-dtyu * lyc.a, -dtyu * lyc.c, -dtyu * lyc.b};
-            float gMy[3] = {-dtyu1 * ly1.a, -dtyu1 * ly1.c, -dtyu1 * ly1.b};
-            float gdy[3];
-            fvad::limiter_bw(f2, lim2, dty, gPy, gMy, acc2, gdy);
-            const float Ls[3] = {f2.L1, f2.L2, f2.L3};
-            float kdn[3], kup[3];
-      #pragma unroll
-            for (int p = 0; p < 3; ++p) {
-              kdn[p] = Ls[p] > 0.0f ? gdy[p] : 0.0f;   // dotted with interface m-1
-              kup[p] = gdy[p] - kdn[p];                // ... with m+1
-            }
-            fvad::add_scaled(acc2, kdn, f3.w);
-            fvad::add_scaled(acc2, kup, f1.w);
-            fvad::add_scaled(acc3, kdn, f2.w);
-            fvad::add_scaled(acc1, kup, f2.w);
+            const Vec3 e1 = shl1(lb), e2 = shl1(e1);
+            if (i == nx - 3 && j >= 2 && j <= ny - 3) a = a + e1 + e2;
           }
-          if (r >= y0 + 3) {
-            // ---- Riemann transpose of interface m-1 = r-3 (rows r-4 | r-3) ----
-            fvad::SumAdj sy = fvad::riemann_bw(fvna::add_nc(b4.q1, b3.q1), fvna::add_nc(b4.un, b3.un),
-                                               fvna::add_nc(b4.ut, b3.ut), b3.q1 - b4.q1, b3.q3 - b4.q3,
-                                               hbeta, f3, acc3);
-            sy = keep_if(col_in && r - 3 >= 1 && r - 3 <= ny - 1, sy);
-            // ---- lam_b of row r-4: lam_y, interface r-4's R part, interface r-3's L part ----
-            Vec3 lb;
-            {
-              const float br1 = (gRp.s1 + gRp.dr) + (sy.s1 - sy.dr);
-              const float bn_ = gRp.dn - sy.dn;
-              const float bt = gRp.dt - sy.dt;
-              const float bun = gRp.s2 + sy.s2;
-              const float but = gRp.s3 + sy.s3;
-              const float ir4 = fvna::rcp(1.0f + b4.q1);
-              lb.a = ly2.a + br1 - fvna::fma_(bun, b4.un, but * b4.ut) * ir4;
-              lb.c = ly2.c + fvna::fma_(bun, ir4, bn_);     // normal = rho v
-              lb.b = ly2.b + fvna::fma_(but, ir4, bt);      // tangential = rho u
-            }
-            gRp = sy;
-
-            // ---- B^T and the drive gradient of row j = r-4 ----
-            const int j = r - 4;
-            if (j >= y0 && j < y1) {
-              const bool interior = col_int && j >= 2 && j <= ny - 3;
-              Vec3 a = sel(interior, lb);
-              bool defer = false;
-              if (BC::kId != kBcTaylor) {
-                // zero-gradient copies (apps/common/boundary.h): left columns <- 2
-                // (zero-gradient app only; the edge drive sets them), right columns
-                // <- nx-3 on rows 2..ny-3, rows 0, 1 <- 2 and ny-2, ny-1 <- ny-3 off
-                // the left columns
-                if (BC::kId == kBcZeroGrad) {
-                  const Vec3 w1 = shr1(lb), w2 = shr1(w1);
-                  if (i == 2) a = a + w1 + w2;
-                }
-                {
-                  const Vec3 e1 = shl1(lb), e2 = shl1(e1);
-                  if (i == nx - 3 && j >= 2 && j <= ny - 3) a = a + e1 + e2;
-                }
-                if (j == 0) hbot = lb;
-                if (j == 1) hbot = hbot + lb;
-                if (j == 2 && i >= 2) a = a + hbot;
-                if (j >= ny - 2) htop = htop + lb;
-                if (j == ny - 3) {
-                  hold = a;
-                  defer = true;
-                }
-              }
-              const uint32_t o = (uint32_t)j * row_bytes + cb;
-              if (out_lane && !defer) {
-                stg(o1, o, a.a);
-                stg(o2, o, a.b);
-                stg(o3, o, a.c);
-              }
-              if (BC::kId != kBcTaylor && j == ny - 1) {
-                const Vec3 t = i >= 2 ? hold + htop : hold;
-                const uint32_t ot = (uint32_t)(ny - 3) * row_bytes + cb;
-                if (out_lane) {
-                  stg(o1, ot, t.a);
-                  stg(o2, ot, t.b);
-                  stg(o3, ot, t.c);
-                }
-              }
-              if (BC::kId == kBcEdgeDrive && it.xo0 == 0) {
-                // left columns hold p = ampc sin(theta) + amps cos(theta), rho u = p (p + 1)
-                const float ec = ((kfloat*)D.edge_c)[j], es = ((kfloat*)D.edge_s)[j];
-                fvbc::Cell<float> dc;
-                fvbc::drive(ec, es, sin_t, cos_t, dc);
-                const float p = dc.q1;
-                const float tp = (i == 0 || i == 1) ? fvna::fma_(lb.b, 2.0f * p + 1.0f, lb.a) : 0.0f;
-                // columns 0 and 1 are lanes kHalo and kHalo + 1 of the first strip
-                const int tb0 = __builtin_amdgcn_readlane(__float_as_int(tp), kHalo);
-                const int tb1 = __builtin_amdgcn_readlane(__float_as_int(tp), kHalo + 1);
-                const double g = (double)__int_as_float(tb0) + (double)__int_as_float(tb1);
-                if (lane == 0) {
-                  D.grad_ec[j] += g * (double)sin_t;
-                  D.grad_es[j] += g * (double)cos_t;
-                }
-              }
-            }
-
+          if (j == 0) hbot = lb;
+          if (j == 1) hbot = hbot + lb;
+          if (j == 2 && i >= 2) a = a + hbot;
+          if (j >= ny - 2) htop = htop + lb;
+          if (j == ny - 3) {
+            hold = a;
+            defer = true;
+          }
+        }
+        const uint32_t o = (uint32_t)j * row_bytes + cb;
+        if (out_lane && !defer) {
+          stg(o1, o, a.a);
+          stg(o2, o, a.b);
+          stg(o3, o, a.c);
+        }
+        if (BC::kId != kBcTaylor && j == ny - 1) {
+          const Vec3 t = i >= 2 ? hold + htop : hold;
+          const uint32_t ot = (uint32_t)(ny - 3) * row_bytes + cb;
+          if (out_lane) {
+            stg(o1, ot, t.a);
+            stg(o2, ot, t.b);
+            stg(o3, ot, t.c);
+          }
+        }
+        if (BC::kId == kBcEdgeDrive && it.xo0 == 0) {
+          // left columns hold p = ampc sin(theta) + amps cos(theta), rho u = p (p + 1)
+          const float ec = ((kfloat*)D.edge_c)[j], es = ((kfloat*)D.edge_s)[j];
+          fvbc::Cell<float> dc;
+          fvbc::drive(ec, es, sin_t, cos_t, dc);
+          const float p = dc.q1;
+          const float tp = (i == 0 || i == 1) ? fvna::fma_(lb.b, 2.0f * p + 1.0f, lb.a) : 0.0f;
+          // columns 0 and 1 are lanes kHalo and kHalo + 1 of the first strip
+          const int tb0 = __builtin_amdgcn_readlane(__float_as_int(tp), kHalo);
+          const int tb1 = __builtin_amdgcn_readlane(__float_as_int(tp), kHalo + 1);
+          const double g = (double)__int_as_float(tb0) + (double)__int_as_float(tb1);
+          if (lane == 0) {
+            D.grad_ec[j] += g * (double)sin_t;
+            D.grad_es[j] += g * (double)cos_t;
           }
         }
       }
+
     }
 
     // ---- rotate the windows ----
