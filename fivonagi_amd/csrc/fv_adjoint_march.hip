@@ -504,32 +504,38 @@ using namespace adjm;
 
 // One reverse step of every instance: step index k of the run (its logged
 // control block), segment slot `slot` (q_n; slot+1 holds q_{n+1}), adjoint
-// ping-pong index par (lam[par] -> lam[par^1]).
+// ping-pong index par (lam[par] -> lam[par^1]).  Wave w runs the items
+// wave_first[w] .. wave_first[w+1] in turn (the end of one strip and the
+// start of the next, when the host balances the waves' rows).
 template <bool kCheck>
 __global__ __launch_bounds__(64) void adj_reverse_kernel(const AdjDesc* __restrict__ ds,
                                                          const AdjItem* __restrict__ items,
-                                                         int n_items, int k, int slot, int par) {
-  // XCD-aware order, as the step kernel: consecutive items of a strip on one XCD
-  int item;
+                                                         const int32_t* __restrict__ wave_first,
+                                                         int n_waves, int k, int slot, int par) {
+  // XCD-aware order, as the step kernel: consecutive waves of a strip on one XCD
+  int w;
   {
     const int b = blockIdx.x;
-    const int q = n_items >> 3, rem = n_items & 7, x = b & 7, kk = b >> 3;
-    item = x * q + min(x, rem) + kk;
+    const int q = n_waves >> 3, rem = n_waves & 7, x = b & 7, kk = b >> 3;
+    w = x * q + min(x, rem) + kk;
   }
-  const AdjItem it = items[item];
-  const AdjDesc& D = ds[it.inst];
-  switch (__builtin_amdgcn_readfirstlane(D.bc)) {
-    case kBcZeroGrad:
-      reverse_item<fvbc::ZeroGradient, kCheck>(D, it, item, k, slot, par);
-      break;
-    case kBcEdgeDrive:
-      reverse_item<fvbc::EdgeDrive, kCheck>(D, it, item, k, slot, par);
-      break;
-    case kBcTaylor:
-      reverse_item<fvbc::Taylor, kCheck>(D, it, item, k, slot, par);
-      break;
-    default:
-      break;
+  const int i0 = wave_first[w], i1 = wave_first[w + 1];
+  for (int item = i0; item < i1; ++item) {
+    const AdjItem it = items[item];
+    const AdjDesc& D = ds[it.inst];
+    switch (__builtin_amdgcn_readfirstlane(D.bc)) {
+      case kBcZeroGrad:
+        reverse_item<fvbc::ZeroGradient, kCheck>(D, it, item, k, slot, par);
+        break;
+      case kBcEdgeDrive:
+        reverse_item<fvbc::EdgeDrive, kCheck>(D, it, item, k, slot, par);
+        break;
+      case kBcTaylor:
+        reverse_item<fvbc::Taylor, kCheck>(D, it, item, k, slot, par);
+        break;
+      default:
+        break;
+    }
   }
 }
 
@@ -538,15 +544,15 @@ __global__ __launch_bounds__(64) void adj_reverse_kernel(const AdjDesc* __restri
 // ---------------------------------------------------------------- launches --
 
 extern "C" hipError_t fv_adj_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items,
-                                     int n_items, int k, int slot, int par, int check,
-                                     hipStream_t s) {
-  if (n_items <= 0) return hipSuccess;
+                                     const int32_t* wave_first, int n_waves, int k, int slot,
+                                     int par, int check, hipStream_t s) {
+  if (n_waves <= 0) return hipSuccess;
   if (check)
-    hipLaunchKernelGGL(fv::adj_reverse_kernel<true>, dim3(n_items), dim3(64), 0, s, ds, items,
-                       n_items, k, slot, par);
+    hipLaunchKernelGGL(fv::adj_reverse_kernel<true>, dim3(n_waves), dim3(64), 0, s, ds, items,
+                       wave_first, n_waves, k, slot, par);
   else
-    hipLaunchKernelGGL(fv::adj_reverse_kernel<false>, dim3(n_items), dim3(64), 0, s, ds, items,
-                       n_items, k, slot, par);
+    hipLaunchKernelGGL(fv::adj_reverse_kernel<false>, dim3(n_waves), dim3(64), 0, s, ds, items,
+                       wave_first, n_waves, k, slot, par);
   return hipGetLastError();
 }
 

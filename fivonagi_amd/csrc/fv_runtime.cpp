@@ -32,8 +32,8 @@ extern "C" hipError_t fv_launch_collect(const fv::InstDesc* insts, int n_inst, i
                                         int phase, int fp64, hipStream_t stream);
 extern "C" hipError_t fv_step_waves_per_cu(int* waves, int fp64);
 extern "C" hipError_t fv_adj_reverse(const fv::AdjDesc* ds, const fv::AdjItem* items,
-                                     int n_items, int k, int slot, int par, int check,
-                                     hipStream_t s);
+                                     const int32_t* wave_first, int n_waves, int k, int slot,
+                                     int par, int check, hipStream_t s);
 extern "C" hipError_t fv_adj_occupancy(int* waves_per_cu);
 extern "C" hipError_t fv_launch_taylor_ghosts(const fv::InstDesc* insts, int n_inst,
                                               int max_ring, int phase, hipStream_t stream);
@@ -1024,6 +1024,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   // split so that every wave of a step is resident at once ----
   std::vector<fv::AdjItem> items;
   std::vector<size_t> first_item;
+  std::vector<int32_t> wave_first;
   {
     int occ = 0, cus = 0;
     if (fv_adj_occupancy(&occ) != hipSuccess || occ <= 0) occ = 2;
@@ -1032,7 +1033,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
         cus <= 0)
       cus = 256;
     fvh::build_adj_items(c->params, (long)cus * occ, opts ? opts->rows_per_wave : 0, items,
-                         first_item);
+                         first_item, wave_first);
   }
 
   // ---- device buffers of the call ----
@@ -1043,6 +1044,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   double* jpart = nullptr;
   uint32_t* d_mis = nullptr;
   fv::AdjItem* d_items = nullptr;
+  int32_t* d_wave_first = nullptr;
   fv::AdjDesc* d_ad = nullptr;
   fv::InstDesc* d_rdesc = nullptr;    // the step kernel's descriptors and control
   fv::Ctl* d_rctl = nullptr;          // blocks for the segment recompute: slot s of
@@ -1068,6 +1070,7 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
       cps[k] = keep_all ? nullptr : A.get<float>((size_t)ncp * 3 * n);
     }
     d_items = A.get<fv::AdjItem>(items.size());
+    d_wave_first = A.get<int32_t>(wave_first.size());
     d_ad = A.get<fv::AdjDesc>(ni);
     d_rdesc = A.get<fv::InstDesc>((size_t)K * ni);
     d_rctl = A.get<fv::Ctl>((size_t)K * ni * 2);
@@ -1114,6 +1117,8 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   HIP_TRY(hipMemsetAsync(d_mis, 0, sizeof(uint32_t), c->stream));
   HIP_TRY(hipMemsetAsync(d_rsync, 0, sizeof(fv::SyncSlots) * 3 * ni, c->stream));
   HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(fv::AdjItem) * items.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(d_wave_first, wave_first.data(), sizeof(int32_t) * wave_first.size(),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(d_ad, ad.data(), sizeof(fv::AdjDesc) * ni, hipMemcpyHostToDevice,
                          c->stream));
@@ -1260,8 +1265,8 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
       if (c->taylor_ring > 0)
         HIP_TRY(fv_launch_taylor_ghosts(d_rdesc + (size_t)s * ni, ni, c->taylor_ring, 0,
                                         c->stream));
-      HIP_TRY(fv_adj_reverse(d_ad, d_items, (int)items.size(), (int)(s0 + s),
-                             (int)(slot0 + s), par, check ? 1 : 0, c->stream));
+      HIP_TRY(fv_adj_reverse(d_ad, d_items, d_wave_first, (int)wave_first.size() - 1,
+                             (int)(s0 + s), (int)(slot0 + s), par, check ? 1 : 0, c->stream));
       par ^= 1;
     }
   }

@@ -282,7 +282,7 @@ void build_work(const std::vector<fivonagi_params>& ps, const std::vector<int>& 
 
 void build_adj_items(const std::vector<fivonagi_params>& ps, long target_waves,
                      int rows_per_wave, std::vector<fv::AdjItem>& items,
-                     std::vector<size_t>& first) {
+                     std::vector<size_t>& first, std::vector<int32_t>& wave_first) {
   constexpr int kCols = 56;   // output columns per strip (a 4-column halo each side)
   // strip starts; the last one never falls on nx-2 or nx-1 (column nx-3's
   // wave also holds nx-2 and nx-1)
@@ -293,33 +293,76 @@ void build_adj_items(const std::vector<fivonagi_params>& ps, long target_waves,
     b.push_back(nx);
     return b;
   };
-  int H = rows_per_wave > 0 ? std::max(8, rows_per_wave) : 8;
-  for (; H < 4096 && rows_per_wave <= 0; ++H) {
-    long waves = 0;
-    for (const auto& p : ps)
-      waves += (long)(strips(p.nx).size() - 1) * std::max(1L, std::lround((double)p.ny / H));
-    if (waves <= target_waves) break;
-  }
+  // a row cut never leaves row 2 apart from rows 0, 1, nor row ny-3 apart
+  // from ny-2, ny-1 (a ghost copy and its source in one item)
+  auto legal_cut = [](int cut, int ny) {
+    if (cut == 1 || cut == 2) return 3;
+    if (cut == ny - 2 || cut == ny - 1) return ny;
+    return cut;
+  };
   items.clear();
+  wave_first.assign(1, 0);
   first.assign(ps.size() + 1, 0);
-  for (size_t k = 0; k < ps.size(); ++k) {
-    const auto& p = ps[k];
-    const int chunks = std::max(1, (int)std::lround((double)p.ny / H));
-    std::vector<int> rb(chunks + 1);
-    for (int ch = 0; ch <= chunks; ++ch) rb[ch] = (int)((long)p.ny * ch / chunks);
-    // row 2 stays with rows 0, 1 and row ny-3 with ny-2, ny-1
-    for (int ch = 1; ch < chunks; ++ch) {
-      if (rb[ch] <= 2) rb[ch] = 3;
-      if (rb[ch] >= p.ny - 2) rb[ch] = p.ny - 3;
+  if (rows_per_wave > 0) {
+    // fixed heights: rows_per_wave rows per item, one item per wave
+    const int H = std::max(8, rows_per_wave);
+    for (size_t k = 0; k < ps.size(); ++k) {
+      const auto& p = ps[k];
+      const std::vector<int> sb = strips(p.nx);
+      first[k] = items.size();
+      for (size_t s = 0; s + 1 < sb.size(); ++s)
+        for (int y = 0; y < p.ny;) {
+          const int y1 = std::min(p.ny, legal_cut(std::min(p.ny, y + H), p.ny));
+          items.push_back({(int32_t)k, sb[s], sb[s], sb[s + 1], y, y1});
+          wave_first.push_back((int32_t)items.size());
+          y = y1;
+        }
     }
-    const std::vector<int> sb = strips(p.nx);
-    first[k] = items.size();
-    for (size_t s = 0; s + 1 < sb.size(); ++s)
-      for (int ch = 0; ch < chunks; ++ch)
-        if (rb[ch + 1] > rb[ch])
-          items.push_back({(int32_t)k, sb[s], sb[s], sb[s + 1], rb[ch], rb[ch + 1]});
+    first[ps.size()] = items.size();
+    return;
   }
-  first[ps.size()] = items.size();
+  // balanced: the strips of every instance, one after another, are cut into
+  // waves of about equal cost, an item costing its rows plus kWarm for the
+  // rows the march starts with before the first it emits; a wave may end one
+  // strip and start the next (two items).  The least cost per wave whose
+  // waves fit target_waves is found by bisection.
+  constexpr int kWarm = 4, kMinRows = 8;
+  auto pack = [&](long T, bool emit) {
+    long waves = 0, left = 0;
+    for (size_t k = 0; k < ps.size(); ++k) {
+      const auto& p = ps[k];
+      const std::vector<int> sb = strips(p.nx);
+      if (emit) first[k] = items.size();
+      for (size_t s = 0; s + 1 < sb.size(); ++s)
+        for (int y = 0; y < p.ny;) {
+          if (left - kWarm < kMinRows) {       // start a wave
+            if (emit && waves > 0) wave_first.push_back((int32_t)items.size());
+            ++waves;
+            left = T;
+          }
+          const long take = std::min<long>(p.ny - y, left - kWarm);
+          const int y1 = std::min(p.ny, legal_cut(y + (int)take, p.ny));
+          if (emit) items.push_back({(int32_t)k, sb[s], sb[s], sb[s + 1], y, y1});
+          left -= (y1 - y) + kWarm;
+          y = y1;
+        }
+    }
+    if (emit) {
+      first[ps.size()] = items.size();
+      wave_first.push_back((int32_t)items.size());
+    }
+    return waves;
+  };
+  long total = 0;
+  for (const auto& p : ps) total += (long)(strips(p.nx).size() - 1) * (p.ny + kWarm);
+  long lo = kWarm + kMinRows, hi = std::max(lo, total + kWarm + kMinRows);
+  const long target = std::max(1L, target_waves);
+  while (lo < hi) {                          // the least T with pack(T) <= target
+    const long mid = lo + (hi - lo) / 2;
+    if (pack(mid, false) <= target) hi = mid;
+    else lo = mid + 1;
+  }
+  pack(lo, true);
 }
 
 }  // namespace fvh

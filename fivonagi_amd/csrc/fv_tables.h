@@ -73,12 +73,15 @@ void build_work(const std::vector<fivonagi_params>& ps, const std::vector<int>& 
                 int auto_h, float skew, bool rotate, WorkList& out);
 
 // Work items of the adjoint's reverse march (fv_adjoint_march.hip): strips
-// of 56 output columns and row chunks of rows_per_wave rows, or (0) the
-// fewest rows whose waves fit target_waves; first[k] .. first[k+1] are
-// instance k's items.
+// of 56 output columns cut into row ranges, items in instance and strip
+// order; first[k] .. first[k+1] are instance k's items, and wave w runs items
+// wave_first[w] .. wave_first[w+1] in turn.  rows_per_wave > 0: items of that
+// many rows, one per wave; 0: waves of about equal cost, at most target_waves
+// of them, a wave taking one or more items (the end of one strip and the
+// start of the next).
 void build_adj_items(const std::vector<fivonagi_params>& ps, long target_waves,
                      int rows_per_wave, std::vector<fv::AdjItem>& items,
-                     std::vector<size_t>& first);
+                     std::vector<size_t>& first, std::vector<int32_t>& wave_first);
 
 }  // namespace fvh
 

@@ -188,8 +188,10 @@ typedef struct {
                                 q_{n+1} and compares it bit for bit with the
                                 stored state; any difference returns
                                 FIVONAGI_ERR_CHECK                            */
-  int32_t rows_per_wave;     /* rows per wave of the reverse march (0 = the
-                                least that starts every wave at once)         */
+  int32_t rows_per_wave;     /* rows per work item of the reverse march, one
+                                item per wave; 0 = waves of about equal cost,
+                                as many as start at once, a wave taking the
+                                end of one strip and the start of the next    */
 } fivonagi_adjoint_opts;
 
 /* Advance every instance nsteps accepted steps (as fivonagi_step, recording

@@ -157,8 +157,18 @@ void check_work(const std::vector<fivonagi_params>& ps, long cus, int per_cu, in
 void check_adj(const std::vector<fivonagi_params>& ps, long target, int rows) {
   std::vector<fv::AdjItem> items;
   std::vector<size_t> first;
-  fvh::build_adj_items(ps, target, rows, items, first);
+  std::vector<int32_t> wave_first;
+  fvh::build_adj_items(ps, target, rows, items, first, wave_first);
   CHECK(first.size() == ps.size() + 1 && first.back() == items.size(), "first indices");
+  // waves: every item in exactly one, in order, none empty; at most target of
+  // them when the heights are automatic
+  CHECK(wave_first.size() >= 2 && wave_first.front() == 0 &&
+            wave_first.back() == (int32_t)items.size(), "wave ranges");
+  for (size_t w = 0; w + 1 < wave_first.size(); ++w)
+    CHECK(wave_first[w + 1] > wave_first[w], "empty wave %zu", w);
+  if (rows == 0)
+    CHECK((long)wave_first.size() - 1 <= std::max(1L, target), "%zu waves for %ld",
+          wave_first.size() - 1, target);
   for (size_t k = 0; k < ps.size(); ++k) {
     const auto& p = ps[k];
     std::vector<int> cover((size_t)p.nx * p.ny, 0);
