@@ -23,6 +23,8 @@
 //                   the limiters of m-2, m-1 and m are done) -> lam_b(m-2)
 //   row m-2:        B^T -> lam_n(m-2)
 // x neighbours come from DPP lane shifts, y neighbours from register windows.
+// A wave runs one or more work items (row ranges of a strip) in turn; the
+// host groups them into waves of about equal cost.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
