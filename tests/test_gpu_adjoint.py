@@ -296,3 +296,27 @@ def test_repeated_calls_reuse_the_context_block():
         assert np.array_equal(ga, gar[0]) and np.array_equal(gp, gpr[0]), (steps, K)
         assert all(np.array_equal(a, b) for a, b in zip(lam, lamr)), (steps, K)
         before += steps
+
+
+def test_waves_of_several_items_match_one_item_per_wave():
+    """At 2560x2048 the automatic reverse split gives waves of about equal
+    cost, some of which end one strip and start the next (two items in one
+    wave: 30 of the ~2000 waves on a 256-CU device); a fixed height of 24 rows
+    gives one item per wave.  Both runs of 12 steps, with the recompute check,
+    give the same gradients and dJ/dq0 bit for bit: the grouping of items
+    into waves changes nothing.  J is summed per item, so its rounding follows
+    the item split (within 1e-14)."""
+    from fivonagi_amd import workloads as wl
+    prob, _, _ = wl.adjoint_single(nx=2560, ny=2048)
+    box = (2, 2048 // 2 - 32, 64, 64)          # at the drive, so J > 0 after 12 steps
+    n = 12
+    res = []
+    for rows in (0, 24):
+        with fa.Context(prob) as ctx:
+            ctx.set_dt_sequence(0, np.full(n + 3, 0.04))
+            J, ga, gp = ctx.step_adjoint(n, box, check_recompute=True, rows_per_wave=rows)
+            res.append((J[0], ga[0], gp[0], ctx.adjoint_state(0)))
+    assert res[0][0] > 0 and abs(res[0][0] - res[1][0]) <= 1e-14 * res[0][0]
+    assert np.abs(res[0][1]).max() > 0
+    assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+    assert np.array_equal(res[0][3], res[1][3])
