@@ -166,9 +166,24 @@ void check_adj(const std::vector<fivonagi_params>& ps, long target, int rows) {
             wave_first.back() == (int32_t)items.size(), "wave ranges");
   for (size_t w = 0; w + 1 < wave_first.size(); ++w)
     CHECK(wave_first[w + 1] > wave_first[w], "empty wave %zu", w);
-  if (rows == 0)
+  if (rows == 0) {
     CHECK((long)wave_first.size() - 1 <= std::max(1L, target), "%zu waves for %ld",
           wave_first.size() - 1, target);
+    // balance: no wave's cost (rows + 4 per item) above the mean of the
+    // waves' costs plus one item's warm-up, one minimal item and the two rows
+    // a ghost-row cut can add
+    long total = 0, worst = 0;
+    for (size_t w = 0; w + 1 < wave_first.size(); ++w) {
+      long cost = 0;
+      for (int i = wave_first[w]; i < wave_first[w + 1]; ++i)
+        cost += (items[i].y1 - items[i].y0) + 4;
+      total += cost;
+      worst = std::max(worst, cost);
+    }
+    const long waves = (long)wave_first.size() - 1;
+    CHECK(worst <= (total + waves - 1) / waves + 4 + 8 + 2 + 4, "wave cost %ld, mean %ld", worst,
+          total / waves);
+  }
   for (size_t k = 0; k < ps.size(); ++k) {
     const auto& p = ps[k];
     std::vector<int> cover((size_t)p.nx * p.ny, 0);
