@@ -1302,13 +1302,18 @@ int fivonagi_step_adjoint(fivonagi_ctx* c, int64_t nsteps, const fivonagi_object
   if (check && mism != 0)
     return fail(FIVONAGI_ERR_CHECK, "adjoint: " + std::to_string(mism) +
                                          " recomputed cells differ from the forward's");
-  // keep dJ/dq0 for fivonagi_adjoint_state
-  for (float* q : c->adj_lam) (void)hipFree(q);
-  c->adj_lam.assign(ni, nullptr);
+  // keep dJ/dq0 for fivonagi_adjoint_state (the grids of the first call are
+  // reused: an instance's size does not change)
+  if (c->adj_lam.size() != (size_t)ni) c->adj_lam.assign(ni, nullptr);
   for (int k = 0; k < ni; ++k) {
     const size_t n = (size_t)c->params[k].nx * c->params[k].ny;
-    hipError_t e = hipMalloc(&c->adj_lam[k], sizeof(float) * 3 * n);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint state)");
+    if (!c->adj_lam[k]) {
+      hipError_t e = hipMalloc(&c->adj_lam[k], sizeof(float) * 3 * n);
+      if (e != hipSuccess) {
+        c->adj_lam[k] = nullptr;
+        return hip_fail(e, "hipMalloc (adjoint state)");
+      }
+    }
     HIP_TRY(hipMemcpyAsync(c->adj_lam[k], ad[k].lam[par], sizeof(float) * 3 * n,
                            hipMemcpyDeviceToDevice, c->stream));
   }
